@@ -1,0 +1,814 @@
+// ft_build.hip -- gfx950 kernels of the HLBVH build (build_bvh, src/hlbvh.hpp:2165-2302):
+//   bbox -> SFC encode -> stable LSD radix sort (key u64 + index u32) -> leaf gather
+//   -> Karras topology -> bottom-up refit.
+// Every stage is stream-ordered; nothing synchronises with the host.
+#include "ft_internal.hpp"
+
+#include <algorithm>
+#include <vector>
+
+namespace ft {
+
+// ===========================================================================
+// Peano-Hilbert state machine (the curve of src/sfc.peano_hilbert.hpp:332-376):
+// entry [state*8 + pix] = next_state << 3 | 3-bit digit.
+// ===========================================================================
+__constant__ uint16_t c_ph_table[384] = {
+    0x120, 0x0e7, 0x0c9, 0x0de, 0x053, 0x054, 0x0ca, 0x0dd, 0x0ef, 0x05c, 0x0c6, 0x0c5, 0x128, 0x05b, 0x0d1, 0x0d2, 0x044, 0x043, 0x0cd, 0x0da, 0x0f7, 0x130, 0x0ce, 0x0d9, 0x04b, 0x138, 0x0c2, 0x0c1, 0x04c, 0x0ff, 0x0d5, 0x0d6,
+    0x141, 0x0c0, 0x166, 0x107, 0x142, 0x033, 0x165, 0x034, 0x0c8, 0x03b, 0x10f, 0x03c, 0x149, 0x14a, 0x16e, 0x16d, 0x023, 0x152, 0x024, 0x175, 0x0d0, 0x151, 0x117, 0x176, 0x15a, 0x159, 0x17d, 0x17e, 0x02b, 0x0d8, 0x02c, 0x11f,
+    0x10e, 0x119, 0x127, 0x0e0, 0x10d, 0x11a, 0x014, 0x013, 0x101, 0x102, 0x0e8, 0x01b, 0x116, 0x115, 0x12f, 0x01c, 0x10a, 0x11d, 0x003, 0x004, 0x109, 0x11e, 0x0f0, 0x137, 0x105, 0x106, 0x00c, 0x13f, 0x112, 0x111, 0x00b, 0x0f8,
+    0x0c7, 0x156, 0x100, 0x171, 0x074, 0x155, 0x073, 0x172, 0x15e, 0x15d, 0x179, 0x17a, 0x0cf, 0x07c, 0x108, 0x07b, 0x145, 0x064, 0x162, 0x063, 0x146, 0x0d7, 0x161, 0x110, 0x06c, 0x0df, 0x06b, 0x118, 0x14d, 0x14e, 0x16a, 0x169,
+    0x0e6, 0x14f, 0x0e5, 0x0b4, 0x131, 0x158, 0x132, 0x0b3, 0x157, 0x140, 0x0bc, 0x0bb, 0x0ee, 0x139, 0x0ed, 0x13a, 0x148, 0x121, 0x0a3, 0x122, 0x15f, 0x0f6, 0x0a4, 0x0f5, 0x129, 0x0fe, 0x12a, 0x0fd, 0x150, 0x147, 0x0ab, 0x0ac,
+    0x0e2, 0x093, 0x0e1, 0x168, 0x135, 0x094, 0x136, 0x17f, 0x09b, 0x09c, 0x170, 0x167, 0x0ea, 0x13d, 0x0e9, 0x13e, 0x084, 0x125, 0x16f, 0x126, 0x083, 0x0f2, 0x178, 0x0f1, 0x12d, 0x0fa, 0x12e, 0x0f9, 0x08c, 0x08b, 0x177, 0x160,
+    0x067, 0x020, 0x00e, 0x019, 0x114, 0x113, 0x00d, 0x01a, 0x028, 0x11b, 0x001, 0x002, 0x06f, 0x11c, 0x016, 0x015, 0x103, 0x104, 0x00a, 0x01d, 0x030, 0x077, 0x009, 0x01e, 0x10c, 0x07f, 0x005, 0x006, 0x10b, 0x038, 0x012, 0x011,
+    0x086, 0x007, 0x0a1, 0x040, 0x085, 0x0f4, 0x0a2, 0x0f3, 0x00f, 0x0fc, 0x048, 0x0fb, 0x08e, 0x08d, 0x0a9, 0x0aa, 0x0e4, 0x095, 0x0e3, 0x0b2, 0x017, 0x096, 0x050, 0x0b1, 0x09d, 0x09e, 0x0ba, 0x0b9, 0x0ec, 0x01f, 0x0eb, 0x058,
+    0x049, 0x05e, 0x060, 0x027, 0x04a, 0x05d, 0x0d3, 0x0d4, 0x046, 0x045, 0x02f, 0x0dc, 0x051, 0x052, 0x068, 0x0db, 0x04d, 0x05a, 0x0c4, 0x0c3, 0x04e, 0x059, 0x037, 0x070, 0x042, 0x041, 0x0cb, 0x078, 0x055, 0x056, 0x0cc, 0x03f,
+    0x000, 0x091, 0x047, 0x0b6, 0x133, 0x092, 0x134, 0x0b5, 0x099, 0x09a, 0x0be, 0x0bd, 0x008, 0x13b, 0x04f, 0x13c, 0x082, 0x123, 0x0a5, 0x124, 0x081, 0x010, 0x0a6, 0x057, 0x12b, 0x018, 0x12c, 0x05f, 0x08a, 0x089, 0x0ad, 0x0ae,
+    0x021, 0x088, 0x022, 0x173, 0x076, 0x09f, 0x075, 0x174, 0x090, 0x087, 0x17b, 0x17c, 0x029, 0x07e, 0x02a, 0x07d, 0x08f, 0x066, 0x164, 0x065, 0x098, 0x031, 0x163, 0x032, 0x06e, 0x039, 0x06d, 0x03a, 0x097, 0x080, 0x16c, 0x16b,
+    0x025, 0x154, 0x026, 0x0af, 0x072, 0x153, 0x071, 0x0b8, 0x15c, 0x15b, 0x0b7, 0x0a0, 0x02d, 0x07a, 0x02e, 0x079, 0x143, 0x062, 0x0a8, 0x061, 0x144, 0x035, 0x0bf, 0x036, 0x06a, 0x03d, 0x069, 0x03e, 0x14b, 0x14c, 0x0b0, 0x0a7,
+};
+
+// Morton bit spread: bit b of a 21-bit value moves to bit 3b (spread3_u64,
+// src/sfc.morton.hpp:33-41), by the usual shift-and-mask cascade.
+__device__ __host__ inline uint64_t spread3_21(uint64_t v) {
+    v &= 0x1FFFFFull;
+    v = (v | (v << 32)) & 0x001F00000000FFFFull;
+    v = (v | (v << 16)) & 0x001F0000FF0000FFull;
+    v = (v | (v << 8)) & 0x100F00F00F00F00Full;
+    v = (v | (v << 4)) & 0x10C30C30C30C30C3ull;
+    v = (v | (v << 2)) & 0x1249249249249249ull;
+    return v;
+}
+
+template <typename T> struct BboxDev {
+    T lo[3];
+    T hi[3];
+    T inv[3];
+};
+
+// ===========================================================================
+// K1: bounding box (compute_bbox, src/hlbvh.hpp:462-578): exact min / max.
+// ===========================================================================
+template <typename T> __device__ inline T tmin(T a, T b) { return b < a ? b : a; }
+template <typename T> __device__ inline T tmax(T a, T b) { return a < b ? b : a; }
+
+template <typename T> __device__ inline T wave_min(T v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = tmin(v, __shfl_xor(v, o, 64));
+    return v;
+}
+template <typename T> __device__ inline T wave_max(T v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = tmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void bbox_partial_kernel(const T *__restrict__ x,
+                                                           const T *__restrict__ y,
+                                                           const T *__restrict__ z, size_t n,
+                                                           T *__restrict__ partials) {
+    T v[6];
+    {
+        T f = x[0], g = y[0], h = z[0];
+        v[0] = f; v[1] = f; v[2] = g; v[3] = g; v[4] = h; v[5] = h;
+    }
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        T a = x[i], b = y[i], c = z[i];
+        v[0] = tmin(v[0], a); v[1] = tmax(v[1], a);
+        v[2] = tmin(v[2], b); v[3] = tmax(v[3], b);
+        v[4] = tmin(v[4], c); v[5] = tmax(v[5], c);
+    }
+    __shared__ T red[4][6];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int a = 0; a < 6; ++a) {
+        T r = (a & 1) ? wave_max(v[a]) : wave_min(v[a]);
+        if (lane == 0) red[w][a] = r;
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int a = threadIdx.x;
+        T r = red[0][a];
+        for (int q = 1; q < 4; ++q) r = (a & 1) ? tmax(r, red[q][a]) : tmin(r, red[q][a]);
+        partials[blockIdx.x * 6 + a] = r;
+    }
+}
+
+// Reduces the partials (or takes a caller-supplied box) and stores lo/hi/inv.
+// inv = 0 for a flat axis, else 1/extent in T (src/hlbvh.hpp:392-394).
+template <typename T>
+__global__ __launch_bounds__(64) void bbox_final_kernel(const T *__restrict__ partials, int np,
+                                                        int use_given, T g0, T g1, T g2, T g3,
+                                                        T g4, T g5, BboxDev<T> *out) {
+    if (threadIdx.x != 0) return;
+    T v[6];
+    if (use_given) {
+        v[0] = g0; v[1] = g1; v[2] = g2; v[3] = g3; v[4] = g4; v[5] = g5;
+    } else {
+        for (int a = 0; a < 6; ++a) v[a] = partials[a];
+        for (int b = 1; b < np; ++b)
+            for (int a = 0; a < 6; ++a)
+                v[a] = (a & 1) ? tmax(v[a], partials[b * 6 + a]) : tmin(v[a], partials[b * 6 + a]);
+    }
+    for (int a = 0; a < 3; ++a) {
+        out->lo[a] = v[2 * a];
+        out->hi[a] = v[2 * a + 1];
+        if constexpr (CoordTraits<T>::fp) {
+            T d = v[2 * a + 1] - v[2 * a];
+            out->inv[a] = (d == T(0)) ? T(0) : (T(1) / d);
+        } else {
+            out->inv[a] = T(0);
+        }
+    }
+}
+
+// ===========================================================================
+// K3/K4: SFC encode (sfc_encode, src/hlbvh.hpp:357-423).  One particle per lane,
+// coalesced SoA reads, PH state table staged in LDS.
+// ===========================================================================
+template <typename T> __device__ inline uint32_t quantise(T nx, int B, int sfc);
+
+// Morton FP: top B mantissa bits of fl(1 + nx) in T (encode_to_sfc1d,
+// src/hlbvh.hpp:246-271); PH: of min(1 + (double)nx, 0x1.fffffffffffffp+0)
+// (quantize_coord, src/sfc.peano_hilbert.hpp:472-476).
+template <> __device__ inline uint32_t quantise<double>(double nx, int B, int sfc) {
+    if (sfc == FT_SFC_MORTON) {
+        double v = 1.0 + nx;
+        uint64_t bits = (uint64_t)__double_as_longlong(v);
+        return (uint32_t)((bits & 0x000FFFFFFFFFFFFFull) >> (52 - B));
+    }
+    double v = fmin(1.0 + nx, 0x1.fffffffffffffp+0);
+    uint64_t bits = (uint64_t)__double_as_longlong(v);
+    return (uint32_t)(((bits & 0x000FFFFFFFFFFFFFull) >> (52 - B)) & ((1ull << B) - 1));
+}
+template <> __device__ inline uint32_t quantise<float>(float nx, int B, int sfc) {
+    if (sfc == FT_SFC_MORTON) {
+        float v = 1.0f + nx;
+        uint32_t bits = (uint32_t)__float_as_uint(v);
+        return (bits & 0x007FFFFFu) >> (23 - B);
+    }
+    double v = fmin(1.0 + (double)nx, 0x1.fffffffffffffp+0);
+    uint64_t bits = (uint64_t)__double_as_longlong(v);
+    return (uint32_t)(((bits & 0x000FFFFFFFFFFFFFull) >> (52 - B)) & ((1ull << B) - 1));
+}
+
+__device__ inline uint64_t ph_encode(uint32_t ix, uint32_t iy, uint32_t iz, int B,
+                                     const uint16_t *tab) {
+    uint64_t key = 0;
+    uint32_t state = 0;
+    for (int bit = B - 1; bit >= 0; --bit) {
+        uint32_t pix = (((ix >> bit) & 1u) << 2) | (((iy >> bit) & 1u) << 1) | ((iz >> bit) & 1u);
+        uint32_t e = tab[state * 8 + pix];
+        key = (key << 3) | (e & 7u);
+        state = e >> 3;
+    }
+    return key;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void encode_kernel(const T *__restrict__ x,
+                                                     const T *__restrict__ y,
+                                                     const T *__restrict__ z, size_t n, int B,
+                                                     int sfc, const BboxDev<T> *__restrict__ bb,
+                                                     uint64_t *__restrict__ keys) {
+    __shared__ uint16_t tab[384];
+    for (int i = threadIdx.x; i < 384; i += blockDim.x) tab[i] = c_ph_table[i];
+    __syncthreads();
+    T lo[3] = {T(0), T(0), T(0)}, inv[3] = {T(0), T(0), T(0)};
+    if constexpr (CoordTraits<T>::fp) {
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = bb->lo[a];
+            inv[a] = bb->inv[a];
+        }
+    }
+    const T clamp_hi = CoordTraits<T>::fp ? (T)(sizeof(T) == 8 ? 0x1.fffffffffffffp-1 : 0x1.fffffep-1) : T(0);
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const T p[3] = {x[i], y[i], z[i]};
+        uint32_t q[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            if constexpr (CoordTraits<T>::fp) {
+                T t = (p[a] - lo[a]) * inv[a];
+                T nx = ffmin(ffmax(t, T(0)), clamp_hi);  // sycl::clamp
+                q[a] = quantise<T>(nx, B, sfc);
+            } else {
+                q[a] = (uint32_t)((uint64_t)p[a] & ((1ull << B) - 1));  // x & (2^B-1)
+            }
+        }
+        uint64_t key;
+        if (sfc == FT_SFC_MORTON) key = spread3_21(q[0]) | (spread3_21(q[1]) << 1) | (spread3_21(q[2]) << 2);
+        else key = ph_encode(q[0], q[1], q[2], B, tab);
+        keys[i] = key;
+    }
+}
+
+// ===========================================================================
+// K7: stable LSD radix sort of (key u64, index u32), 8-bit digits, one sweep per
+// digit with decoupled look-back (tile prefixes published per digit, read back
+// by later tiles), wave-level multisplit ranking (8 ballots per item) and an
+// LDS-staged scatter so each digit's run is written contiguously.
+// Equal keys keep their input order (stable).
+// ===========================================================================
+constexpr int RS_BLOCK = 256;
+constexpr int RS_WAVES = RS_BLOCK / 64;
+constexpr int RS_ITEMS = 16;
+constexpr int RS_TILE = RS_BLOCK * RS_ITEMS;
+constexpr int RS_MAX_PASSES = 8;
+
+__global__ void fill_u32_kernel(uint32_t *p, uint32_t v, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// Global digit histograms of every pass in one read of the keys.
+__global__ __launch_bounds__(256) void radix_hist_kernel(const uint64_t *__restrict__ keys,
+                                                         size_t n, int passes,
+                                                         uint32_t *__restrict__ hist) {
+    __shared__ uint32_t h[RS_MAX_PASSES][256];
+    for (int i = threadIdx.x; i < RS_MAX_PASSES * 256; i += blockDim.x) (&h[0][0])[i] = 0;
+    __syncthreads();
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        uint64_t k = keys[i];
+        for (int p = 0; p < passes; ++p) atomicAdd(&h[p][(k >> (8 * p)) & 255u], 1u);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < passes * 256; i += blockDim.x) {
+        uint32_t v = (&h[0][0])[i];
+        if (v) atomicAdd(&hist[i], v);
+    }
+}
+
+__device__ inline uint32_t block_excl_scan256(uint32_t v, uint32_t *s_tmp /*>= 4*/) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) s_tmp[w] = x;
+    __syncthreads();
+    uint32_t add = 0;
+    for (int q = 0; q < w; ++q) add += s_tmp[q];
+    __syncthreads();
+    return x - v + add;
+}
+
+// Exclusive scan of each pass's histogram -> digit base offsets; resets the
+// per-pass tile counters.
+__global__ __launch_bounds__(256) void radix_base_kernel(const uint32_t *__restrict__ hist,
+                                                         int passes, uint32_t *__restrict__ base,
+                                                         uint32_t *__restrict__ tile_ctr) {
+    __shared__ uint32_t tmp[4];
+    for (int p = 0; p < passes; ++p) {
+        uint32_t v = hist[p * 256 + threadIdx.x];
+        base[p * 256 + threadIdx.x] = block_excl_scan256(v, tmp);
+    }
+    if (threadIdx.x < RS_MAX_PASSES) tile_ctr[threadIdx.x] = 0;
+}
+
+__device__ inline uint64_t lanemask_lt() {
+    const int lane = threadIdx.x & 63;
+    return lane ? (~0ull >> (64 - lane)) : 0ull;
+}
+
+template <bool FIRST>
+__global__ __launch_bounds__(RS_BLOCK) void onesweep_kernel(
+    const uint64_t *__restrict__ keys_in, const uint32_t *__restrict__ vals_in,
+    uint64_t *__restrict__ keys_out, uint32_t *__restrict__ vals_out, int n, int shift,
+    const uint32_t *__restrict__ digit_base, uint64_t *__restrict__ lookback, uint32_t epoch,
+    uint32_t *__restrict__ tile_ctr) {
+    __shared__ uint64_t s_keys[RS_TILE];
+    __shared__ uint32_t s_vals[RS_TILE];
+    __shared__ uint32_t s_wave[RS_WAVES][256];
+    __shared__ uint32_t s_local[256];
+    __shared__ uint32_t s_global[256];
+    __shared__ uint32_t s_tmp[4];
+    __shared__ int s_tile;
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid == 0) s_tile = (int)atomicAdd(tile_ctr, 1u);
+    for (int i = tid; i < RS_WAVES * 256; i += RS_BLOCK) (&s_wave[0][0])[i] = 0;
+    __syncthreads();
+    const int tile = s_tile;
+    const int base = tile * RS_TILE + wave * (64 * RS_ITEMS);
+
+    uint64_t k[RS_ITEMS];
+    uint32_t v[RS_ITEMS];
+    uint32_t rank[RS_ITEMS];
+#pragma unroll
+    for (int j = 0; j < RS_ITEMS; ++j) {
+        const int idx = base + j * 64 + lane;
+        const bool ok = idx < n;
+        k[j] = ok ? keys_in[idx] : ~0ull;
+        v[j] = FIRST ? (uint32_t)idx : (ok ? vals_in[idx] : 0u);
+    }
+    // Wave-level multisplit: peers = lanes holding the same digit; the rank of
+    // an item is the wave's running count of its digit plus the peers below it.
+    const uint64_t lt = lanemask_lt();
+#pragma unroll
+    for (int j = 0; j < RS_ITEMS; ++j) {
+        const int idx = base + j * 64 + lane;
+        const bool ok = idx < n;
+        const uint32_t d = (uint32_t)(k[j] >> shift) & 255u;
+        uint64_t peers = __ballot(ok);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const bool bit = (d >> b) & 1u;
+            const uint64_t m = __ballot(bit);
+            peers &= bit ? m : ~m;
+        }
+        const uint32_t before = s_wave[wave][d];
+        const uint32_t below = (uint32_t)__popcll(peers & lt);
+        rank[j] = before + below;
+        __builtin_amdgcn_wave_barrier();
+        if (ok && below == 0) s_wave[wave][d] = before + (uint32_t)__popcll(peers);
+        __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();
+
+    // Per digit (thread = digit): wave offsets, tile count, look-back prefix.
+    const uint32_t dg = (uint32_t)tid;
+    uint32_t count = 0;
+#pragma unroll
+    for (int w = 0; w < RS_WAVES; ++w) {
+        uint32_t c = s_wave[w][dg];
+        s_wave[w][dg] = count;
+        count += c;
+    }
+    const uint64_t tag = (uint64_t)(epoch & 0x3FFFFFFFu) << 34;
+    uint64_t *slot = lookback + (size_t)tile * 256 + dg;
+    uint32_t excl = 0;
+    if (tile == 0) {
+        __hip_atomic_store(slot, tag | (2ull << 32) | count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+        __hip_atomic_store(slot, tag | (1ull << 32) | count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        int p = tile - 1;
+        while (true) {
+            const uint64_t w = __hip_atomic_load(lookback + (size_t)p * 256 + dg, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT);
+            if ((w & ~0x3FFFFFFFFull) != tag || ((w >> 32) & 3u) == 0) {
+                __builtin_amdgcn_s_sleep(1);
+                continue;
+            }
+            excl += (uint32_t)w;
+            if (((w >> 32) & 3u) == 2u) break;
+            --p;
+        }
+        __hip_atomic_store(slot, tag | (2ull << 32) | (uint32_t)(excl + count), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    }
+    const uint32_t local = block_excl_scan256(count, s_tmp);
+    s_local[dg] = local;
+    s_global[dg] = digit_base[dg] + excl;
+    __syncthreads();
+
+    // Scatter into the tile's digit-sorted order in LDS.
+#pragma unroll
+    for (int j = 0; j < RS_ITEMS; ++j) {
+        const int idx = base + j * 64 + lane;
+        if (idx < n) {
+            const uint32_t d = (uint32_t)(k[j] >> shift) & 255u;
+            const uint32_t lp = s_local[d] + s_wave[wave][d] + rank[j];
+            s_keys[lp] = k[j];
+            s_vals[lp] = v[j];
+        }
+    }
+    __syncthreads();
+    const int tile_n = min(RS_TILE, n - tile * RS_TILE);
+    for (int i = tid; i < tile_n; i += RS_BLOCK) {
+        const uint64_t key = s_keys[i];
+        const uint32_t d = (uint32_t)(key >> shift) & 255u;
+        const uint32_t pos = s_global[d] + ((uint32_t)i - s_local[d]);
+        keys_out[pos] = key;
+        vals_out[pos] = s_vals[i];
+    }
+}
+
+// ===========================================================================
+// K8: leaf gather (src/hlbvh.hpp:2266-2279): leaf slot i+n-1 <- particle perm[i]
+// into both min_* and max_*; id (iota when absent), ghost (0 when absent), orig.
+// ===========================================================================
+template <typename T>
+__global__ __launch_bounds__(256) void gather_kernel(
+    const T *__restrict__ x, const T *__restrict__ y, const T *__restrict__ z,
+    const uint32_t *__restrict__ id, const int8_t *__restrict__ ghost,
+    const uint32_t *__restrict__ perm, int n, T *mnx, T *mxx, T *mny, T *mxy, T *mnz, T *mxz,
+    uint32_t *tid_out, int8_t *tghost, int32_t *torig) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t o = perm[i];
+    const int leaf = i + n - 1;
+    const T a = x[o], b = y[o], c = z[o];
+    mnx[leaf] = a; mxx[leaf] = a;
+    mny[leaf] = b; mxy[leaf] = b;
+    mnz[leaf] = c; mxz[leaf] = c;
+    tid_out[leaf] = id ? id[o] : o;
+    tghost[leaf] = ghost ? ghost[o] : (int8_t)0;
+    torig[leaf] = (int32_t)o;
+}
+
+// Copies caller-sorted leaves (build_tree's optional arrays, src/hlbvh.hpp:910-922).
+template <typename T>
+__global__ __launch_bounds__(256) void leaves_from_sorted_kernel(
+    const T *__restrict__ sx, const T *__restrict__ sy, const T *__restrict__ sz,
+    const uint32_t *__restrict__ sid, const int8_t *__restrict__ sghost,
+    const int32_t *__restrict__ sorig, int n, T *mnx, T *mxx, T *mny, T *mxy, T *mnz, T *mxz,
+    uint32_t *tid_out, int8_t *tghost, int32_t *torig) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int leaf = i + n - 1;
+    const T a = sx[i], b = sy[i], c = sz[i];
+    mnx[leaf] = a; mxx[leaf] = a;
+    mny[leaf] = b; mxy[leaf] = b;
+    mnz[leaf] = c; mxz[leaf] = c;
+    if (sid) tid_out[leaf] = sid[i];
+    if (sghost) tghost[leaf] = sghost[i];
+    if (sorig) torig[leaf] = sorig[i];
+}
+
+// ===========================================================================
+// K11: Karras internal nodes (src/hlbvh.hpp:773-903).  delta(i,j) = -1 outside
+// [0,n); clz64 of differing keys; 64 + clz32(i^j) for equal keys (an
+// order-preserving offset of the reference's prefix lengths, so every
+// comparison -- and therefore the tree -- is the same).  Also records each
+// node's leaf range and clears the refit counter.
+// ===========================================================================
+__device__ inline int kdelta(const uint64_t *__restrict__ k, int n, int i, uint64_t ki, int j) {
+    if (j < 0 || j >= n) return -1;
+    const uint64_t kj = k[j];
+    if (ki != kj) return __clzll((long long)(ki ^ kj));
+    return 64 + __clz((int)((uint32_t)i ^ (uint32_t)j));
+}
+
+__global__ __launch_bounds__(256) void karras_kernel(const uint64_t *__restrict__ k, int n,
+                                                     int32_t *__restrict__ left,
+                                                     int32_t *__restrict__ right,
+                                                     int32_t *__restrict__ parent,
+                                                     int32_t *__restrict__ rfirst,
+                                                     int32_t *__restrict__ rlast,
+                                                     uint32_t *__restrict__ counters) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n - 1) return;
+    const uint64_t ki = k[i];
+    const int dd = kdelta(k, n, i, ki, i + 1) - kdelta(k, n, i, ki, i - 1);
+    const int d = (dd > 0) - (dd < 0);
+    const int dmin = kdelta(k, n, i, ki, i - d);
+    int lmax = 2;
+    while (kdelta(k, n, i, ki, i + lmax * d) > dmin) lmax *= 2;
+    int l = 0;
+    for (int t = lmax / 2; t >= 1; t /= 2)
+        if (kdelta(k, n, i, ki, i + (l + t) * d) > dmin) l += t;
+    const int j = i + l * d;
+    const int dnode = kdelta(k, n, i, ki, j);
+    int s = 0, ts = 1;
+    while (ts <= l) ts *= 2;
+    ts /= 2;
+    for (int t = ts; t >= 1; t /= 2)
+        if (s + t < l && kdelta(k, n, i, ki, i + (s + t) * d) > dnode) s += t;
+    const int split = i + s * d + min(d, 0);
+    const int lo = min(i, j), hi = max(i, j);
+    const int li = (lo == split) ? split + n - 1 : split;
+    const int ri = (hi == split + 1) ? split + 1 + n - 1 : split + 1;
+    left[i] = li;
+    right[i] = ri;
+    parent[li] = i;
+    parent[ri] = i;
+    rfirst[i] = lo;
+    rlast[i] = hi;
+    counters[i] = 0;
+    if (i == 0) parent[0] = -1;
+}
+
+// ===========================================================================
+// K13: refit (src/hlbvh.hpp:925-969).  Each leaf climbs; the first arrival at a
+// node stops, the second (after an acquire-release counter increment at agent
+// scope) combines its own box with the sibling's (fmin/fmax for FP, </> for
+// integers, sfc_min/sfc_max :580-594) and continues.
+// ===========================================================================
+template <typename T> __device__ inline T smin(T a, T b) {
+    if constexpr (CoordTraits<T>::fp) return ffmin(a, b);
+    else return (a < b) ? a : b;
+}
+template <typename T> __device__ inline T smax(T a, T b) {
+    if constexpr (CoordTraits<T>::fp) return ffmax(a, b);
+    else return (a < b) ? b : a;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void refit_kernel(int n, const int32_t *__restrict__ left,
+                                                    const int32_t *__restrict__ right,
+                                                    const int32_t *__restrict__ parent,
+                                                    uint32_t *__restrict__ counters, T *mnx,
+                                                    T *mxx, T *mny, T *mxy, T *mnz, T *mxz) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int cur = i + n - 1;
+    T b0 = mnx[cur], b1 = mxx[cur], b2 = mny[cur], b3 = mxy[cur], b4 = mnz[cur], b5 = mxz[cur];
+    while (cur != 0) {
+        const int p = parent[cur];
+        if (p < 0 || p >= n - 1) break;
+        const uint32_t prev =
+            __hip_atomic_fetch_add(&counters[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (prev == 0) return;
+        const int l = left[p], r = right[p];
+        const int sib = (l == cur) ? r : l;
+        b0 = smin(b0, mnx[sib]); b1 = smax(b1, mxx[sib]);
+        b2 = smin(b2, mny[sib]); b3 = smax(b3, mxy[sib]);
+        b4 = smin(b4, mnz[sib]); b5 = smax(b5, mxz[sib]);
+        mnx[p] = b0; mxx[p] = b1; mny[p] = b2; mxy[p] = b3; mnz[p] = b4; mxz[p] = b5;
+        cur = p;
+    }
+}
+
+__global__ void single_leaf_kernel(int32_t *parent) { parent[0] = -1; }
+
+// ===========================================================================
+// Host orchestration
+// ===========================================================================
+static inline unsigned grid_for(size_t n, int block, unsigned cap = 0) {
+    size_t g = (n + block - 1) / block;
+    if (g == 0) g = 1;
+    if (cap && g > cap) g = cap;
+    return (unsigned)g;
+}
+
+static int scratch_ensure(Scratch &S, size_t n) {
+    if (n <= S.cap_n && S.block) return FT_OK;
+    if (S.block) {
+        FT_HIP(hipFree(S.block));
+        S.block = nullptr;
+    }
+    const size_t cap = n + n / 2 + 1024;
+    const size_t tiles = (cap + RS_TILE - 1) / RS_TILE + 1;
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    size_t off = 0;
+    const size_t o_keys_in = off; off += al(cap * 8);
+    const size_t o_keys_a = off; off += al(cap * 8);
+    const size_t o_keys_b = off; off += al(cap * 8);
+    const size_t o_vals_a = off; off += al(cap * 4);
+    const size_t o_vals_b = off; off += al(cap * 4);
+    const size_t o_perm = off; off += al(cap * 4);
+    const size_t o_ctr = off; off += al(cap * 4);
+    const size_t o_hist = off; off += al(RS_MAX_PASSES * 256 * 4 * 2);
+    const size_t o_look = off; off += al(tiles * 256 * 8);
+    const size_t o_tile = off; off += al(64);
+    const size_t o_bbox = off; off += al(16 * 8);
+    const size_t o_part = off; off += al(1024 * 6 * 8);
+    void *blk = nullptr;
+    if (hipMalloc(&blk, off) != hipSuccess) return fail(FT_ENOMEM, "scratch allocation failed");
+    // One-time clear: look-back words carry an epoch, so they never need clearing again.
+    FT_HIP(hipMemset(blk, 0, off));
+    char *b = (char *)blk;
+    S.block = blk;
+    S.block_bytes = off;
+    S.cap_n = cap;
+    S.keys_in = (uint64_t *)(b + o_keys_in);
+    S.keys_a = (uint64_t *)(b + o_keys_a);
+    S.keys_b = (uint64_t *)(b + o_keys_b);
+    S.vals_a = (uint32_t *)(b + o_vals_a);
+    S.vals_b = (uint32_t *)(b + o_vals_b);
+    S.perm = (uint32_t *)(b + o_perm);
+    S.counters = (uint32_t *)(b + o_ctr);
+    S.hist = (uint32_t *)(b + o_hist);
+    S.lookback = (uint64_t *)(b + o_look);
+    S.lookback_words = tiles * 256;
+    S.tile_ctr = (uint32_t *)(b + o_tile);
+    S.bbox_dev = (void *)(b + o_bbox);
+    S.partials = (void *)(b + o_part);
+    return FT_OK;
+}
+
+int scratch_ensure_public(Scratch &S, size_t n) { return scratch_ensure(S, n); }
+
+static void prof_mark(Scratch &S, hipStream_t s, int idx) {
+    if (!S.profiling) return;
+    if (!S.ev[idx]) hipEventCreate(&S.ev[idx]);
+    hipEventRecord(S.ev[idx], s);
+}
+
+template <typename T>
+static int launch_bbox(hipStream_t s, Scratch &S, const T *x, const T *y, const T *z, size_t n,
+                       const T *given6) {
+    BboxDev<T> *bb = (BboxDev<T> *)S.bbox_dev;
+    if (given6) {
+        hipLaunchKernelGGL(bbox_final_kernel<T>, dim3(1), dim3(64), 0, s, (const T *)nullptr, 0, 1,
+                           given6[0], given6[1], given6[2], given6[3], given6[4], given6[5], bb);
+    } else {
+        const unsigned g = grid_for(n, 256, 1024);
+        hipLaunchKernelGGL(bbox_partial_kernel<T>, dim3(g), dim3(256), 0, s, x, y, z, n,
+                           (T *)S.partials);
+        hipLaunchKernelGGL(bbox_final_kernel<T>, dim3(1), dim3(64), 0, s, (const T *)S.partials,
+                           (int)g, 0, T(0), T(0), T(0), T(0), T(0), T(0), bb);
+    }
+    FT_CHECK_LAUNCH();
+    return FT_OK;
+}
+
+template <typename T>
+static int launch_encode(hipStream_t s, Scratch &S, const ft_config &cfg, const T *x, const T *y,
+                         const T *z, size_t n, uint64_t *keys) {
+    const unsigned g = grid_for(n, 256, 8192);
+    hipLaunchKernelGGL(encode_kernel<T>, dim3(g), dim3(256), 0, s, x, y, z, n, cfg.bits_per_dim,
+                       cfg.sfc, (const BboxDev<T> *)S.bbox_dev, keys);
+    FT_CHECK_LAUNCH();
+    return FT_OK;
+}
+
+// Sorts keys_in (n keys of `key_bits` significant bits) into sorted_out, and
+// the permutation into S.perm.
+static int radix_sort(hipStream_t s, Scratch &S, const uint64_t *keys_in, size_t n, int key_bits,
+                      uint64_t *sorted_out) {
+    int passes = (key_bits + 7) / 8;
+    if (passes < 1) passes = 1;
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(grid_for(RS_MAX_PASSES * 256, 256)), dim3(256), 0, s,
+                       S.hist, 0u, (size_t)(RS_MAX_PASSES * 256));
+    uint32_t *base = S.hist + RS_MAX_PASSES * 256;
+    hipLaunchKernelGGL(radix_hist_kernel, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s, keys_in,
+                       n, passes, S.hist);
+    hipLaunchKernelGGL(radix_base_kernel, dim3(1), dim3(256), 0, s, (const uint32_t *)S.hist,
+                       passes, base, S.tile_ctr);
+    FT_CHECK_LAUNCH();
+    const unsigned tiles = (unsigned)((n + RS_TILE - 1) / RS_TILE);
+    const uint64_t *kin = keys_in;
+    const uint32_t *vin = nullptr;
+    for (int p = 0; p < passes; ++p) {
+        const bool last = (p == passes - 1);
+        uint64_t *kout = last ? sorted_out : ((p & 1) ? S.keys_b : S.keys_a);
+        uint32_t *vout = last ? S.perm : ((p & 1) ? S.vals_b : S.vals_a);
+        const uint32_t ep = (uint32_t)(S.epoch++ & 0x3FFFFFFFu);
+        if (p == 0)
+            hipLaunchKernelGGL(onesweep_kernel<true>, dim3(tiles), dim3(RS_BLOCK), 0, s, kin, vin,
+                               kout, vout, (int)n, 8 * p, base + p * 256, S.lookback, ep,
+                               S.tile_ctr + p);
+        else
+            hipLaunchKernelGGL(onesweep_kernel<false>, dim3(tiles), dim3(RS_BLOCK), 0, s, kin, vin,
+                               kout, vout, (int)n, 8 * p, base + p * 256, S.lookback, ep,
+                               S.tile_ctr + p);
+        FT_CHECK_LAUNCH();
+        kin = kout;
+        vin = vout;
+    }
+    if (S.epoch >= 0x3FFFFFF0ull) S.epoch = 1;  // practically never; epochs stay unique
+    return FT_OK;
+}
+
+template <typename T> static int launch_topology(hipStream_t s, ft_tree *t, Scratch &S) {
+    const int n = (int)t->num_leaves;
+    if (n == 1) {
+        hipLaunchKernelGGL(single_leaf_kernel, dim3(1), dim3(1), 0, s, t->parent);
+        FT_CHECK_LAUNCH();
+        return FT_OK;
+    }
+    hipLaunchKernelGGL(karras_kernel, dim3(grid_for(n - 1, 256)), dim3(256), 0, s,
+                       (const uint64_t *)t->sorted_keys, n, t->left_child, t->right_child,
+                       t->parent, t->range_first, t->range_last, S.counters);
+    FT_CHECK_LAUNCH();
+    prof_mark(S, s, 5);
+    hipLaunchKernelGGL(refit_kernel<T>, dim3(grid_for(n, 256)), dim3(256), 0, s, n,
+                       (const int32_t *)t->left_child, (const int32_t *)t->right_child,
+                       (const int32_t *)t->parent, S.counters, (T *)t->min_x, (T *)t->max_x,
+                       (T *)t->min_y, (T *)t->max_y, (T *)t->min_z, (T *)t->max_z);
+    FT_CHECK_LAUNCH();
+    return FT_OK;
+}
+
+template <typename T>
+int build_bvh_t(hipStream_t s, ft_tree *t, Scratch &S, const T *x, const T *y, const T *z,
+                const uint32_t *id, const int8_t *ghost, size_t n, const T *bbox6,
+                const uint64_t *keys) {
+    FT_TRY(scratch_ensure(S, n));
+    const ft_config &cfg = t->config;
+    prof_mark(S, s, 0);
+    const uint64_t *kin = keys;
+    if (!keys) {
+        if constexpr (CoordTraits<T>::fp) FT_TRY(launch_bbox<T>(s, S, x, y, z, n, bbox6));
+        prof_mark(S, s, 1);
+        FT_TRY(launch_encode<T>(s, S, cfg, x, y, z, n, S.keys_in));
+        kin = S.keys_in;
+    } else {
+        prof_mark(S, s, 1);
+    }
+    prof_mark(S, s, 2);
+    FT_TRY(radix_sort(s, S, kin, n, 3 * cfg.bits_per_dim, t->sorted_keys));
+    prof_mark(S, s, 3);
+    hipLaunchKernelGGL(gather_kernel<T>, dim3(grid_for(n, 256)), dim3(256), 0, s, x, y, z, id,
+                       ghost, (const uint32_t *)S.perm, (int)n, (T *)t->min_x, (T *)t->max_x,
+                       (T *)t->min_y, (T *)t->max_y, (T *)t->min_z, (T *)t->max_z, t->id,
+                       t->is_ghost, t->orig_idx);
+    FT_CHECK_LAUNCH();
+    prof_mark(S, s, 4);
+    FT_TRY(launch_topology<T>(s, t, S));
+    prof_mark(S, s, 6);
+    if (S.profiling) S.n_stages = 6;
+    return FT_OK;
+}
+
+int collect_profile(Scratch &S) {
+    if (!S.profiling || S.n_stages == 0) return FT_OK;
+    FT_HIP(hipEventSynchronize(S.ev[S.n_stages]));
+    for (int i = 0; i < S.n_stages; ++i) {
+        float ms = 0.f;
+        FT_HIP(hipEventElapsedTime(&ms, S.ev[i], S.ev[i + 1]));
+        S.stage_ms[i] = ms;
+    }
+    return FT_OK;
+}
+
+template int build_bvh_t<float>(hipStream_t, ft_tree *, Scratch &, const float *, const float *,
+                                const float *, const uint32_t *, const int8_t *, size_t,
+                                const float *, const uint64_t *);
+template int build_bvh_t<double>(hipStream_t, ft_tree *, Scratch &, const double *,
+                                 const double *, const double *, const uint32_t *,
+                                 const int8_t *, size_t, const double *, const uint64_t *);
+template int build_bvh_t<uint32_t>(hipStream_t, ft_tree *, Scratch &, const uint32_t *,
+                                   const uint32_t *, const uint32_t *, const uint32_t *,
+                                   const int8_t *, size_t, const uint32_t *, const uint64_t *);
+template int build_bvh_t<uint64_t>(hipStream_t, ft_tree *, Scratch &, const uint64_t *,
+                                   const uint64_t *, const uint64_t *, const uint32_t *,
+                                   const int8_t *, size_t, const uint64_t *, const uint64_t *);
+
+// compute_bbox: device reduction, result copied to the host (blocking, as the
+// reference reads the shared-USM result on the host).
+template <typename T>
+int compute_bbox_t(hipStream_t s, Scratch &S, const T *x, const T *y, const T *z, size_t n,
+                   T *out6) {
+    FT_TRY(scratch_ensure(S, 1));
+    FT_TRY(launch_bbox<T>(s, S, x, y, z, n, nullptr));
+    BboxDev<T> hb;
+    FT_HIP(hipMemcpyAsync(&hb, S.bbox_dev, sizeof(hb), hipMemcpyDeviceToHost, s));
+    FT_HIP(hipStreamSynchronize(s));
+    for (int a = 0; a < 3; ++a) {
+        out6[2 * a] = hb.lo[a];
+        out6[2 * a + 1] = hb.hi[a];
+    }
+    return FT_OK;
+}
+template int compute_bbox_t<float>(hipStream_t, Scratch &, const float *, const float *,
+                                   const float *, size_t, float *);
+template int compute_bbox_t<double>(hipStream_t, Scratch &, const double *, const double *,
+                                    const double *, size_t, double *);
+template int compute_bbox_t<uint32_t>(hipStream_t, Scratch &, const uint32_t *,
+                                      const uint32_t *, const uint32_t *, size_t, uint32_t *);
+template int compute_bbox_t<uint64_t>(hipStream_t, Scratch &, const uint64_t *,
+                                      const uint64_t *, const uint64_t *, size_t, uint64_t *);
+
+// sfc_encode with a caller box (FP) into device keys.
+template <typename T>
+int sfc_encode_t(hipStream_t s, Scratch &S, const ft_config &cfg, const T *x, const T *y,
+                 const T *z, size_t n, uint64_t *keys, const T *bbox6) {
+    FT_TRY(scratch_ensure(S, 1));
+    if constexpr (CoordTraits<T>::fp) {
+        if (!bbox6) return fail(FT_EINVAL, "sfc_encode: FP coordinates need a bounding box");
+        FT_TRY(launch_bbox<T>(s, S, x, y, z, n, bbox6));
+    }
+    return launch_encode<T>(s, S, cfg, x, y, z, n, keys);
+}
+template int sfc_encode_t<float>(hipStream_t, Scratch &, const ft_config &, const float *,
+                                 const float *, const float *, size_t, uint64_t *, const float *);
+template int sfc_encode_t<double>(hipStream_t, Scratch &, const ft_config &, const double *,
+                                  const double *, const double *, size_t, uint64_t *,
+                                  const double *);
+template int sfc_encode_t<uint32_t>(hipStream_t, Scratch &, const ft_config &, const uint32_t *,
+                                    const uint32_t *, const uint32_t *, size_t, uint64_t *,
+                                    const uint32_t *);
+template int sfc_encode_t<uint64_t>(hipStream_t, Scratch &, const ft_config &, const uint64_t *,
+                                    const uint64_t *, const uint64_t *, size_t, uint64_t *,
+                                    const uint64_t *);
+
+// build_tree over caller-sorted keys (+ optional sorted leaves).
+template <typename T>
+int build_tree_t(hipStream_t s, ft_tree *t, Scratch &S, const uint64_t *sorted_keys, const T *sx,
+                 const T *sy, const T *sz, const uint32_t *sid, const int8_t *sghost,
+                 const int32_t *sorig) {
+    const size_t n = t->num_leaves;
+    FT_TRY(scratch_ensure(S, n));
+    if (sorted_keys != t->sorted_keys)
+        FT_HIP(hipMemcpyAsync(t->sorted_keys, sorted_keys, n * 8, hipMemcpyDeviceToDevice, s));
+    if (sx && sy && sz) {
+        hipLaunchKernelGGL(leaves_from_sorted_kernel<T>, dim3(grid_for(n, 256)), dim3(256), 0, s,
+                           sx, sy, sz, sid, sghost, sorig, (int)n, (T *)t->min_x, (T *)t->max_x,
+                           (T *)t->min_y, (T *)t->max_y, (T *)t->min_z, (T *)t->max_z, t->id,
+                           t->is_ghost, t->orig_idx);
+        FT_CHECK_LAUNCH();
+    }
+    return launch_topology<T>(s, t, S);
+}
+template int build_tree_t<float>(hipStream_t, ft_tree *, Scratch &, const uint64_t *,
+                                 const float *, const float *, const float *, const uint32_t *,
+                                 const int8_t *, const int32_t *);
+template int build_tree_t<double>(hipStream_t, ft_tree *, Scratch &, const uint64_t *,
+                                  const double *, const double *, const double *,
+                                  const uint32_t *, const int8_t *, const int32_t *);
+template int build_tree_t<uint32_t>(hipStream_t, ft_tree *, Scratch &, const uint64_t *,
+                                    const uint32_t *, const uint32_t *, const uint32_t *,
+                                    const uint32_t *, const int8_t *, const int32_t *);
+template int build_tree_t<uint64_t>(hipStream_t, ft_tree *, Scratch &, const uint64_t *,
+                                    const uint64_t *, const uint64_t *, const uint64_t *,
+                                    const uint32_t *, const int8_t *, const int32_t *);
+
+}  // namespace ft

@@ -1,0 +1,250 @@
+// ft_internal.hpp -- shared definitions of the gfx950 HLBVH library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <type_traits>
+
+#include "../../include/fasttree_c.h"
+
+namespace ft {
+
+// ---------------------------------------------------------------------------
+// Errors: every C entry point returns a status; the message is thread-local.
+// ---------------------------------------------------------------------------
+void set_error(const std::string &msg);
+int fail(int code, const std::string &msg);
+
+#define FT_HIP(expr)                                                                      \
+    do {                                                                                  \
+        hipError_t _e = (expr);                                                           \
+        if (_e != hipSuccess)                                                             \
+            return ::ft::fail(FT_EHIP, std::string(#expr ": ") + hipGetErrorString(_e)); \
+    } while (0)
+
+#define FT_CHECK_LAUNCH()                                                                 \
+    do {                                                                                  \
+        hipError_t _e = hipGetLastError();                                                \
+        if (_e != hipSuccess)                                                             \
+            return ::ft::fail(FT_EHIP, std::string("kernel launch: ") + hipGetErrorString(_e)); \
+    } while (0)
+
+#define FT_TRY(...)                    \
+    do {                               \
+        int _rc = (__VA_ARGS__);       \
+        if (_rc != FT_OK) return _rc;  \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// Coordinate / distance types.  dist_t is double for FP coordinates and
+// uint64_t for integer coordinates (src/sfc.peano_hilbert.hpp:185-193).
+// ---------------------------------------------------------------------------
+template <typename T> struct CoordTraits;
+template <> struct CoordTraits<float> { using D = double; static constexpr bool fp = true; static constexpr int ct = FT_COORD_F32; };
+template <> struct CoordTraits<double> { using D = double; static constexpr bool fp = true; static constexpr int ct = FT_COORD_F64; };
+template <> struct CoordTraits<uint32_t> { using D = uint64_t; static constexpr bool fp = false; static constexpr int ct = FT_COORD_U32; };
+template <> struct CoordTraits<uint64_t> { using D = uint64_t; static constexpr bool fp = false; static constexpr int ct = FT_COORD_U64; };
+
+inline size_t coord_size(int ct) { return (ct == FT_COORD_F32 || ct == FT_COORD_U32) ? 4 : 8; }
+
+// Runtime dispatch over the coordinate type recorded in the tree.
+#define FT_DISPATCH_COORD(ct, ...)                              \
+    switch (ct) {                                               \
+    case FT_COORD_F32: { using T = float; __VA_ARGS__; } break;    \
+    case FT_COORD_F64: { using T = double; __VA_ARGS__; } break;   \
+    case FT_COORD_U32: { using T = uint32_t; __VA_ARGS__; } break; \
+    case FT_COORD_U64: { using T = uint64_t; __VA_ARGS__; } break; \
+    default: return ::ft::fail(FT_EINVAL, "unknown coordinate type"); \
+    }
+
+// Metric parameters handed to every query kernel by value.
+struct Metric {
+    int bits;      // B
+    int periodic;  // PERIODIC_BC
+    int dshift;    // _DIST_SHIFT = max(B - 31, 0)
+    int pad;
+    double box;    // FP periodic period L
+};
+
+inline Metric make_metric(const ft_config &c) {
+    Metric m;
+    m.bits = c.bits_per_dim;
+    m.periodic = c.periodic;
+    m.dshift = c.bits_per_dim > 31 ? c.bits_per_dim - 31 : 0;
+    m.pad = 0;
+    m.box = c.box_length;
+    return m;
+}
+
+__host__ __device__ inline float ffmax(float a, float b) { return fmaxf(a, b); }
+__host__ __device__ inline double ffmax(double a, double b) { return fmax(a, b); }
+__host__ __device__ inline float ffmin(float a, float b) { return fminf(a, b); }
+__host__ __device__ inline double ffmin(double a, double b) { return fmin(a, b); }
+__host__ __device__ inline float ffabs(float a) { return fabsf(a); }
+__host__ __device__ inline double ffabs(double a) { return fabs(a); }
+
+// Periodic gap of one axis for integer coordinates: the signed difference
+// wrapped to B bits (shortest_periodic_distance, src/sfc.peano_hilbert.hpp:532-538,
+// with SHIFT_BITS = bits(T) - B), absolute value.
+template <typename T> __host__ __device__ inline uint64_t wrap_gap(T a, T b, int B) {
+    using S = typename std::make_signed<T>::type;
+    const int sh = (int)(sizeof(T) * 8) - B;
+    T diff = (T)(a - b);
+    S v = (S)(T)(diff << sh);
+    v = v >> sh;
+    return (uint64_t)(v < 0 ? -v : v);
+}
+
+// Squared distance from a point to a box (node_distance_sq, src/hlbvh.hpp:998-1055).
+// FP: (dx*dx + dy*dy) + dz*dz in T, no contraction (built with -ffp-contract=off),
+// widened to double.  Integer: per-axis gap >> _DIST_SHIFT, summed squares in u64.
+template <typename T>
+__host__ __device__ inline typename CoordTraits<T>::D box_d2(const Metric &m, T px, T py, T pz,
+                                                             T x0, T x1, T y0, T y1, T z0, T z1) {
+    if constexpr (CoordTraits<T>::fp) {
+        T dx, dy, dz;
+        if (!m.periodic) {
+            dx = ffmax(x0 - px, ffmax(T(0), px - x1));
+            dy = ffmax(y0 - py, ffmax(T(0), py - y1));
+            dz = ffmax(z0 - pz, ffmax(T(0), pz - z1));
+        } else {
+            // No reference counterpart (PERIODIC_BC is a no-op for FP coords): 0 inside
+            // [lo, hi], else the smaller minimum-image distance to either face.
+            const T L = (T)m.box;
+            auto ax = [L](T p, T lo, T hi) -> T {
+                if (p >= lo && p <= hi) return T(0);
+                T a = ffabs(p - lo);
+                a = ffmin(a, L - a);
+                T b = ffabs(p - hi);
+                b = ffmin(b, L - b);
+                return ffmin(a, b);
+            };
+            dx = ax(px, x0, x1);
+            dy = ax(py, y0, y1);
+            dz = ax(pz, z0, z1);
+        }
+        T s = dx * dx;
+        s = s + dy * dy;
+        s = s + dz * dz;
+        return (double)s;
+    } else {
+        uint64_t g[3];
+        const T p[3] = {px, py, pz}, lo[3] = {x0, y0, z0}, hi[3] = {x1, y1, z1};
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            T v = p[a];
+            if (m.periodic) {
+                if (sizeof(T) >= 8) v &= (T)((T(1) << m.bits) - T(1));
+                if (v >= lo[a] && v <= hi[a]) {
+                    g[a] = 0;
+                } else {
+                    uint64_t al = wrap_gap<T>(v, lo[a], m.bits);
+                    uint64_t ah = wrap_gap<T>(v, hi[a], m.bits);
+                    g[a] = (al < ah ? al : ah) >> m.dshift;
+                }
+            } else {
+                if (v < lo[a]) g[a] = (uint64_t)(T)(lo[a] - v) >> m.dshift;
+                else if (v > hi[a]) g[a] = (uint64_t)(T)(v - hi[a]) >> m.dshift;
+                else g[a] = 0;
+            }
+        }
+        return g[0] * g[0] + g[1] * g[1] + g[2] * g[2];
+    }
+}
+
+// Point-to-point distance: the box degenerates to the point.
+template <typename T>
+__host__ __device__ inline typename CoordTraits<T>::D point_d2(const Metric &m, T px, T py, T pz,
+                                                               T x, T y, T z) {
+    return box_d2<T>(m, px, py, pz, x, x, y, y, z, z);
+}
+
+// Sentinel written for a missing neighbour: type_identity_max<dist_t>()
+// (src/hlbvh.hpp:59-69): -DBL_MAX for FP, 0 for integers.
+template <typename D> __host__ __device__ inline D missing_dist() {
+    if constexpr (std::is_floating_point<D>::value) return -DBL_MAX;
+    else return 0;
+}
+
+// Device view of a built tree.
+template <typename T> struct DTree {
+    const T *mnx, *mxx, *mny, *mxy, *mnz, *mxz;
+    const int32_t *left, *right, *parent, *orig;
+    const int32_t *rfirst, *rlast;
+    int n;
+    int orig_indices;
+    Metric m;
+};
+
+template <typename T> inline DTree<T> make_dtree(const ft_tree &t) {
+    DTree<T> d;
+    d.mnx = (const T *)t.min_x; d.mxx = (const T *)t.max_x;
+    d.mny = (const T *)t.min_y; d.mxy = (const T *)t.max_y;
+    d.mnz = (const T *)t.min_z; d.mxz = (const T *)t.max_z;
+    d.left = t.left_child; d.right = t.right_child; d.parent = t.parent; d.orig = t.orig_idx;
+    d.rfirst = t.range_first; d.rlast = t.range_last;
+    d.n = (int)t.num_leaves;
+    d.orig_indices = t.config.return_orig_indices;
+    d.m = make_metric(t.config);
+    return d;
+}
+
+// ---------------------------------------------------------------------------
+// Host staging (ensure_device_readable / copy_back_and_free, src/hlbvh.hpp:125-200).
+// ---------------------------------------------------------------------------
+bool is_device_accessible(const void *p);
+
+struct Staged {
+    void *dev = nullptr;
+    const void *host_src = nullptr;
+    void *host_dst = nullptr;
+    size_t bytes = 0;
+    bool owned = false;
+};
+
+// Returns a device-readable pointer for p (copying host data in), or p itself.
+int stage_in(const void *p, size_t bytes, hipStream_t s, Staged &st);
+// Returns a device-writable pointer (device scratch when p is host memory).
+int stage_out(void *p, size_t bytes, hipStream_t s, Staged &st);
+// Copies staged output back (synchronising the stream) and frees staging memory.
+int unstage(Staged &st, hipStream_t s, bool copy_back);
+
+inline hipStream_t as_stream(ft_stream s) { return (hipStream_t)s; }
+
+// Device scratch owned by ft_scratch: grows as n + n/2 + 1024 (BvhScratch::ensure).
+struct Scratch {
+    void *block = nullptr;
+    size_t block_bytes = 0;
+    size_t cap_n = 0;
+    // carved views
+    uint64_t *keys_in = nullptr;  // encoded keys (n)
+    uint64_t *keys_a = nullptr, *keys_b = nullptr;
+    uint32_t *vals_a = nullptr, *vals_b = nullptr;
+    uint32_t *perm = nullptr;     // final permutation (n)
+    uint32_t *counters = nullptr; // refit counters (n)
+    uint32_t *hist = nullptr;     // 8 passes x 256
+    uint64_t *lookback = nullptr; // tiles x 256 status words
+    size_t lookback_words = 0;
+    uint32_t *tile_ctr = nullptr; // 8 passes
+    void *bbox_dev = nullptr;     // 6 coords + 3 inverse extents (16 x 8 bytes)
+    void *partials = nullptr;     // bbox partials (1024 x 6 x 8 bytes)
+    uint64_t epoch = 1;           // look-back epochs: never reused
+    // staging of inputs (host pointers)
+    void *stage_block = nullptr;
+    size_t stage_bytes = 0;
+    // profiling
+    int profiling = 0;
+    hipEvent_t ev[8] = {};
+    double stage_ms[8] = {};
+    int n_stages = 0;
+};
+
+}  // namespace ft
+
+struct ft_scratch {
+    ft::Scratch s;
+};

@@ -1,0 +1,206 @@
+/*
+ * fasttree_c.h -- C ABI of the MI355X (gfx950) HLBVH build and neighbour-query path.
+ *
+ * Every entry point is plain C: pointers, sizes and a stream handle.  Coordinates
+ * are passed as `const void *` and interpreted with the coordinate type recorded
+ * in the tree (FT_COORD_*), which replaces the reference's compile-time
+ * `coord_t` (src/hlbvh.hpp:202-209).  Distances are `double` for FP coordinates
+ * and `uint64_t` for integer coordinates, as the reference's `dist_t`
+ * (src/sfc.peano_hilbert.hpp:185-193, src/sfc.morton.hpp:18-23).
+ *
+ * The reference's compile-time switches (SFC_TYPE_*, BITS_PER_DIMENSION,
+ * PERIODIC_BC, RETURN_ORIG_INDICES; CMakeLists.txt:63-125) become the runtime
+ * ft_config stored in the tree when it is allocated.
+ *
+ * Pointers may be device pointers (hipMalloc / ft_malloc) or plain host
+ * pointers; host pointers are staged through device memory and copied back,
+ * as the reference's ensure_device_readable / copy_back_and_free do
+ * (src/hlbvh.hpp:125-200).  Calls with device pointers are stream-ordered and
+ * asynchronous; calls that stage host memory synchronise the stream.
+ *
+ * Return value: FT_OK or an FT_E* code; ft_last_error() gives the message.  The
+ * C++ wrapper (fasttree/hlbvh.hpp) turns the codes back into the reference's
+ * exceptions.
+ */
+#ifndef FASTTREE_C_H
+#define FASTTREE_C_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------- */
+#define FT_OK 0
+#define FT_EINVAL 1   /* reference: std::invalid_argument (k > _MAX_K_), bad argument   */
+#define FT_ERANGE 2   /* reference: std::runtime_error (n > INT_MAX/2), size limits       */
+#define FT_ELDS 3     /* reference: std::runtime_error (local memory requirement)         */
+#define FT_ENOMEM 4   /* device allocation failed                                          */
+#define FT_EHIP 5     /* HIP runtime error                                                  */
+
+/* ---- coordinate types (reference coord_t) ----------------------------- */
+#define FT_COORD_F32 0
+#define FT_COORD_F64 1
+#define FT_COORD_U32 2 /* POSITIONS_IN_32BIT + FASTTREE_INTEGER_COORDS */
+#define FT_COORD_U64 3 /* POSITIONS_IN_64BIT + FASTTREE_INTEGER_COORDS */
+
+/* ---- space-filling curves --------------------------------------------- */
+#define FT_SFC_MORTON 0
+#define FT_SFC_PEANO_HILBERT 1
+
+/* ---- query semantics --------------------------------------------------- */
+#define FT_MODE_CANONICAL 0 /* results ordered by (d2, returned index); exact top-k set        */
+#define FT_MODE_REFERENCE 1 /* the reference's heap mechanics: same tie picks, 64-entry stack  */
+
+#define FT_ALGO_AUTO 0
+#define FT_ALGO_PLAIN 1   /* one lane per query, independent DFS (reference knn_query_small_k) */
+#define FT_ALGO_GROUPED 2 /* SFC-grouped lockstep DFS with a wave vote (reference self-kNN)    */
+#define FT_ALGO_PACKET 3  /* one shared traversal per wavefront of SFC-adjacent queries        */
+
+/* Stream handle: a hipStream_t passed as an opaque pointer.  NULL = null stream. */
+typedef void *ft_stream;
+
+/* Replaces the reference's compile-time configuration. */
+typedef struct ft_config {
+    int32_t sfc;                 /* FT_SFC_MORTON | FT_SFC_PEANO_HILBERT                   */
+    int32_t bits_per_dim;        /* B in [1, 21]: 3B-bit keys                              */
+    int32_t periodic;            /* PERIODIC_BC: integer coords wrap at 2^B                */
+    int32_t return_orig_indices; /* RETURN_ORIG_INDICES                                    */
+    double box_length;           /* FP coords with periodic=1 only: period L (no reference */
+                                 /* counterpart; the reference ignores PERIODIC_BC for FP) */
+} ft_config;
+
+/*
+ * TreeSoA (src/hlbvh.hpp:609-673): the first fourteen fields keep the
+ * reference's names and order.  Internal nodes are [0, n-2] (root 0), leaves
+ * [n-1, 2n-2].  The remaining fields are extensions with no reference
+ * counterpart.
+ */
+typedef struct ft_tree {
+    void *min_x, *max_x; /* coord_t[2n-1] */
+    void *min_y, *max_y;
+    void *min_z, *max_z;
+    int32_t *left_child;  /* [n-1] */
+    int32_t *right_child; /* [n-1] */
+    int32_t *parent;      /* [2n-1], root -1 */
+    uint32_t *id;         /* [2n-1], set on leaves */
+    int8_t *is_ghost;     /* [2n-1], set on leaves */
+    int32_t *orig_idx;    /* [2n-1], set on leaves */
+    size_t num_leaves;
+    size_t num_internal;
+    /* ---- extensions ---- */
+    int32_t coord_type;    /* FT_COORD_* */
+    int32_t reserved0;
+    ft_config config;
+    uint64_t *sorted_keys; /* [n] SFC keys in leaf order                          */
+    int32_t *range_first;  /* [n-1] first leaf rank under each internal node      */
+    int32_t *range_last;   /* [n-1] last leaf rank under each internal node       */
+    size_t capacity;       /* leaves the allocation can hold (TreePool semantics) */
+    void *block;           /* single backing allocation owned by the library      */
+} ft_tree;
+
+/* Persistent build scratch (BvhScratch, src/hlbvh.hpp:720-745). Opaque. */
+typedef struct ft_scratch ft_scratch;
+
+/* Per-call query options.  NULL = all defaults. */
+typedef struct ft_query_opts {
+    int32_t max_k;       /* the reference's _MAX_K_ template argument; 0 = that entry   */
+                         /* point's reference default (32 self, 256 subset/knn)          */
+    int32_t mode;        /* FT_MODE_*                                                   */
+    int32_t algorithm;   /* FT_ALGO_*                                                   */
+    int32_t group_width; /* lanes that share a traversal (FT_ALGO_PACKET): 0 = auto      */
+    const void *seed_bound; /* optional per-query squared-distance bound (dist_t[num_q]):*/
+                            /* the search is first limited to it; queries that find     */
+                            /* fewer than k neighbours inside it are re-run unbounded.  */
+                            /* Canonical mode only.  Results are unchanged by it.       */
+    uint64_t *stats;     /* optional device uint64_t[8] counters, accumulated:           */
+                         /* [0] nodes visited [1] leaves tested [2] stack overflows     */
+                         /* [3] seeded queries re-run unbounded [4] heap insertions     */
+} ft_query_opts;
+
+/* ---- library / device utilities --------------------------------------- */
+const char *ft_last_error(void);
+const char *ft_version(void);
+int ft_device_count(int *count);
+int ft_malloc(void **ptr, size_t bytes);
+int ft_free(void *ptr);
+int ft_memcpy_h2d(void *dst, const void *src, size_t bytes, ft_stream s);
+int ft_memcpy_d2h(void *dst, const void *src, size_t bytes, ft_stream s);
+int ft_memcpy_d2d(void *dst, const void *src, size_t bytes, ft_stream s);
+int ft_memset(void *dst, int value, size_t bytes, ft_stream s);
+int ft_stream_create(ft_stream *s);
+int ft_stream_destroy(ft_stream s);
+int ft_stream_synchronize(ft_stream s);
+int ft_event_create(void **ev);
+int ft_event_destroy(void *ev);
+int ft_event_record(void *ev, ft_stream s);
+int ft_event_elapsed_ms(void *start, void *stop, float *ms);
+
+/* ---- tree and scratch lifetime ---------------------------------------- */
+/* TreeSoA(q, n) (src/hlbvh.hpp:632-650).  Allocates device memory for n leaves. */
+int ft_tree_alloc(ft_tree *tree, size_t n, int32_t coord_type, const ft_config *cfg, ft_stream s);
+/* TreePool::ensure (src/hlbvh.hpp:682-715): reuse the allocation when n fits,  */
+/* else reallocate with capacity n + n/2 + 1024.                                */
+int ft_tree_ensure(ft_tree *tree, size_t n, int32_t coord_type, const ft_config *cfg, ft_stream s);
+/* TreeSoA::free(q) (src/hlbvh.hpp:657-672). */
+int ft_tree_free(ft_tree *tree, ft_stream s);
+/* BvhScratch (src/hlbvh.hpp:720-745). */
+int ft_scratch_create(ft_scratch **scratch);
+int ft_scratch_destroy(ft_scratch *scratch);
+/* Stage timing of the next builds: on = record HIP events between build stages. */
+int ft_scratch_set_profiling(ft_scratch *scratch, int on);
+/* ms of the last profiled build per stage: bbox, encode, sort, gather, karras, refit. */
+int ft_scratch_stage_ms(ft_scratch *scratch, double *ms, int max_stages, int *n_stages);
+
+/* ---- build ------------------------------------------------------------- */
+/* compute_bbox (src/hlbvh.hpp:462-578).  out6 (host) = min_x,max_x,min_y,max_y,min_z,max_z. */
+int ft_compute_bbox(ft_stream s, int32_t coord_type, const void *x, const void *y, const void *z,
+                    size_t n, void *out6);
+/* sfc_encode (src/hlbvh.hpp:357-423).  bbox6 (host, coord_t[6]) is used by FP coords only. */
+int ft_sfc_encode(ft_stream s, int32_t coord_type, const ft_config *cfg, const void *x,
+                  const void *y, const void *z, size_t n, uint64_t *keys, const void *bbox6);
+/* build_tree (src/hlbvh.hpp:773-982): Karras topology + refit over sorted keys.  The   */
+/* sorted coordinate/attribute arrays are optional as in the reference (NULL = leaves  */
+/* already hold them).                                                                 */
+int ft_build_tree(ft_stream s, ft_tree *tree, const uint64_t *sorted_keys, const void *sx,
+                  const void *sy, const void *sz, const uint32_t *sid, const int8_t *sghost,
+                  const int32_t *sorig);
+/* build_bvh (src/hlbvh.hpp:2165-2302).  id / is_ghost may be NULL (iota / zeros).  bbox6 */
+/* (host coord_t[6]) and keys (n unsorted SFC keys) are optional precomputed inputs.     */
+/* scratch may be NULL (a temporary one is used, as the reference's 3-arg overload).     */
+int ft_build_bvh(ft_stream s, ft_tree *tree, ft_scratch *scratch, const void *x, const void *y,
+                 const void *z, const uint32_t *id, const int8_t *is_ghost, size_t n,
+                 const void *bbox6, const uint64_t *keys);
+
+/* ---- queries ------------------------------------------------------------ */
+/* self_knn_query (src/hlbvh.hpp:1929-1954).  results: uint64_t[n*k] (size_t in the     */
+/* reference), dists: dist_t[n*k].  Missing neighbours: index UINT64_MAX, distance       */
+/* -DBL_MAX (FP) or 0 (integer), as type_identity_max (src/hlbvh.hpp:59-69).             */
+int ft_self_knn_query(ft_stream s, const ft_tree *tree, int k, uint64_t *results, void *dists,
+                      int exclude_self, const ft_query_opts *opts);
+/* self_knn_query_subset (src/hlbvh.hpp:1956-1983): leaf ranks, results in query order. */
+int ft_self_knn_query_subset(ft_stream s, const ft_tree *tree, const int32_t *leaf_ids,
+                             int num_ids, int k, uint64_t *results, void *dists,
+                             int exclude_self, const ft_query_opts *opts);
+/* knn_query (src/hlbvh.hpp:1405-1472): external query points. */
+int ft_knn_query(ft_stream s, const ft_tree *tree, const void *qx, const void *qy,
+                 const void *qz, int k, int num_queries, uint64_t *results, void *dists,
+                 const ft_query_opts *opts);
+/* range_query (src/hlbvh.hpp:2006-2136): fixed stride output, counts = total hits. */
+int ft_range_query(ft_stream s, const ft_tree *tree, const void *qx, const void *qy,
+                   const void *qz, const void *r_min, const void *r_max, int num_queries,
+                   int32_t *results, int32_t *result_counts, int max_results_per_query);
+/* Range query with CSR output (no reference counterpart): row_offsets int64_t[Q+1],   */
+/* cols int32_t[capacity].  Pass cols = NULL to get only counts/offsets (total in      */
+/* row_offsets[Q]); hits beyond col_capacity return FT_ERANGE after filling offsets.   */
+int ft_range_query_csr(ft_stream s, const ft_tree *tree, const void *qx, const void *qy,
+                       const void *qz, const void *r_min, const void *r_max, int num_queries,
+                       int64_t *row_offsets, int32_t *cols, int64_t col_capacity);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FASTTREE_C_H */

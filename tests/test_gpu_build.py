@@ -1,0 +1,187 @@
+"""GPU parity of the build: every TreeSoA array, the sorted keys and the node
+ranges must equal the oracle's, bit for bit."""
+import numpy as np
+import pytest
+
+import oracle as O
+from fasttree_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+ARRAYS = ["min_x", "max_x", "min_y", "max_y", "min_z", "max_z"]
+
+
+def gpu_build(ft, ct, x, y, z, sfc, bits, periodic=0, box=0.0, ids=None, ghost=None,
+              bbox=None, keys=None, scratch=None):
+    cfg = ft.Config(sfc=sfc, bits_per_dim=bits, periodic=periodic, box_length=box)
+    tree = ft.TreeSoA(len(x), ct, cfg)
+    dt = ft.COORD_DTYPES[ct]
+    p = ft.particles(*(ft.DeviceArray.from_host(a, dt) for a in (x, y, z)),
+                     id=None if ids is None else ft.DeviceArray.from_host(ids, np.uint32),
+                     is_ghost=None if ghost is None else ft.DeviceArray.from_host(ghost, np.int8))
+    ft.build_bvh(p, tree, scratch=scratch, bbox=bbox, key=keys)
+    return tree
+
+
+def compare_trees(g, t, n):
+    m = 2 * n - 1
+    o = n - 1
+    assert np.array_equal(g["sorted_keys"], t.sorted_keys[:n]), "sorted keys"
+    assert np.array_equal(g["orig_idx"][o:m], t.orig[o:m]), "permutation"
+    assert np.array_equal(g["id"][o:m], t.id[o:m]), "ids"
+    assert np.array_equal(g["is_ghost"][o:m], t.ghost[o:m]), "ghost flags"
+    for a in ARRAYS:
+        assert np.array_equal(g[a][:m].view(np.uint8), t.arrays[a][:m].view(np.uint8)), a
+    if n > 1:
+        assert np.array_equal(g["left_child"], t.left), "left"
+        assert np.array_equal(g["right_child"], t.right), "right"
+        assert np.array_equal(g["range_first"], t.range_first)
+        assert np.array_equal(g["range_last"], t.range_last)
+    assert np.array_equal(g["parent"][:m], t.parent[:m]), "parent"
+
+
+CASES = [
+    # (name, coord type, sfc, bits, periodic, generator)
+    ("ref-uniform-u64-ph", O.CT_U64, O.SFC_PH, 21, 1, ("ref", O.GEN_UNIFORM, 1000, 42)),
+    ("ref-clustered-f64-ph", O.CT_F64, O.SFC_PH, 21, 0, ("ref", O.GEN_CLUSTERED, 10000, 54321)),
+    ("ref-degenerate-u32-morton", O.CT_U32, O.SFC_MORTON, 21, 0, ("ref", O.GEN_DEGENERATE, 1000, 999)),
+    ("ref-grid-f32-morton", O.CT_F32, O.SFC_MORTON, 10, 0, ("ref", O.GEN_GRID, 1000, 0)),
+    ("uniform-f64-morton10", O.CT_F64, O.SFC_MORTON, 10, 0, ("uni", 100_000, 1)),
+    ("nfw-u64-ph21", O.CT_U64, O.SFC_PH, 21, 1, ("nfw", 300_000, 3)),
+    ("nfw-u32-morton21", O.CT_U32, O.SFC_MORTON, 21, 0, ("nfw", 150_000, 5)),
+    ("nfw-f64-ph21", O.CT_F64, O.SFC_PH, 21, 0, ("nfw", 120_000, 9)),
+    ("tiny-3", O.CT_F64, O.SFC_MORTON, 10, 0, ("uni", 3, 2)),
+    ("tiny-2", O.CT_U64, O.SFC_PH, 21, 0, ("uni", 2, 4)),
+]
+
+
+def make_points(gen, ct):
+    dt = O.COORD_DTYPES[ct]
+    if gen[0] == "ref":
+        return O.gen_reference(gen[1], gen[2], gen[3], ct, bits=21)
+    if gen[0] == "uni":
+        x, y, z = synth.uniform(gen[1], gen[2], 1.0)
+    else:
+        x, y, z = synth.nfw_clustered(gen[1], gen[2], box=100.0, n_halos=200)
+    if np.issubdtype(dt, np.integer):
+        return tuple(synth.to_fixed(a, 100.0 if gen[0] == "nfw" else 1.0, 21, dt)
+                     for a in (x, y, z))
+    return tuple(a.astype(dt) for a in (x, y, z))
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_build_matches_oracle(ft, orc, case):
+    name, ct, sfc, bits, periodic, gen = case
+    print(f"[build] {name}", flush=True)
+    x, y, z = make_points(gen, ct)
+    n = len(x)
+    ghost = (np.arange(n) % 7 == 0).astype(np.int8)
+    ids = (np.arange(n, dtype=np.uint32) * 3 + 11)
+    tree = gpu_build(ft, ct, x, y, z, sfc, bits, periodic, ids=ids, ghost=ghost)
+    g = tree.to_host()
+    t = O.build_bvh(ct, x, y, z, sfc=sfc, bits=bits, periodic=periodic, ids=ids, ghost=ghost)
+    compare_trees(g, t, n)
+
+
+def test_single_particle(ft, orc):
+    x = np.array([5.0]); y = np.array([6.0]); z = np.array([7.0])
+    tree = gpu_build(ft, O.CT_F64, x, y, z, O.SFC_MORTON, 10)
+    g = tree.to_host()
+    assert g["min_x"][0] == 5.0 and g["max_z"][0] == 7.0 and g["parent"][0] == -1
+    assert g["orig_idx"][0] == 0 and g["id"][0] == 0
+
+
+def test_default_ids_and_ghosts(ft, orc):
+    x, y, z = make_points(("uni", 5000, 8), O.CT_F64)
+    tree = gpu_build(ft, O.CT_F64, x, y, z, O.SFC_MORTON, 10)
+    g = tree.to_host()
+    o = 4999
+    assert np.array_equal(g["id"][o:], g["orig_idx"][o:].astype(np.uint32))
+    assert not g["is_ghost"][o:].any()
+
+
+def test_precomputed_bbox_and_keys(ft, orc):
+    x, y, z = make_points(("uni", 20000, 12), O.CT_F64)
+    bb = np.array([-1.0, 2.0, -1.0, 2.0, -1.0, 2.0])
+    tree = gpu_build(ft, O.CT_F64, x, y, z, O.SFC_PH, 21, bbox=bb)
+    t = O.build_bvh(O.CT_F64, x, y, z, sfc=O.SFC_PH, bits=21, bbox6=bb)
+    compare_trees(tree.to_host(), t, len(x))
+    keys = (np.arange(len(x), dtype=np.uint64) * 2654435761) % 1000
+    tree2 = gpu_build(ft, O.CT_F64, x, y, z, O.SFC_PH, 21,
+                      keys=ft.DeviceArray.from_host(keys, np.uint64))
+    t2 = O.build_bvh(O.CT_F64, x, y, z, sfc=O.SFC_PH, bits=21, keys=keys)
+    compare_trees(tree2.to_host(), t2, len(x))
+
+
+def test_scratch_reuse_and_tree_pool(ft, orc):
+    """BvhScratch / TreePool semantics: one scratch and one pooled tree across
+    builds of different sizes (src/hlbvh.hpp:682-745)."""
+    scratch = ft.BvhScratch()
+    cfg = ft.Config(sfc=O.SFC_PH, bits_per_dim=21, periodic=1)
+    tree = ft.TreeSoA(10, O.CT_U64, cfg)
+    for n, seed in ((50_000, 21), (7_000, 22), (80_000, 23)):
+        x, y, z = make_points(("nfw", n, seed), O.CT_U64)
+        tree.ensure(n)
+        p = ft.particles(*(ft.DeviceArray.from_host(a) for a in (x, y, z)))
+        ft.build_bvh(p, tree, scratch=scratch)
+        t = O.build_bvh(O.CT_U64, x, y, z, sfc=O.SFC_PH, bits=21, periodic=1)
+        compare_trees(tree.to_host(), t, n)
+
+
+def test_host_pointer_staging(ft, orc):
+    x, y, z = make_points(("uni", 4096, 31), O.CT_F64)
+    cfg = ft.Config(sfc=O.SFC_MORTON, bits_per_dim=10)
+    tree = ft.TreeSoA(len(x), O.CT_F64, cfg)
+    ft.build_bvh(ft.particles(x, y, z), tree)  # host numpy arrays: staged
+    t = O.build_bvh(O.CT_F64, x, y, z, sfc=O.SFC_MORTON, bits=10)
+    compare_trees(tree.to_host(), t, len(x))
+
+
+@pytest.mark.parametrize("ct", [O.CT_F64, O.CT_F32, O.CT_U32, O.CT_U64])
+@pytest.mark.parametrize("sfc", [O.SFC_MORTON, O.SFC_PH])
+def test_encode_and_bbox_match_oracle(ft, orc, ct, sfc):
+    x, y, z = make_points(("nfw", 30000, 40 + ct), ct)
+    if ct in (O.CT_F64, O.CT_F32):
+        # include exact extremes so the Morton maximum quirk is exercised
+        x = x.copy(); x[5] = x.max(); x[6] = x.min()
+    bb = ft.compute_bbox(x, y, z)
+    assert np.array_equal(bb, O.bbox(ct, x, y, z))
+    bits = 10 if (ct == O.CT_F32 and sfc == O.SFC_MORTON) else 21
+    cfg = ft.Config(sfc=sfc, bits_per_dim=bits)
+    keys = ft.sfc_encode(x, y, z, cfg, bbox=bb)
+    assert np.array_equal(keys, O.encode(ct, sfc, bits, x, y, z, bb))
+
+
+def test_build_tree_collinear_kat(ft, orc):
+    """test/test_query.cpp: build_tree over 10 collinear points with keys = i, then
+    the range search [5.5 +- 2] finds leaves 4..7."""
+    n = 10
+    cfg = ft.Config(sfc=O.SFC_MORTON, bits_per_dim=10)
+    tree = ft.TreeSoA(n, O.CT_F64, cfg)
+    keys = np.arange(n, dtype=np.uint64)
+    sx = np.arange(n, dtype=np.float64)
+    ft.build_tree(tree, keys, sx, np.zeros(n), np.zeros(n))
+    g = tree.to_host()
+    left, right, parent, rf, rl = O.karras(keys)
+    assert np.array_equal(g["left_child"], left) and np.array_equal(g["right_child"], right)
+    res, cnt = ft.range_query(tree, np.array([5.5]), np.zeros(1), np.zeros(1), np.zeros(1),
+                              np.array([2.0]), 10)
+    assert cnt[0] == 4 and list(res[0, :4]) == [4, 5, 6, 7]
+
+
+@pytest.mark.parametrize("n", [10_000_000])
+def test_full_size_build_properties(ft, orc, n):
+    """The 10M clustered int64 snapshot: size-independent properties on the GPU
+    tree, then a full comparison with the oracle build."""
+    print(f"[build] full size n={n}", flush=True)
+    x, y, z = synth.clustered_fixed(n, 0)
+    tree = gpu_build(ft, O.CT_U64, x, y, z, O.SFC_PH, 21, periodic=1)
+    g = tree.to_host()
+    k = g["sorted_keys"]
+    assert np.all(k[1:] >= k[:-1])
+    o = n - 1
+    perm = g["orig_idx"][o:]
+    assert np.array_equal(np.sort(perm), np.arange(n, dtype=np.int32))
+    assert np.array_equal(g["min_x"][o:], x[perm])
+    t = O.build_bvh(O.CT_U64, x, y, z, sfc=O.SFC_PH, bits=21, periodic=1)
+    compare_trees(g, t, n)

@@ -1,0 +1,186 @@
+"""GPU parity of the k-nearest-neighbour queries.
+
+Canonical mode (the default): indices and squared distances equal the oracle's
+brute force ordered by (d2, index) -- bit for bit, ties included.
+Reference mode: equal to the oracle's restatement of the reference's traversal
+and heap mechanics -- the same tie picks the reference makes.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from fasttree_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def make(ft, ct, sfc, bits, periodic, pts, box=0.0):
+    x, y, z = pts
+    cfg = ft.Config(sfc=sfc, bits_per_dim=bits, periodic=periodic, box_length=box)
+    tree = ft.TreeSoA(len(x), ct, cfg)
+    ft.build_bvh(ft.particles(*(ft.DeviceArray.from_host(a) for a in (x, y, z))), tree)
+    t = O.build_bvh(ct, x, y, z, sfc=sfc, bits=bits, periodic=periodic, box=box)
+    return tree, t
+
+
+def ref_points(kind, n, seed, ct):
+    return O.gen_reference(kind, n, seed, ct, bits=21)
+
+
+def nfw_points(n, seed, ct, halos=200):
+    x, y, z = synth.nfw_clustered(n, seed, box=100.0, n_halos=halos)
+    if ct in (O.CT_U64, O.CT_U32):
+        dt = O.COORD_DTYPES[ct]
+        return tuple(synth.to_fixed(a, 100.0, 21, dt) for a in (x, y, z))
+    return tuple(a.astype(O.COORD_DTYPES[ct]) for a in (x, y, z))
+
+
+ALGOS = {"packet": 3, "grouped": 2, "plain": 1}
+
+DATASETS = [
+    ("ref-uniform-u64-periodic", O.CT_U64, O.SFC_PH, 21, 1, lambda: ref_points(O.GEN_UNIFORM, 1000, 42, O.CT_U64)),
+    ("ref-clustered-f64", O.CT_F64, O.SFC_PH, 21, 0, lambda: ref_points(O.GEN_CLUSTERED, 1000, 12345, O.CT_F64)),
+    ("ref-degenerate-u64", O.CT_U64, O.SFC_PH, 21, 0, lambda: ref_points(O.GEN_DEGENERATE, 1000, 999, O.CT_U64)),
+    ("ref-clustered10k-u32-morton", O.CT_U32, O.SFC_MORTON, 21, 0, lambda: ref_points(O.GEN_CLUSTERED, 10000, 54321, O.CT_U32)),
+    ("nfw-u64-periodic", O.CT_U64, O.SFC_PH, 21, 1, lambda: nfw_points(12000, 5, O.CT_U64)),
+    ("uniform-f64-morton10", O.CT_F64, O.SFC_MORTON, 10, 0, lambda: tuple(synth.uniform(8000, 1, 1.0))),
+    ("nfw-f32", O.CT_F32, O.SFC_PH, 21, 0, lambda: nfw_points(6000, 6, O.CT_F32)),
+]
+
+
+@pytest.mark.parametrize("ds", DATASETS, ids=[d[0] for d in DATASETS])
+@pytest.mark.parametrize("algo", list(ALGOS))
+def test_self_knn_canonical(ft, orc, ds, algo):
+    name, ct, sfc, bits, periodic, gen = ds
+    tree, t = make(ft, ct, sfc, bits, periodic, gen())
+    ks = (1, 5, 16, 32) if algo == "packet" else (1, 16, 32, 33, 64)
+    for k in ks:
+        for excl in (True, False):
+            print(f"[knn] {name} {algo} k={k} excl={excl}", flush=True)
+            r, d = ft.self_knn_query(tree, k, exclude_self=excl, max_k=256,
+                                     algorithm=ALGOS[algo])
+            rb, db = O.knn_bruteforce(t, k, excl)
+            assert np.array_equal(d, db), f"d2 k={k} excl={excl}"
+            assert np.array_equal(r, rb), f"idx k={k} excl={excl}"
+
+
+@pytest.mark.parametrize("ds", DATASETS[:4], ids=[d[0] for d in DATASETS[:4]])
+def test_self_knn_reference_mode(ft, orc, ds):
+    """Reference heap mechanics on the GPU reproduce the reference's tie picks."""
+    name, ct, sfc, bits, periodic, gen = ds
+    tree, t = make(ft, ct, sfc, bits, periodic, gen())
+    for k in (16, 32, 33, 64, 128, 256):
+        for excl in (True, False):
+            print(f"[knn-ref] {name} k={k} excl={excl}", flush=True)
+            r, d = ft.self_knn_query(tree, k, exclude_self=excl, max_k=256,
+                                     mode=ft.MODE_REFERENCE)
+            rr, dr = O.knn_reference(t, k, excl, max_k=256)
+            assert np.array_equal(d, dr)
+            assert np.array_equal(r, rr)
+
+
+@pytest.mark.parametrize("algo", ["packet", "grouped"])
+def test_self_knn_subset(ft, orc, algo):
+    """self_knn_query_subset: leaf ranks in any order, results in query order,
+    including the reference's adversarial interleaved-cluster subset
+    (test/test_self_knn.cpp:688-782)."""
+    pts = ref_points(O.GEN_CLUSTERED, 10000, 54321, O.CT_U64)
+    tree, t = make(ft, O.CT_U64, O.SFC_PH, 21, 0, pts)
+    lx = t.leaf_coords()[0]
+    mid = np.uint64(1 << 20)
+    c1 = np.nonzero(lx < mid)[0]
+    c2 = np.nonzero(lx >= mid)[0]
+    inter = np.array([c1[(i // 2) % len(c1)] if i % 2 == 0 else c2[(i // 2) % len(c2)]
+                      for i in range(64)], np.int32)
+    rng = np.random.default_rng(3)
+    rand = rng.permutation(10000)[:1999].astype(np.int32)
+    for ids in (inter, rand, np.array([0, 9999, 5000], np.int32)):
+        for k in (8, 32):
+            r, d = ft.self_knn_query_subset(tree, ids, k, algorithm=ALGOS[algo])
+            rb, db = O.knn_bruteforce(t, k, True, leaf_ids=ids)
+            assert np.array_equal(d, db) and np.array_equal(r, rb)
+
+
+def test_external_knn(ft, orc):
+    pts = nfw_points(20000, 7, O.CT_U64)
+    tree, t = make(ft, O.CT_U64, O.SFC_PH, 21, 1, pts)
+    rng = np.random.default_rng(11)
+    q = tuple(rng.integers(0, 1 << 21, 777, dtype=np.uint64) for _ in range(3))
+    for k in (1, 8, 32, 33, 128):
+        r, d = ft.knn_query(tree, *q, k)
+        rb, db = O.knn_bruteforce(t, k, queries=q)
+        assert np.array_equal(d, db) and np.array_equal(r, rb), f"k={k}"
+
+
+def test_external_knn_reference_merge_tier(ft, orc):
+    """knn_query k > 32 in reference mode: SortedMergeHeap tie order."""
+    pts = ref_points(O.GEN_UNIFORM, 1000, 42, O.CT_U64)
+    tree, t = make(ft, O.CT_U64, O.SFC_PH, 21, 0, pts)
+    x, y, z = pts
+    q = (x[::31].copy(), y[::31].copy(), z[::31].copy())
+    for k in (33, 64, 128):
+        r, d = ft.knn_query(tree, *q, k, mode=ft.MODE_REFERENCE)
+        rr, dr = O.knn_reference(t, k, queries=q, kind=1, max_k=256)
+        assert np.array_equal(d, dr) and np.array_equal(r, rr), f"k={k}"
+
+
+def test_return_orig_indices(ft, orc):
+    x, y, z = nfw_points(5000, 8, O.CT_F64)
+    cfg = ft.Config(sfc=O.SFC_PH, bits_per_dim=21, return_orig_indices=True)
+    tree = ft.TreeSoA(len(x), O.CT_F64, cfg)
+    ft.build_bvh(ft.particles(x, y, z), tree)
+    t = O.build_bvh(O.CT_F64, x, y, z, sfc=O.SFC_PH, bits=21, orig_indices=1)
+    for algo in ("packet", "grouped"):
+        r, d = ft.self_knn_query(tree, 12, algorithm=ALGOS[algo])
+        rb, db = O.knn_bruteforce(t, 12, True)
+        assert np.array_equal(d, db) and np.array_equal(r, rb)
+
+
+@pytest.mark.parametrize("n", [1, 2, 7, 31, 33, 63, 65, 99])
+def test_edge_sizes_and_sentinels(ft, orc, n):
+    """Edge cases of test/test_self_knn.cpp:876-1003: n=1, n=2, partial wavefronts;
+    k > n-1 leaves missing neighbours (index UINT64_MAX, -DBL_MAX / 0)."""
+    for ct in (O.CT_F64, O.CT_U64):
+        x, y, z = ref_points(O.GEN_UNIFORM, n, 777, ct)
+        tree, t = make(ft, ct, O.SFC_PH, 21, 0, (x, y, z))
+        for k in (1, 2, 4, 32, 33, 64):
+            for excl in (True, False):
+                for algo in (("packet", "grouped") if k <= 32 else ("grouped",)):
+                    r, d = ft.self_knn_query(tree, k, exclude_self=excl, max_k=256,
+                                             algorithm=ALGOS[algo])
+                    rb, db = O.knn_bruteforce(t, k, excl)
+                    assert np.array_equal(d, db) and np.array_equal(r, rb), (ct, k, excl, algo)
+                r, d = ft.self_knn_query(tree, k, exclude_self=excl, max_k=256,
+                                         mode=ft.MODE_REFERENCE)
+                rr, dr = O.knn_reference(t, k, excl, max_k=256)
+                assert np.array_equal(d, dr) and np.array_equal(r, rr)
+
+
+def test_large_sample_against_bruteforce(ft, orc):
+    """300k clustered int64 periodic particles, k=32: all queries on the GPU,
+    a 3000-query sample checked against brute force."""
+    pts = nfw_points(300_000, 13, O.CT_U64, halos=2000)
+    tree, t = make(ft, O.CT_U64, O.SFC_PH, 21, 1, pts)
+    ids = np.random.default_rng(5).choice(300_000, 3000, replace=False).astype(np.int32)
+    rb, db = O.knn_bruteforce(t, 32, True, leaf_ids=ids)
+    for algo in ("packet", "grouped", "plain"):
+        r, d = ft.self_knn_query(tree, 32, algorithm=ALGOS[algo])
+        assert np.array_equal(d[ids], db) and np.array_equal(r[ids], rb), algo
+
+
+def test_seeded_bound_never_changes_results(ft, orc):
+    """A per-query seed bound (too small for some queries, loose for others)
+    gives exactly the unbounded results; too-small seeds are re-run."""
+    pts = nfw_points(40000, 17, O.CT_U64)
+    tree, t = make(ft, O.CT_U64, O.SFC_PH, 21, 1, pts)
+    k = 16
+    rb, db = O.knn_bruteforce(t, k, True)
+    true_r2 = db[:, -1].astype(np.float64)
+    rng = np.random.default_rng(2)
+    scale = rng.choice([0.25, 0.9, 1.0, 1.5, 4.0], size=len(true_r2))
+    seed = np.floor(true_r2 * scale).astype(np.uint64)
+    stats = ft.DeviceArray(8, np.uint64).zero()
+    r, d = ft.self_knn_query(tree, k, seed_bound=ft.DeviceArray.from_host(seed), stats=stats)
+    assert np.array_equal(d, db) and np.array_equal(r, rb)
+    st = stats.to_host()
+    assert st[3] == int(np.sum(seed < db[:, -1]))  # exactly the too-small seeds re-ran

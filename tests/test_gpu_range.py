@@ -1,0 +1,84 @@
+"""GPU parity of range_query (fixed stride + counts) and the CSR range query."""
+import numpy as np
+import pytest
+
+import oracle as O
+from fasttree_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def make(ft, ct, sfc, bits, periodic, pts, box=0.0):
+    x, y, z = pts
+    cfg = ft.Config(sfc=sfc, bits_per_dim=bits, periodic=periodic, box_length=box)
+    tree = ft.TreeSoA(len(x), ct, cfg)
+    ft.build_bvh(ft.particles(*(ft.DeviceArray.from_host(a) for a in (x, y, z))), tree)
+    t = O.build_bvh(ct, x, y, z, sfc=sfc, bits=bits, periodic=periodic, box=box)
+    return tree, t
+
+
+@pytest.mark.parametrize("ct,periodic", [(O.CT_U64, 1), (O.CT_U64, 0), (O.CT_F64, 0),
+                                         (O.CT_U32, 0), (O.CT_F32, 0)])
+def test_range_matches_reference(ft, orc, ct, periodic):
+    x, y, z = synth.nfw_clustered(20000, 3, box=100.0, n_halos=100)
+    dt = O.COORD_DTYPES[ct]
+    if np.issubdtype(dt, np.integer):
+        pts = tuple(synth.to_fixed(a, 100.0, 21, dt) for a in (x, y, z))
+        rmax = np.full(500, 1 << 14, dt)
+        rmax[::3] = 1 << 16
+    else:
+        pts = tuple(a.astype(dt) for a in (x, y, z))
+        rmax = np.full(500, 0.8, dt)
+        rmax[::3] = 3.0
+    tree, t = make(ft, ct, O.SFC_PH, 21, periodic, pts)
+    q = tuple(a[::40].copy() for a in pts)
+    rmin = np.zeros(500, dt)
+    rmin[::5] = rmax[::5] // 2 if np.issubdtype(dt, np.integer) else rmax[::5] / 2
+    for maxr in (0, 7, 64, 4000):
+        res, cnt = ft.range_query(tree, *q, rmin, rmax, maxr)
+        rr, rc = O.range_reference(t, *q, rmin, rmax, maxr)
+        assert np.array_equal(cnt, rc)
+        for i in range(500):
+            m = min(cnt[i], maxr)
+            assert np.array_equal(res[i, :m], rr[i, :m])
+    rp, cols = ft.range_query_csr(tree, *q, rmin, rmax)
+    erp, ecols = O.range_csr(t, *q, rmin, rmax)
+    assert np.array_equal(rp, erp) and np.array_equal(cols, ecols)
+
+
+def test_range_grid_kats(ft, orc):
+    """test/main.cpp TEST 2 and TEST 4 (10^3 grid; non-periodic at (52,52,52),
+    r <= 20; periodic int64 at (2,50,50), r <= 10)."""
+    x, y, z = O.gen_reference(O.GEN_GRID, 1000, 0, O.CT_F64)
+    tree, t = make(ft, O.CT_F64, O.SFC_MORTON, 10, 0, (x, y, z))
+    res, cnt = ft.range_query(tree, [52.0], [52.0], [52.0], [0.0], [20.0], 1000)
+    rr, rc = O.range_reference(t, [52.0], [52.0], [52.0], [0.0], [20.0], 1000)
+    assert cnt[0] == rc[0] and np.array_equal(res[0, :cnt[0]], rr[0, :rc[0]])
+    xi, yi, zi = O.gen_reference(O.GEN_GRID, 1000, 0, O.CT_U64, bits=21)
+    tree, t = make(ft, O.CT_U64, O.SFC_PH, 21, 1, (xi, yi, zi))
+    s = float(1 << 21) / 100.0
+    q = ([np.uint64(int(2 * s))], [np.uint64(int(50 * s))], [np.uint64(int(50 * s))])
+    r = [np.uint64(int(10 * s))]
+    res, cnt = ft.range_query(tree, *q, [np.uint64(0)], r, 1000)
+    rr, rc = O.range_reference(t, *q, [np.uint64(0)], r, 1000)
+    assert cnt[0] == rc[0] and cnt[0] > 0
+    assert np.array_equal(res[0, :cnt[0]], rr[0, :rc[0]])
+
+
+def test_fp_periodic_range_and_knn(ft, orc):
+    """Double coordinates in a periodic box (no reference counterpart: the
+    reference ignores PERIODIC_BC for FP).  Pinned by brute force."""
+    L = 100.0
+    x, y, z = synth.nfw_clustered(15000, 4, box=L, n_halos=60)
+    tree, t = make(ft, O.CT_F64, O.SFC_PH, 21, 1, (x, y, z), box=L)
+    r, d = ft.self_knn_query(tree, 24)
+    rb, db = O.knn_bruteforce(t, 24, True)
+    assert np.array_equal(d, db) and np.array_equal(r, rb)
+    lx, ly, lz = t.leaf_coords()
+    q = (lx[::50].copy(), ly[::50].copy(), lz[::50].copy())
+    rad = np.sqrt(db[::50, -1]) * 1.2
+    rp, cols = ft.range_query_csr(tree, *q, np.zeros(len(rad)), rad)
+    erp, ecols = O.range_csr(t, *q, np.zeros(len(rad)), rad)
+    assert np.array_equal(rp, erp) and np.array_equal(cols, ecols)
+    # each query finds itself and at least its 24 nearest neighbours
+    assert np.all(np.diff(rp) >= 25)
