@@ -40,6 +40,10 @@ def main():
     ap.add_argument("--no-ladder", action="store_true", help="skip the plain/grouped kernels")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--bucket", type=int, default=0, help="packet leaf bucket (0 = auto)")
+    ap.add_argument("--sweep-buckets", action="store_true")
+    ap.add_argument("--seed-probe", action="store_true",
+                    help="time the packet kernel seeded with scaled exact k-th distances")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -69,7 +73,7 @@ def main():
         ft.build_bvh(p, tree, scratch=scratch, stream=stream.handle)
         ev[1].record(stream)
         ft.self_knn_query(tree, k, results=res, dists=dst, exclude_self=True,
-                          stream=stream.handle, algorithm=algorithm)
+                          stream=stream.handle, algorithm=algorithm, leaf_bucket=args.bucket)
         ev[2].record(stream)
         stream.synchronize()
         return ev[0].elapsed_ms(ev[1]), ev[1].elapsed_ms(ev[2])
@@ -108,6 +112,28 @@ def main():
     st = stats.to_host()
 
     ladder = {"packet_qps": qps}
+    if args.sweep_buckets:
+        for bk in (8, 16, 24, 32, 48, 64):
+            ev[0].record(stream)
+            ft.self_knn_query(tree, k, results=res, dists=dst, stream=stream.handle,
+                              algorithm=ft.ALGO_PACKET, leaf_bucket=bk)
+            ev[1].record(stream)
+            stream.synchronize()
+            log(f"bucket {bk}: {ev[0].elapsed_ms(ev[1]):.2f} ms")
+    if args.seed_probe:
+        dk = dst.to_host().reshape(n, k)[:, k - 1].astype(np.float64)
+        for f in (1.0, 1.2, 1.5, 2.0, 0.9):
+            sd = ft.DeviceArray.from_host(np.floor(dk * f).astype(np.uint64))
+            st2 = ft.DeviceArray(8, np.uint64).zero()
+            ev[0].record(stream)
+            ft.self_knn_query(tree, k, results=res, dists=dst, stream=stream.handle,
+                              seed_bound=sd, stats=st2, leaf_bucket=args.bucket)
+            ev[1].record(stream)
+            stream.synchronize()
+            s2 = st2.to_host()
+            log(f"seed x{f}: {ev[0].elapsed_ms(ev[1]):.2f} ms, nodes/wave "
+                f"{s2[0] / ((n + 63) // 64):.1f}, leaves/wave {s2[1] / ((n + 63) // 64):.1f}, "
+                f"inserts/q {s2[4] / n:.1f}, reruns {int(s2[3])}")
     if not args.no_ladder:
         for name, algo in (("grouped_qps", ft.ALGO_GROUPED), ("plain_qps", ft.ALGO_PLAIN)):
             ev[0].record(stream)

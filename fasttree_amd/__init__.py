@@ -74,7 +74,8 @@ class _Tree(C.Structure):
 
 class QueryOpts(C.Structure):
     _fields_ = [("max_k", C.c_int32), ("mode", C.c_int32), ("algorithm", C.c_int32),
-                ("group_width", C.c_int32), ("seed_bound", C.c_void_p), ("stats", C.c_void_p)]
+                ("group_width", C.c_int32), ("leaf_bucket", C.c_int32), ("reserved1", C.c_int32),
+                ("seed_bound", C.c_void_p), ("stats", C.c_void_p)]
 
 
 EXPORTS = [
@@ -432,10 +433,10 @@ def build_bvh(p: particles, tree: TreeSoA, scratch: BvhScratch | None = None, bb
 # ---------------------------------------------------------------------------
 # Queries
 # ---------------------------------------------------------------------------
-def _opts(max_k=0, mode=MODE_CANONICAL, algorithm=ALGO_AUTO, group_width=0, seed_bound=None,
-          stats=None):
-    return QueryOpts(int(max_k), int(mode), int(algorithm), int(group_width), _ptr(seed_bound),
-                     _ptr(stats))
+def _opts(max_k=0, mode=MODE_CANONICAL, algorithm=ALGO_AUTO, group_width=0, leaf_bucket=0,
+          seed_bound=None, stats=None):
+    return QueryOpts(int(max_k), int(mode), int(algorithm), int(group_width), int(leaf_bucket),
+                     0, _ptr(seed_bound), _ptr(stats))
 
 
 def _outputs(tree, nq, k, results, dists):
@@ -479,7 +480,7 @@ def knn_query(tree: TreeSoA, qx, qy, qz, k: int, results=None, dists=None, max_k
               stream=None, **opts):
     """knn_query<_MAX_K_=256> (src/hlbvh.hpp:1405-1472) over external points."""
     dt = COORD_DTYPES[tree.coord_type]
-    if isinstance(qx, np.ndarray):
+    if not isinstance(qx, DeviceArray):
         qx, qy, qz = (np.ascontiguousarray(a, dt) for a in (qx, qy, qz))
     nq = qx.n if isinstance(qx, DeviceArray) else len(qx)
     results, dists = _outputs(tree, nq, k, results, dists)
@@ -493,7 +494,7 @@ def range_query(tree: TreeSoA, qx, qy, qz, r_min, r_max, max_results_per_query: 
                 results=None, result_counts=None, stream=None):
     """range_query (src/hlbvh.hpp:2006-2136): fixed-stride indices, counts = total hits."""
     dt = COORD_DTYPES[tree.coord_type]
-    if isinstance(qx, np.ndarray):
+    if not isinstance(qx, DeviceArray):
         qx, qy, qz, r_min, r_max = (np.ascontiguousarray(a, dt) for a in (qx, qy, qz, r_min,
                                                                            r_max))
     nq = qx.n if isinstance(qx, DeviceArray) else len(qx)
@@ -512,7 +513,7 @@ def range_query(tree: TreeSoA, qx, qy, qz, r_min, r_max, max_results_per_query: 
 def range_query_csr(tree: TreeSoA, qx, qy, qz, r_min, r_max, stream=None):
     """Range query with counts and a CSR neighbour list (ascending leaf rank per row)."""
     dt = COORD_DTYPES[tree.coord_type]
-    if isinstance(qx, np.ndarray):
+    if not isinstance(qx, DeviceArray):
         qx, qy, qz, r_min, r_max = (np.ascontiguousarray(a, dt) for a in (qx, qy, qz, r_min,
                                                                            r_max))
     nq = qx.n if isinstance(qx, DeviceArray) else len(qx)

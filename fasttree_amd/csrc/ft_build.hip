@@ -486,9 +486,16 @@ __global__ __launch_bounds__(256) void karras_kernel(const uint64_t *__restrict_
 
 // ===========================================================================
 // K13: refit (src/hlbvh.hpp:925-969).  Each leaf climbs; the first arrival at a
-// node stops, the second (after an acquire-release counter increment at agent
-// scope) combines its own box with the sibling's (fmin/fmax for FP, </> for
-// integers, sfc_min/sfc_max :580-594) and continues.
+// node stops, the second combines its own box with the sibling's (fmin/fmax for
+// FP, </> for integers: sfc_min/sfc_max, :580-594) and continues.
+//
+// Cross-XCD visibility without whole-cache maintenance: internal boxes are
+// written with agent-scope stores (write-through to the device coherence
+// point), each climber waits for its own stores to complete (s_waitcnt
+// vmcnt(0)) before its counter increment, and the second arrival reads the
+// sibling with agent-scope loads issued after its increment returned.  This is
+// the agent-scope release/acquire pair minus the L2 write-back / invalidate,
+// which only non-atomic accesses need.
 // ===========================================================================
 template <typename T> __device__ inline T smin(T a, T b) {
     if constexpr (CoordTraits<T>::fp) return ffmin(a, b);
@@ -497,6 +504,12 @@ template <typename T> __device__ inline T smin(T a, T b) {
 template <typename T> __device__ inline T smax(T a, T b) {
     if constexpr (CoordTraits<T>::fp) return ffmax(a, b);
     else return (a < b) ? b : a;
+}
+template <typename T> __device__ inline T ld_agent(const T *p) {
+    return __hip_atomic_load(const_cast<T *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T> __device__ inline void st_agent(T *p, T v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 template <typename T>
@@ -512,15 +525,20 @@ __global__ __launch_bounds__(256) void refit_kernel(int n, const int32_t *__rest
     while (cur != 0) {
         const int p = parent[cur];
         if (p < 0 || p >= n - 1) break;
+        // This climber's box stores (previous iteration) complete before the increment.
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
         const uint32_t prev =
-            __hip_atomic_fetch_add(&counters[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(&counters[p], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (prev == 0) return;
+        asm volatile("" ::: "memory");
         const int l = left[p], r = right[p];
         const int sib = (l == cur) ? r : l;
-        b0 = smin(b0, mnx[sib]); b1 = smax(b1, mxx[sib]);
-        b2 = smin(b2, mny[sib]); b3 = smax(b3, mxy[sib]);
-        b4 = smin(b4, mnz[sib]); b5 = smax(b5, mxz[sib]);
-        mnx[p] = b0; mxx[p] = b1; mny[p] = b2; mxy[p] = b3; mnz[p] = b4; mxz[p] = b5;
+        b0 = smin(b0, ld_agent(mnx + sib)); b1 = smax(b1, ld_agent(mxx + sib));
+        b2 = smin(b2, ld_agent(mny + sib)); b3 = smax(b3, ld_agent(mxy + sib));
+        b4 = smin(b4, ld_agent(mnz + sib)); b5 = smax(b5, ld_agent(mxz + sib));
+        st_agent(mnx + p, b0); st_agent(mxx + p, b1);
+        st_agent(mny + p, b2); st_agent(mxy + p, b3);
+        st_agent(mnz + p, b4); st_agent(mxz + p, b5);
         cur = p;
     }
 }

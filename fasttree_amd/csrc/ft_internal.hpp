@@ -163,6 +163,44 @@ __host__ __device__ inline typename CoordTraits<T>::D point_d2(const Metric &m, 
     return box_d2<T>(m, px, py, pz, x, x, y, y, z, z);
 }
 
+// The same value as point_d2, computed with fewer operations:
+//  * FP: with lo == hi the face gap fmax(lo - p, fmax(0, p - hi)) is |p - x|
+//    exactly (negation is exact), periodic: min(|p - x|, L - |p - x|).
+//  * integer periodic: only the low B bits of (p - x) survive the wrap
+//    (shortest_periodic_distance shifts everything above them out), so the
+//    signed B-bit difference is formed in 32 bits; the coordinate mask of the
+//    query does not change those bits either.  Valid for B <= 31.
+//  * integer non-periodic: the general formula.
+template <typename T>
+__host__ __device__ inline typename CoordTraits<T>::D leaf_d2(const Metric &m, T px, T py, T pz,
+                                                              T x, T y, T z) {
+    if constexpr (CoordTraits<T>::fp) {
+        T dx = ffabs(px - x), dy = ffabs(py - y), dz = ffabs(pz - z);
+        if (m.periodic) {
+            const T L = (T)m.box;
+            dx = ffmin(dx, L - dx);
+            dy = ffmin(dy, L - dy);
+            dz = ffmin(dz, L - dz);
+        }
+        T s = dx * dx;
+        s = s + dy * dy;
+        s = s + dz * dz;
+        return (double)s;
+    } else {
+        if (m.periodic && m.bits <= 31) {
+            const int sh = 32 - m.bits;
+            auto g = [sh](T a, T b) -> uint64_t {
+                int32_t v = (int32_t)(((uint32_t)a - (uint32_t)b) << sh) >> sh;
+                return (uint64_t)(uint32_t)(v < 0 ? -v : v);
+            };
+            const uint64_t gx = g(px, x) >> m.dshift, gy = g(py, y) >> m.dshift,
+                           gz = g(pz, z) >> m.dshift;
+            return gx * gx + gy * gy + gz * gz;
+        }
+        return box_d2<T>(m, px, py, pz, x, x, y, y, z, z);
+    }
+}
+
 // Sentinel written for a missing neighbour: type_identity_max<dist_t>()
 // (src/hlbvh.hpp:59-69): -DBL_MAX for FP, 0 for integers.
 template <typename D> __host__ __device__ inline D missing_dist() {

@@ -43,6 +43,20 @@ __device__ inline float rl(float v, int l) { return __uint_as_float(rl32(__float
 
 __device__ inline int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// Per-lane source lane (ds_bpermute): lane i receives v from lane src_i.
+__device__ inline uint32_t shfl32(uint32_t v, int src) {
+    return (uint32_t)__builtin_amdgcn_ds_bpermute(src << 2, (int)v);
+}
+__device__ inline uint64_t shfl_any(uint64_t v, int src) {
+    return ((uint64_t)shfl32((uint32_t)(v >> 32), src) << 32) | shfl32((uint32_t)v, src);
+}
+__device__ inline uint32_t shfl_any(uint32_t v, int src) { return shfl32(v, src); }
+__device__ inline int32_t shfl_any(int32_t v, int src) { return (int32_t)shfl32((uint32_t)v, src); }
+__device__ inline double shfl_any(double v, int src) {
+    return __longlong_as_double((long long)shfl_any((uint64_t)__double_as_longlong(v), src));
+}
+__device__ inline float shfl_any(float v, int src) { return __uint_as_float(shfl32(__float_as_uint(v), src)); }
+
 __device__ inline void wave_add_stats(uint64_t *stats, int slot, uint64_t v) {
     if (!stats) return;
 #pragma unroll
@@ -99,27 +113,25 @@ template <typename D, int M> struct TopK {
     }
     __device__ inline D worst() const { return d[M - 1]; }
     __device__ inline bool beats_worst(D cd, int32_t ci) const {
-        return cd < d[M - 1] || (cd == d[M - 1] && ci < i[M - 1]);
+        return (cd < d[M - 1]) | ((cd == d[M - 1]) & (ci < i[M - 1]));
     }
-    // Precondition: (cd, ci) beats the worst entry.  Walks down from the worst
-    // slot keeping two comparison results live: slot s takes slot s-1 when the
-    // candidate is below slot s-1, the candidate when it falls between.
-    __device__ inline void insert(D cd, int32_t ci) {
-        bool lt_s = true;
+    // Predicated insertion as a compare-and-swap chain from the smallest slot:
+    // the carried entry swaps into every slot it is below, so it lands in order
+    // and the former worst drops off the top.  Every slot stays in its own
+    // registers; the comparisons are branch-free.
+    __device__ inline void insert(bool take, D cd, int32_t ci) {
 #pragma unroll
-        for (int s = M - 1; s >= 1; --s) {
-            const bool lt_p = cd < d[s - 1] || (cd == d[s - 1] && ci < i[s - 1]);
-            const D nd = lt_p ? d[s - 1] : (lt_s ? cd : d[s]);
-            const int32_t ni = lt_p ? i[s - 1] : (lt_s ? ci : i[s]);
-            d[s] = nd;
-            i[s] = ni;
-            lt_s = lt_p;
-        }
-        if (lt_s) {
-            d[0] = cd;
-            i[0] = ci;
+        for (int s = 0; s < M; ++s) {
+            const bool lt = take & ((cd < d[s]) | ((cd == d[s]) & (ci < i[s])));
+            const D td = d[s];
+            const int32_t ti = i[s];
+            d[s] = lt ? cd : td;
+            i[s] = lt ? ci : ti;
+            cd = lt ? td : cd;
+            ci = lt ? ti : ci;
         }
     }
+    __device__ inline D worst_d() const { return d[M - 1]; }
     // Number of live slots still holding the initial sentinel.
     __device__ inline int missing(int k) const {
         int c = 0;
@@ -184,10 +196,9 @@ __global__ __launch_bounds__(LANE_BLOCK) void lane_knn_reg_kernel(DTree<T> t, QS
             const int rank = leaf - (n - 1);
             if (excl && rank == self) continue;
             const int32_t ri = ret_index(t, leaf);
-            if (h.beats_worst(d2, ri)) {
-                h.insert(d2, ri);
-                ++inserts;
-            }
+            const bool take = h.beats_worst(d2, ri);
+            h.insert(take, d2, ri);
+            inserts += take ? 1 : 0;
         } else {
             const int l = t.left[node], r = t.right[node];
             const D ld = node_d2(t, px, py, pz, l);
@@ -476,134 +487,238 @@ __global__ __launch_bounds__(64) void merge_knn_kernel(DTree<T> t, QSrc<T> qs, i
     }
 }
 
+// ---------------------------------------------------------------------------
+// Packed top-k for integer distances: key = min(d2, SAT) << ib | index, one
+// 64-bit compare orders by (d2, index).  ib = bit length of n, so every index
+// is below 2^ib - 1 and an all-ones index marks a sentinel.  A key whose d2
+// part saturated cannot be ordered exactly; the traversal reports such queries
+// (and queries left with sentinels) for an exact re-run.
+// ---------------------------------------------------------------------------
+template <int M> struct TopKPacked {
+    uint64_t key[M];
+    int ib;
+    uint64_t sat, imask;
+    __device__ inline void init(int k, uint64_t bound, int ibits) {
+        ib = ibits;
+        sat = (1ull << (64 - ib)) - 1;
+        imask = (1ull << ib) - 1;
+        const uint64_t sent = ((bound < sat ? bound : sat) << ib) | imask;
+#pragma unroll
+        for (int s = 0; s < M; ++s) key[s] = (s >= M - k) ? sent : 0ull;
+    }
+    __device__ inline uint64_t make(uint64_t d, int32_t i) const {
+        return ((d < sat ? d : sat) << ib) | (uint64_t)(uint32_t)i;
+    }
+    __device__ inline uint64_t worst_d() const { return key[M - 1] >> ib; }
+    __device__ inline bool beats_worst(uint64_t d, int32_t i) const { return make(d, i) < key[M - 1]; }
+    __device__ inline void insert(bool take, uint64_t d, int32_t i) {
+        uint64_t c = make(d, i);
+#pragma unroll
+        for (int s = 0; s < M; ++s) {
+            const bool lt = take & (c < key[s]);
+            const uint64_t t = key[s];
+            key[s] = lt ? c : t;
+            c = lt ? t : c;
+        }
+    }
+    // Sentinels left in live slots, or a saturated k-th distance.
+    __device__ inline bool needs_exact(int k) const {
+        bool bad = (key[M - 1] >> ib) >= sat;
+#pragma unroll
+        for (int s = 0; s < M; ++s) bad |= (s >= M - k) & ((key[s] & imask) == imask);
+        return bad;
+    }
+    __device__ inline bool missing(int k) const {
+        bool bad = false;
+#pragma unroll
+        for (int s = 0; s < M; ++s) bad |= (s >= M - k) & ((key[s] & imask) == imask);
+        return bad;
+    }
+    template <typename DOut>
+    __device__ inline void store(int k, uint64_t *res, DOut *dists, size_t q) const {
+        const size_t o = q * (size_t)k;
+#pragma unroll
+        for (int s = 0; s < M; ++s) {
+            if (s >= M - k) {
+                const size_t j = o + (size_t)(s - (M - k));
+                const bool ok = (key[s] & imask) != imask;
+                res[j] = ok ? (key[s] & imask) : ~0ull;
+                dists[j] = ok ? (DOut)(key[s] >> ib) : missing_dist<DOut>();
+            }
+        }
+    }
+};
+
+template <typename D, int M, bool PACKED> struct TopKSel { using type = TopK<D, M>; };
+template <int M> struct TopKSel<uint64_t, M, true> { using type = TopKPacked<M>; };
+
 // ===========================================================================
 // Packet traversal: a wavefront of 64 queries shares one DFS.  A node is
 // expanded when some lane's box distance does not exceed that lane's current
 // k-th distance; leaf children are tested at once; a subtree with at most
 // `bucket` leaves is scanned as a contiguous run of leaf slots.  Node data is
-// wave-uniform (scalar loads), the stack lives in LDS, results are canonical.
-// An optional per-query seed bound starts each lane's k-th distance at the
-// seed; a lane that ends with fewer than k neighbours inside it is appended to
-// a re-run list (its seeded results are discarded).
+// wave-uniform, the stack lives in LDS, results are canonical.
+//
+// Query mapping: direct (lane -> query blockIdx*64+lane) or LISTED (each wave
+// takes 64 entries of a device list of query positions; the list length is
+// read on the device, so no host round trip is needed).
+// SEEDED: each lane's k-th distance starts at its seed bound; a lane that ends
+// with fewer than k neighbours inside the seed is appended to the re-run list.
+// PACKED: 64-bit (d2, index) keys; queries the packed form cannot order
+// exactly are appended to the re-run list too.  The re-run is LISTED,
+// unseeded and lexicographic, so its results are exact.
 // ===========================================================================
 constexpr int PK_STACK = 128;
 
-template <typename T, int M, bool SEEDED>
+struct PacketArgs {
+    int nq, k, excl, bucket, ibits, pad;
+    const int32_t *list;        // LISTED: query positions
+    const uint32_t *list_count; // LISTED: number of entries
+    int32_t *rerun;             // re-run list (SEEDED or PACKED)
+    uint32_t *rerun_count;
+    uint64_t *stats;
+};
+
+template <typename T, int M, bool SEEDED, bool PACKED, bool LISTED>
 __global__ __launch_bounds__(64) void packet_knn_kernel(
-    DTree<T> t, QSrc<T> qs, int nq, int k, int excl, int bucket,
-    const typename CoordTraits<T>::D *__restrict__ seed, uint64_t *__restrict__ res,
-    typename CoordTraits<T>::D *__restrict__ dists, int32_t *__restrict__ rerun,
-    uint32_t *__restrict__ rerun_count, uint64_t *__restrict__ stats) {
+    DTree<T> t, QSrc<T> qs, PacketArgs a, const typename CoordTraits<T>::D *__restrict__ seed,
+    uint64_t *__restrict__ res, typename CoordTraits<T>::D *__restrict__ dists) {
     using D = typename CoordTraits<T>::D;
+    using H = typename TopKSel<D, M, PACKED>::type;
     __shared__ int32_t stk[PK_STACK];
     const int lane = threadIdx.x;
-    const int q = blockIdx.x * 64 + lane;
-    const bool active = q < nq;
     const int n = t.n;
     const int off = n - 1;
     const T *lx = t.mnx + off;
     const T *ly = t.mny + off;
     const T *lz = t.mnz + off;
-    T px = T(0), py = T(0), pz = T(0);
-    int self = -1;
-    if (active) load_query(t, qs, q, px, py, pz, self);
-    if (!excl) self = -1;
-    TopK<D, M> h;
-    h.init(k, (SEEDED && active) ? seed[q] : dist_max<D>());
+    const int k = a.k;
+    const int nlist = LISTED ? (int)*a.list_count : a.nq;
     uint64_t nodes = 0, leaves = 0, inserts = 0, overflow = 0;
 
-    // Scans leaf ranks [first, first+cnt) (cnt <= 64) for every lane: one
-    // coalesced load, then each leaf is broadcast from its lane.
-    auto scan_range = [&](int first, int cnt) {
-        T bx = T(0), by = T(0), bz = T(0);
-        int32_t bo = 0;
-        if (lane < cnt) {
-            bx = lx[first + lane];
-            by = ly[first + lane];
-            bz = lz[first + lane];
-            bo = t.orig_indices ? t.orig[off + first + lane] : first + lane;
-        }
-        for (int j = 0; j < cnt; ++j) {
-            const T x = rl(bx, j), y = rl(by, j), z = rl(bz, j);
-            const int32_t ri = rl(bo, j);
-            const D d = point_d2<T>(t.m, px, py, pz, x, y, z);
-            const bool take = active && (first + j) != self && h.beats_worst(d, ri);
-            if (take) {
-                h.insert(d, ri);
-                ++inserts;
-            }
-        }
-        leaves += (uint64_t)cnt;
-    };
+    for (int base = blockIdx.x * 64; base < nlist; base += (LISTED ? gridDim.x * 64 : nlist)) {
+        const int slot = base + lane;
+        const bool active = slot < nlist;
+        const int q = active ? (LISTED ? a.list[slot] : slot) : -1;
+        T px = T(0), py = T(0), pz = T(0);
+        int self = -1;
+        if (active) load_query(t, qs, q, px, py, pz, self);
+        if (!a.excl) self = -1;
+        H h;
+        if constexpr (PACKED) h.init(k, (SEEDED && active) ? (uint64_t)seed[q] : ~0ull, a.ibits);
+        else h.init(k, (SEEDED && active) ? seed[q] : dist_max<D>());
 
-    int sp = 0;
-    int node = 0;  // an internal node some lane needs (or the single leaf when n == 1)
-    while (true) {
-        ++nodes;
-        int scan_first = 0, scan_cnt = 0, next = -1;
-        if (n == 1) {
-            scan_cnt = 1;
-        } else {
-            const int first = uni(t.rfirst[node]);
-            const int last = uni(t.rlast[node]);
-            if (last - first + 1 <= bucket) {
-                scan_first = first;
-                scan_cnt = last - first + 1;
+        // Scans leaf ranks [first, first+cnt) (cnt <= 64).  One coalesced load
+        // puts leaf first+j in lane j.  Phase 1 broadcasts each leaf and marks,
+        // per lane, the leaves not farther than its current k-th distance;
+        // phase 2 inserts the marked leaves in rounds (each lane fetches its own
+        // next candidate from the holding lane), so the insertion chain runs
+        // once per round instead of once per leaf.
+        auto scan_range = [&](int first, int cnt) {
+            T bx = T(0), by = T(0), bz = T(0);
+            int32_t bo = 0;
+            if (lane < cnt) {
+                bx = lx[first + lane];
+                by = ly[first + lane];
+                bz = lz[first + lane];
+                bo = t.orig_indices ? t.orig[off + first + lane] : first + lane;
+            }
+            const D w = (D)h.worst_d();
+            uint64_t cand = 0;
+            for (int j = 0; j < cnt; ++j) {
+                const D d = leaf_d2<T>(t.m, px, py, pz, rl(bx, j), rl(by, j), rl(bz, j));
+                cand |= (uint64_t)(d <= w) << j;
+            }
+            if (!active) cand = 0;
+            const int sr = self - first;
+            if (sr >= 0 && sr < cnt) cand &= ~(1ull << sr);
+            leaves += (uint64_t)cnt;
+            while (__any(cand != 0)) {
+                const bool has = cand != 0;
+                const int j = has ? (__ffsll((long long)cand) - 1) : 0;
+                cand &= cand - 1;
+                const T x = shfl_any(bx, j), y = shfl_any(by, j), z = shfl_any(bz, j);
+                const int32_t ri = shfl_any(bo, j);
+                const D d = leaf_d2<T>(t.m, px, py, pz, x, y, z);
+                const bool take = has && h.beats_worst(d, ri);
+                h.insert(take, d, ri);
+                inserts += take ? 1 : 0;
+            }
+        };
+
+        int sp = 0;
+        int node = 0;  // an internal node some lane needs (or the single leaf when n == 1)
+        while (true) {
+            ++nodes;
+            int scan_first = 0, scan_cnt = 0, next = -1;
+            if (n == 1) {
+                scan_cnt = 1;
             } else {
-                const int l = uni(t.left[node]);
-                const int r = uni(t.right[node]);
-                const D ld = node_d2(t, px, py, pz, l);
-                const D rd = node_d2(t, px, py, pz, r);
-                const D w = h.worst();
-                const bool nl = active && ld <= w;
-                const bool nr = active && rd <= w;
-                const bool anyl = __any(nl), anyr = __any(nr);
-                // With more than `bucket` >= 2 leaves below, at most one child is a leaf.
-                if (l >= off) {
-                    if (anyl) { scan_first = l - off; scan_cnt = 1; }
-                } else if (r >= off) {
-                    if (anyr) { scan_first = r - off; scan_cnt = 1; }
-                }
-                const bool gl = l < off && anyl;
-                const bool gr = r < off && anyr;
-                if (gl && gr) {
-                    // Nearer child first, by vote of the lanes needing either.
-                    const uint64_t needing = __ballot(nl || nr);
-                    const uint64_t lfirst = __ballot((nl || nr) && ld <= rd);
-                    const bool left_first = 2 * __popcll(lfirst) >= __popcll(needing);
-                    next = left_first ? l : r;
-                    if (sp < PK_STACK) stk[sp++] = left_first ? r : l;
-                    else ++overflow;
-                } else if (gl) {
-                    next = l;
-                } else if (gr) {
-                    next = r;
+                const int first = uni(t.rfirst[node]);
+                const int last = uni(t.rlast[node]);
+                if (last - first + 1 <= a.bucket) {
+                    scan_first = first;
+                    scan_cnt = last - first + 1;
+                } else {
+                    const int l = uni(t.left[node]);
+                    const int r = uni(t.right[node]);
+                    const D ld = node_d2(t, px, py, pz, l);
+                    const D rd = node_d2(t, px, py, pz, r);
+                    const D w = (D)h.worst_d();
+                    const bool nl = active && ld <= w;
+                    const bool nr = active && rd <= w;
+                    const bool anyl = __any(nl), anyr = __any(nr);
+                    // With more than `bucket` >= 2 leaves below, at most one child is a leaf.
+                    if (l >= off) {
+                        if (anyl) { scan_first = l - off; scan_cnt = 1; }
+                    } else if (r >= off) {
+                        if (anyr) { scan_first = r - off; scan_cnt = 1; }
+                    }
+                    const bool gl = l < off && anyl;
+                    const bool gr = r < off && anyr;
+                    if (gl && gr) {
+                        // Nearer child first, by vote of the lanes needing either.
+                        const uint64_t needing = __ballot(nl || nr);
+                        const uint64_t lfirst = __ballot((nl || nr) && ld <= rd);
+                        const bool left_first = 2 * __popcll(lfirst) >= __popcll(needing);
+                        next = left_first ? l : r;
+                        if (sp < PK_STACK) stk[sp++] = left_first ? r : l;
+                        else ++overflow;
+                    } else if (gl) {
+                        next = l;
+                    } else if (gr) {
+                        next = r;
+                    }
                 }
             }
+            if (scan_cnt > 0) scan_range(scan_first, scan_cnt);
+            while (next < 0 && sp > 0) {
+                const int c = uni(stk[--sp]);
+                const D dc = node_d2(t, px, py, pz, c);
+                if (__any(active && dc <= (D)h.worst_d())) next = c;
+            }
+            if (next < 0) break;
+            node = next;
         }
-        if (scan_cnt > 0) scan_range(scan_first, scan_cnt);
-        while (next < 0 && sp > 0) {
-            const int c = uni(stk[--sp]);
-            const D dc = node_d2(t, px, py, pz, c);
-            if (__any(active && dc <= h.worst())) next = c;
+        if (active) {
+            bool redo = false;
+            if constexpr (PACKED) redo = h.needs_exact(k);
+            else if constexpr (SEEDED) redo = h.missing(k) > 0;
+            if (redo) {
+                const uint32_t s2 = atomicAdd(a.rerun_count, 1u);
+                a.rerun[s2] = q;
+            } else {
+                h.store(k, res, dists, (size_t)q);
+            }
         }
-        if (next < 0) break;
-        node = next;
+        if (!LISTED) break;
     }
-    if (active) {
-        if (SEEDED && h.missing(k) > 0) {
-            const uint32_t slot = atomicAdd(rerun_count, 1u);
-            rerun[slot] = q;
-        } else {
-            h.store(k, res, dists, (size_t)q);
-        }
+    if (lane == 0 && a.stats) {
+        atomicAdd((unsigned long long *)&a.stats[0], (unsigned long long)nodes);
+        atomicAdd((unsigned long long *)&a.stats[1], (unsigned long long)leaves);
+        if (overflow) atomicAdd((unsigned long long *)&a.stats[2], (unsigned long long)overflow);
     }
-    if (lane == 0 && stats) {
-        atomicAdd((unsigned long long *)&stats[0], (unsigned long long)nodes);
-        atomicAdd((unsigned long long *)&stats[1], (unsigned long long)leaves);
-        if (overflow) atomicAdd((unsigned long long *)&stats[2], (unsigned long long)overflow);
-    }
-    wave_add_stats(stats, 4, inserts);
+    wave_add_stats(a.stats, 4, inserts);
 }
 
 // ===========================================================================
@@ -761,27 +876,10 @@ static int launch_lane_lds(hipStream_t s, const DTree<T> &dt, const QSrc<T> &qs,
     return FT_OK;
 }
 
-template <typename T, bool SEEDED>
-static int launch_packet(hipStream_t s, const DTree<T> &dt, const QSrc<T> &qs, int nq, int k,
-                         int excl, int bucket, const typename CoordTraits<T>::D *seed,
-                         uint64_t *res, typename CoordTraits<T>::D *d, int32_t *rerun,
-                         uint32_t *rerun_count, uint64_t *stats) {
-    const dim3 g(blocks_for(nq, 64)), b(64);
-#define FT_PK(MK) hipLaunchKernelGGL((packet_knn_kernel<T, MK, SEEDED>), g, b, 0, s, dt, qs, nq, k, excl, bucket, seed, res, d, rerun, rerun_count, stats)
-    if (k <= 1) FT_PK(1);
-    else if (k <= 4) FT_PK(4);
-    else if (k <= 8) FT_PK(8);
-    else if (k <= 16) FT_PK(16);
-    else FT_PK(32);
-#undef FT_PK
-    FT_CHECK_LAUNCH();
-    return FT_OK;
-}
-
-// Device buffer for re-run lists, grown on demand (per process).
+// Device buffer for re-run lists, grown on demand (one per process).
 struct RerunBuf {
     int32_t *list = nullptr;
-    uint32_t *count = nullptr;
+    uint32_t *count = nullptr;  // [0] first pass, [1] stats accumulator scratch
     size_t cap = 0;
 };
 static RerunBuf g_rerun;
@@ -789,6 +887,7 @@ static RerunBuf g_rerun;
 static int rerun_ensure(size_t n) {
     if (n <= g_rerun.cap && g_rerun.list) return FT_OK;
     if (g_rerun.list) (void)hipFree(g_rerun.list);
+    g_rerun.list = nullptr;
     const size_t cap = n + n / 2 + 1024;
     void *p = nullptr;
     if (hipMalloc(&p, cap * 4 + 256) != hipSuccess) return fail(FT_ENOMEM, "re-run list allocation failed");
@@ -799,37 +898,83 @@ static int rerun_ensure(size_t n) {
 }
 
 __global__ void zero_u32_kernel(uint32_t *p) { *p = 0; }
-__global__ void add_u64_kernel(uint64_t *p, uint64_t v) { *p += v; }
+__global__ void add_count_kernel(uint64_t *stats, const uint32_t *count) { stats[3] += *count; }
 
-// Gathers the query source of the listed queries into a compact subset.
-template <typename T>
-__global__ void rerun_gather_kernel(QSrc<T> qs, const int32_t *__restrict__ list,
-                                    const uint32_t *__restrict__ count, int n, T *gx, T *gy, T *gz,
-                                    int32_t *gids) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (int)*count) return;
-    const int q = list[i];
-    if (qs.qx) {
-        gx[i] = qs.qx[q]; gy[i] = qs.qy[q]; gz[i] = qs.qz[q];
-    } else {
-        gids[i] = qs.leaf_ids ? qs.leaf_ids[q] : q;
+static inline int bit_length(uint64_t v) {
+    int b = 0;
+    while (v) {
+        ++b;
+        v >>= 1;
     }
-    (void)n;
+    return b;
 }
 
-// Copies the compact re-run results back to their query positions.
-template <typename D>
-__global__ void rerun_scatter_kernel(const int32_t *__restrict__ list,
-                                     const uint32_t *__restrict__ count, int k,
-                                     const uint64_t *__restrict__ cres, const D *__restrict__ cd,
-                                     uint64_t *__restrict__ res, D *__restrict__ d) {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const size_t total = (size_t)(*count) * k;
-    if (i >= total) return;
-    const size_t r = i / k, j = i % k;
-    const size_t dst = (size_t)list[r] * k + j;
-    res[dst] = cres[i];
-    d[dst] = cd[i];
+template <typename T, int M, bool SEEDED, bool PACKED, bool LISTED>
+static void launch_pk(hipStream_t s, dim3 g, const DTree<T> &dt, const QSrc<T> &qs,
+                      const PacketArgs &a, const typename CoordTraits<T>::D *seed, uint64_t *res,
+                      typename CoordTraits<T>::D *d) {
+    hipLaunchKernelGGL((packet_knn_kernel<T, M, SEEDED, PACKED, LISTED>), g, dim3(64), 0, s, dt,
+                       qs, a, seed, res, d);
+}
+
+template <typename T, int M>
+static int packet_run(hipStream_t s, const DTree<T> &dt, const QSrc<T> &qs, int nq, int k,
+                      int excl, int bucket, const typename CoordTraits<T>::D *seed, uint64_t *res,
+                      typename CoordTraits<T>::D *d, uint64_t *stats) {
+    constexpr bool INTD = !CoordTraits<T>::fp;
+    PacketArgs a{};
+    a.nq = nq;
+    a.k = k;
+    a.excl = excl;
+    a.bucket = bucket;
+    a.ibits = bit_length((uint64_t)dt.n);
+    a.stats = stats;
+    const bool need_list = seed != nullptr || INTD;
+    if (need_list) {
+        FT_TRY(rerun_ensure((size_t)nq));
+        hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(1), 0, s, g_rerun.count);
+        a.rerun = g_rerun.list;
+        a.rerun_count = g_rerun.count;
+    }
+    const dim3 g(blocks_for(nq, 64));
+    if (seed) {
+        if constexpr (INTD) launch_pk<T, M, true, true, false>(s, g, dt, qs, a, seed, res, d);
+        else launch_pk<T, M, true, false, false>(s, g, dt, qs, a, seed, res, d);
+    } else {
+        if constexpr (INTD) launch_pk<T, M, false, true, false>(s, g, dt, qs, a, seed, res, d);
+        else launch_pk<T, M, false, false, false>(s, g, dt, qs, a, seed, res, d);
+    }
+    FT_CHECK_LAUNCH();
+    if (need_list) {
+        // Exact re-run of the listed queries: unseeded, lexicographic keys, a
+        // fixed grid that walks the device-side list.
+        PacketArgs b = a;
+        b.list = g_rerun.list;
+        b.list_count = g_rerun.count;
+        b.rerun = nullptr;
+        b.rerun_count = nullptr;
+        b.stats = nullptr;
+        const unsigned waves = std::min<unsigned>(blocks_for(nq, 64), 2048u);
+        launch_pk<T, M, false, false, true>(s, dim3(waves), dt, qs, b, nullptr, res, d);
+        FT_CHECK_LAUNCH();
+        if (stats) {
+            hipLaunchKernelGGL(add_count_kernel, dim3(1), dim3(1), 0, s, stats,
+                               (const uint32_t *)g_rerun.count);
+            FT_CHECK_LAUNCH();
+        }
+    }
+    return FT_OK;
+}
+
+template <typename T>
+static int launch_packet(hipStream_t s, const DTree<T> &dt, const QSrc<T> &qs, int nq, int k,
+                         int excl, int bucket, const typename CoordTraits<T>::D *seed,
+                         uint64_t *res, typename CoordTraits<T>::D *d, uint64_t *stats) {
+    if (k <= 1) return packet_run<T, 1>(s, dt, qs, nq, k, excl, bucket, seed, res, d, stats);
+    if (k <= 4) return packet_run<T, 4>(s, dt, qs, nq, k, excl, bucket, seed, res, d, stats);
+    if (k <= 8) return packet_run<T, 8>(s, dt, qs, nq, k, excl, bucket, seed, res, d, stats);
+    if (k <= 16) return packet_run<T, 16>(s, dt, qs, nq, k, excl, bucket, seed, res, d, stats);
+    return packet_run<T, 32>(s, dt, qs, nq, k, excl, bucket, seed, res, d, stats);
 }
 
 template <typename T>
@@ -866,51 +1011,9 @@ static int knn_dispatch(hipStream_t s, const ft_tree *tree, const QSrc<T> &qs, i
     if (algo == FT_ALGO_AUTO) algo = (k <= 32) ? FT_ALGO_PACKET : FT_ALGO_GROUPED;
     if (o.seed_bound) algo = FT_ALGO_PACKET;
     if (algo == FT_ALGO_PACKET && k <= 32) {
-        const int bucket = o.group_width > 0 ? std::min(o.group_width, 64) : 32;
-        if (!o.seed_bound)
-            return launch_packet<T, false>(s, dt, qs, nq, k, excl, bucket, nullptr, res, dists,
-                                           nullptr, nullptr, stats);
-        FT_TRY(rerun_ensure((size_t)nq));
-        hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(1), 0, s, g_rerun.count);
-        FT_TRY(launch_packet<T, true>(s, dt, qs, nq, k, excl, bucket, (const D *)o.seed_bound,
-                                      res, dists, g_rerun.list, g_rerun.count, stats));
-        // Re-run the queries whose seeded search found fewer than k neighbours.
-        uint32_t nre = 0;
-        FT_HIP(hipMemcpyAsync(&nre, g_rerun.count, 4, hipMemcpyDeviceToHost, s));
-        FT_HIP(hipStreamSynchronize(s));
-        if (nre == 0) return FT_OK;
-        void *tmp = nullptr;
-        const size_t bytes = (size_t)nre * (3 * sizeof(T) + 4) + (size_t)nre * k * (8 + sizeof(D)) + 1024;
-        if (hipMallocAsync(&tmp, bytes, s) != hipSuccess) return fail(FT_ENOMEM, "re-run buffer");
-        char *p = (char *)tmp;
-        T *gx = (T *)p; p += ((size_t)nre * sizeof(T) + 15) & ~15ull;
-        T *gy = (T *)p; p += ((size_t)nre * sizeof(T) + 15) & ~15ull;
-        T *gz = (T *)p; p += ((size_t)nre * sizeof(T) + 15) & ~15ull;
-        int32_t *gids = (int32_t *)p; p += ((size_t)nre * 4 + 15) & ~15ull;
-        uint64_t *cres = (uint64_t *)p; p += (size_t)nre * k * 8;
-        D *cd = (D *)p;
-        hipLaunchKernelGGL(rerun_gather_kernel<T>, dim3(blocks_for(nre, 256)), dim3(256), 0, s,
-                           qs, (const int32_t *)g_rerun.list, (const uint32_t *)g_rerun.count,
-                           tree->num_leaves, gx, gy, gz, gids);
-        QSrc<T> sub{};
-        if (qs.qx) {
-            sub.qx = gx; sub.qy = gy; sub.qz = gz;
-        } else {
-            sub.leaf_ids = gids;
-        }
-        FT_TRY(launch_packet<T, false>(s, dt, sub, (int)nre, k, excl, bucket, nullptr, cres, cd,
-                                       nullptr, nullptr, nullptr));
-        hipLaunchKernelGGL(rerun_scatter_kernel<D>, dim3(blocks_for((size_t)nre * k, 256)),
-                           dim3(256), 0, s, (const int32_t *)g_rerun.list,
-                           (const uint32_t *)g_rerun.count, k, (const uint64_t *)cres,
-                           (const D *)cd, res, dists);
-        FT_CHECK_LAUNCH();
-        if (stats) {
-            hipLaunchKernelGGL(add_u64_kernel, dim3(1), dim3(1), 0, s, stats + 3,
-                               (uint64_t)nre);
-        }
-        FT_HIP(hipFreeAsync(tmp, s));
-        return FT_OK;
+        const int bucket = o.leaf_bucket > 0 ? std::min(o.leaf_bucket, 64) : 32;
+        return launch_packet<T>(s, dt, qs, nq, k, excl, bucket, (const D *)o.seed_bound, res,
+                                dists, stats);
     }
     if (k <= 32) {
         if (algo == FT_ALGO_PLAIN)

@@ -110,7 +110,10 @@ typedef struct ft_query_opts {
                          /* point's reference default (32 self, 256 subset/knn)          */
     int32_t mode;        /* FT_MODE_*                                                   */
     int32_t algorithm;   /* FT_ALGO_*                                                   */
-    int32_t group_width; /* lanes that share a traversal (FT_ALGO_PACKET): 0 = auto      */
+    int32_t group_width; /* lanes that share a traversal (FT_ALGO_PACKET: 64): 0 = auto   */
+    int32_t leaf_bucket; /* FT_ALGO_PACKET: subtrees with at most this many leaves are    */
+                         /* scanned as one contiguous run of leaf slots (1..64, 0 = auto) */
+    int32_t reserved1;
     const void *seed_bound; /* optional per-query squared-distance bound (dist_t[num_q]):*/
                             /* the search is first limited to it; queries that find     */
                             /* fewer than k neighbours inside it are re-run unbounded.  */
