@@ -167,14 +167,19 @@ __device__ inline uint64_t ph_encode(uint32_t ix, uint32_t iy, uint32_t iz, int 
     return key;
 }
 
+// Also accumulates the radix-sort digit histograms of every pass (8-bit
+// digits) so the sort does not read the keys once more just to count them.
 template <typename T>
 __global__ __launch_bounds__(256) void encode_kernel(const T *__restrict__ x,
                                                      const T *__restrict__ y,
                                                      const T *__restrict__ z, size_t n, int B,
                                                      int sfc, const BboxDev<T> *__restrict__ bb,
-                                                     uint64_t *__restrict__ keys) {
+                                                     uint64_t *__restrict__ keys, int passes,
+                                                     uint32_t *__restrict__ hist) {
     __shared__ uint16_t tab[384];
+    __shared__ uint32_t h[8][256];
     for (int i = threadIdx.x; i < 384; i += blockDim.x) tab[i] = c_ph_table[i];
+    for (int i = threadIdx.x; i < 8 * 256; i += blockDim.x) (&h[0][0])[i] = 0;
     __syncthreads();
     T lo[3] = {T(0), T(0), T(0)}, inv[3] = {T(0), T(0), T(0)};
     if constexpr (CoordTraits<T>::fp) {
@@ -202,6 +207,15 @@ __global__ __launch_bounds__(256) void encode_kernel(const T *__restrict__ x,
         if (sfc == FT_SFC_MORTON) key = spread3_21(q[0]) | (spread3_21(q[1]) << 1) | (spread3_21(q[2]) << 2);
         else key = ph_encode(q[0], q[1], q[2], B, tab);
         keys[i] = key;
+        if (hist)
+            for (int p = 0; p < passes; ++p) atomicAdd(&h[p][(key >> (8 * p)) & 255u], 1u);
+    }
+    if (hist) {
+        __syncthreads();
+        for (int i = threadIdx.x; i < passes * 256; i += blockDim.x) {
+            const uint32_t v = (&h[0][0])[i];
+            if (v) atomicAdd(&hist[i], v);
+        }
     }
 }
 
@@ -543,6 +557,129 @@ __global__ __launch_bounds__(256) void refit_kernel(int n, const int32_t *__rest
     }
 }
 
+// Chunked refit, pass 1: workgroup w owns leaves [a, a + REFIT_CHUNK).  Every
+// internal node whose leaf range lies inside the chunk has its Karras label in
+// the chunk too (a node's label is one end of its range), so the chunk's
+// topology (parent / children / ranges of labels [a, a+C)), its leaf points,
+// the arrival counters and the boxes all sit in LDS, loaded and stored with
+// coalesced accesses; the climb synchronises at workgroup scope only.  A
+// climber stops below the first node whose range leaves the chunk and records
+// the node it completed (a "chunk top") for pass 2.
+constexpr int REFIT_CHUNK = 512;
+
+template <typename T>
+__global__ __launch_bounds__(REFIT_CHUNK) void refit_chunk_kernel(
+    int n, const int32_t *__restrict__ left, const int32_t *__restrict__ right,
+    const int32_t *__restrict__ parent, const int32_t *__restrict__ rfirst,
+    const int32_t *__restrict__ rlast, T *mnx, T *mxx, T *mny, T *mxy, T *mnz, T *mxz,
+    int32_t *__restrict__ tops, uint32_t *__restrict__ top_count) {
+    constexpr int C = REFIT_CHUNK;
+    __shared__ T sbox[6][C];
+    __shared__ T spt[3][C];
+    __shared__ int32_t s_pin[C], s_l[C], s_r[C], s_rf[C], s_rl[C], s_pleaf[C];
+    __shared__ uint32_t scnt[C];
+    const int a = blockIdx.x * C;
+    const int hi = min(a + C, n);  // leaves [a, hi)
+    const int t = threadIdx.x;
+    const int lab = a + t;
+    const int off = n - 1;
+    if (lab < n - 1) {
+        s_pin[t] = parent[lab];
+        s_l[t] = left[lab];
+        s_r[t] = right[lab];
+        s_rf[t] = rfirst[lab];
+        s_rl[t] = rlast[lab];
+    }
+    T px = T(0), py = T(0), pz = T(0);
+    if (lab < n) {
+        s_pleaf[t] = parent[off + lab];
+        px = mnx[off + lab];
+        py = mny[off + lab];
+        pz = mnz[off + lab];
+        spt[0][t] = px;
+        spt[1][t] = py;
+        spt[2][t] = pz;
+    }
+    scnt[t] = 0;
+    __syncthreads();
+    if (lab < n) {
+        int cur = off + lab;
+        T b0 = px, b1 = px, b2 = py, b3 = py, b4 = pz, b5 = pz;
+        while (true) {
+            const int p = (cur >= off) ? s_pleaf[cur - off - a] : s_pin[cur - a];
+            if (p < 0) break;  // cur is the root
+            if (p < a || p >= hi || s_rf[p - a] < a || s_rl[p - a] >= hi) {
+                const uint32_t slot = atomicAdd(top_count, 1u);
+                tops[slot] = cur;
+                break;
+            }
+            const int pl = p - a;
+            const uint32_t prev =
+                __hip_atomic_fetch_add(&scnt[pl], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (prev == 0) break;
+            const int sib = (s_l[pl] == cur) ? s_r[pl] : s_l[pl];
+            if (sib >= off) {  // a leaf of this chunk: its box is the particle
+                const int sl = sib - off - a;
+                const T x = spt[0][sl], y = spt[1][sl], z = spt[2][sl];
+                b0 = smin(b0, x); b1 = smax(b1, x);
+                b2 = smin(b2, y); b3 = smax(b3, y);
+                b4 = smin(b4, z); b5 = smax(b5, z);
+            } else {
+                const int sl = sib - a;
+                b0 = smin(b0, sbox[0][sl]); b1 = smax(b1, sbox[1][sl]);
+                b2 = smin(b2, sbox[2][sl]); b3 = smax(b3, sbox[3][sl]);
+                b4 = smin(b4, sbox[4][sl]); b5 = smax(b5, sbox[5][sl]);
+            }
+            sbox[0][pl] = b0; sbox[1][pl] = b1; sbox[2][pl] = b2;
+            sbox[3][pl] = b3; sbox[4][pl] = b4; sbox[5][pl] = b5;
+            cur = p;
+        }
+    }
+    __syncthreads();
+    // Coalesced write-back of the nodes completed here (both children arrived).
+    if (lab < n - 1 && scnt[t] == 2u) {
+        mnx[lab] = sbox[0][t]; mxx[lab] = sbox[1][t];
+        mny[lab] = sbox[2][t]; mxy[lab] = sbox[3][t];
+        mnz[lab] = sbox[4][t]; mxz[lab] = sbox[5][t];
+    }
+}
+
+// Chunked refit, pass 2: the chunk tops climb through the nodes that span
+// chunks, with the agent-scope scheme of refit_kernel (write-through box
+// stores, store completion before the counter increment, agent-scope loads of
+// the sibling after it).  Boxes written by pass 1 are visible after the kernel
+// boundary.  A fixed grid walks the device-side list.
+template <typename T>
+__global__ __launch_bounds__(256) void refit_top_kernel(
+    int n, const int32_t *__restrict__ left, const int32_t *__restrict__ right,
+    const int32_t *__restrict__ parent, uint32_t *__restrict__ counters, T *mnx, T *mxx, T *mny,
+    T *mxy, T *mnz, T *mxz, const int32_t *__restrict__ tops, const uint32_t *__restrict__ top_count) {
+    const int count = (int)*top_count;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < count; t += gridDim.x * blockDim.x) {
+        int cur = tops[t];
+        T b0 = ld_agent(mnx + cur), b1 = ld_agent(mxx + cur), b2 = ld_agent(mny + cur);
+        T b3 = ld_agent(mxy + cur), b4 = ld_agent(mnz + cur), b5 = ld_agent(mxz + cur);
+        while (cur != 0) {
+            const int p = parent[cur];
+            if (p < 0 || p >= n - 1) break;
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+            const uint32_t prev =
+                __hip_atomic_fetch_add(&counters[p], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (prev == 0) break;
+            asm volatile("" ::: "memory");
+            const int l = left[p], r = right[p];
+            const int sib = (l == cur) ? r : l;
+            b0 = smin(b0, ld_agent(mnx + sib)); b1 = smax(b1, ld_agent(mxx + sib));
+            b2 = smin(b2, ld_agent(mny + sib)); b3 = smax(b3, ld_agent(mxy + sib));
+            b4 = smin(b4, ld_agent(mnz + sib)); b5 = smax(b5, ld_agent(mxz + sib));
+            st_agent(mnx + p, b0); st_agent(mxx + p, b1);
+            st_agent(mny + p, b2); st_agent(mxy + p, b3);
+            st_agent(mnz + p, b4); st_agent(mxz + p, b5);
+            cur = p;
+        }
+    }
+}
+
 __global__ void single_leaf_kernel(int32_t *parent) { parent[0] = -1; }
 
 // ===========================================================================
@@ -572,9 +709,11 @@ static int scratch_ensure(Scratch &S, size_t n) {
     const size_t o_vals_b = off; off += al(cap * 4);
     const size_t o_perm = off; off += al(cap * 4);
     const size_t o_ctr = off; off += al(cap * 4);
+    const size_t o_tops = off; off += al(cap * 4);
     const size_t o_hist = off; off += al(RS_MAX_PASSES * 256 * 4 * 2);
     const size_t o_look = off; off += al(tiles * 256 * 8);
     const size_t o_tile = off; off += al(64);
+    const size_t o_topc = off; off += al(64);
     const size_t o_bbox = off; off += al(16 * 8);
     const size_t o_part = off; off += al(1024 * 6 * 8);
     void *blk = nullptr;
@@ -592,6 +731,8 @@ static int scratch_ensure(Scratch &S, size_t n) {
     S.vals_b = (uint32_t *)(b + o_vals_b);
     S.perm = (uint32_t *)(b + o_perm);
     S.counters = (uint32_t *)(b + o_ctr);
+    S.tops = (int32_t *)(b + o_tops);
+    S.top_count = (uint32_t *)(b + o_topc);
     S.hist = (uint32_t *)(b + o_hist);
     S.lookback = (uint64_t *)(b + o_look);
     S.lookback_words = tiles * 256;
@@ -629,25 +770,32 @@ static int launch_bbox(hipStream_t s, Scratch &S, const T *x, const T *y, const 
 
 template <typename T>
 static int launch_encode(hipStream_t s, Scratch &S, const ft_config &cfg, const T *x, const T *y,
-                         const T *z, size_t n, uint64_t *keys) {
-    const unsigned g = grid_for(n, 256, 8192);
+                         const T *z, size_t n, uint64_t *keys, uint32_t *hist = nullptr,
+                         int passes = 0) {
+    const unsigned g = grid_for(n, 256, 4096);
     hipLaunchKernelGGL(encode_kernel<T>, dim3(g), dim3(256), 0, s, x, y, z, n, cfg.bits_per_dim,
-                       cfg.sfc, (const BboxDev<T> *)S.bbox_dev, keys);
+                       cfg.sfc, (const BboxDev<T> *)S.bbox_dev, keys, passes, hist);
     FT_CHECK_LAUNCH();
     return FT_OK;
 }
 
 // Sorts keys_in (n keys of `key_bits` significant bits) into sorted_out, and
 // the permutation into S.perm.
+static inline int radix_passes(int key_bits) {
+    const int p = (key_bits + 7) / 8;
+    return p < 1 ? 1 : p;
+}
+
 static int radix_sort(hipStream_t s, Scratch &S, const uint64_t *keys_in, size_t n, int key_bits,
-                      uint64_t *sorted_out) {
-    int passes = (key_bits + 7) / 8;
-    if (passes < 1) passes = 1;
-    hipLaunchKernelGGL(fill_u32_kernel, dim3(grid_for(RS_MAX_PASSES * 256, 256)), dim3(256), 0, s,
-                       S.hist, 0u, (size_t)(RS_MAX_PASSES * 256));
+                      uint64_t *sorted_out, bool hist_ready) {
+    const int passes = radix_passes(key_bits);
     uint32_t *base = S.hist + RS_MAX_PASSES * 256;
-    hipLaunchKernelGGL(radix_hist_kernel, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s, keys_in,
-                       n, passes, S.hist);
+    if (!hist_ready) {
+        hipLaunchKernelGGL(fill_u32_kernel, dim3(grid_for(RS_MAX_PASSES * 256, 256)), dim3(256), 0,
+                           s, S.hist, 0u, (size_t)(RS_MAX_PASSES * 256));
+        hipLaunchKernelGGL(radix_hist_kernel, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s,
+                           keys_in, n, passes, S.hist);
+    }
     hipLaunchKernelGGL(radix_base_kernel, dim3(1), dim3(256), 0, s, (const uint32_t *)S.hist,
                        passes, base, S.tile_ctr);
     FT_CHECK_LAUNCH();
@@ -687,10 +835,20 @@ template <typename T> static int launch_topology(hipStream_t s, ft_tree *t, Scra
                        t->parent, t->range_first, t->range_last, S.counters);
     FT_CHECK_LAUNCH();
     prof_mark(S, s, 5);
-    hipLaunchKernelGGL(refit_kernel<T>, dim3(grid_for(n, 256)), dim3(256), 0, s, n,
+    const unsigned chunks = (unsigned)((n + REFIT_CHUNK - 1) / REFIT_CHUNK);
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, S.top_count, 0u, (size_t)1);
+    hipLaunchKernelGGL(refit_chunk_kernel<T>, dim3(chunks), dim3(REFIT_CHUNK), 0, s, n,
                        (const int32_t *)t->left_child, (const int32_t *)t->right_child,
+                       (const int32_t *)t->parent, (const int32_t *)t->range_first,
+                       (const int32_t *)t->range_last, (T *)t->min_x, (T *)t->max_x,
+                       (T *)t->min_y, (T *)t->max_y, (T *)t->min_z, (T *)t->max_z, S.tops,
+                       S.top_count);
+    FT_CHECK_LAUNCH();
+    hipLaunchKernelGGL(refit_top_kernel<T>, dim3(std::min(chunks * 4u, 2048u)), dim3(256), 0, s,
+                       n, (const int32_t *)t->left_child, (const int32_t *)t->right_child,
                        (const int32_t *)t->parent, S.counters, (T *)t->min_x, (T *)t->max_x,
-                       (T *)t->min_y, (T *)t->max_y, (T *)t->min_z, (T *)t->max_z);
+                       (T *)t->min_y, (T *)t->max_y, (T *)t->min_z, (T *)t->max_z,
+                       (const int32_t *)S.tops, (const uint32_t *)S.top_count);
     FT_CHECK_LAUNCH();
     return FT_OK;
 }
@@ -703,16 +861,19 @@ int build_bvh_t(hipStream_t s, ft_tree *t, Scratch &S, const T *x, const T *y, c
     const ft_config &cfg = t->config;
     prof_mark(S, s, 0);
     const uint64_t *kin = keys;
+    const int key_bits = 3 * cfg.bits_per_dim;
     if (!keys) {
         if constexpr (CoordTraits<T>::fp) FT_TRY(launch_bbox<T>(s, S, x, y, z, n, bbox6));
         prof_mark(S, s, 1);
-        FT_TRY(launch_encode<T>(s, S, cfg, x, y, z, n, S.keys_in));
+        hipLaunchKernelGGL(fill_u32_kernel, dim3(grid_for(RS_MAX_PASSES * 256, 256)), dim3(256),
+                           0, s, S.hist, 0u, (size_t)(RS_MAX_PASSES * 256));
+        FT_TRY(launch_encode<T>(s, S, cfg, x, y, z, n, S.keys_in, S.hist, radix_passes(key_bits)));
         kin = S.keys_in;
     } else {
         prof_mark(S, s, 1);
     }
     prof_mark(S, s, 2);
-    FT_TRY(radix_sort(s, S, kin, n, 3 * cfg.bits_per_dim, t->sorted_keys));
+    FT_TRY(radix_sort(s, S, kin, n, key_bits, t->sorted_keys, keys == nullptr));
     prof_mark(S, s, 3);
     hipLaunchKernelGGL(gather_kernel<T>, dim3(grid_for(n, 256)), dim3(256), 0, s, x, y, z, id,
                        ghost, (const uint32_t *)S.perm, (int)n, (T *)t->min_x, (T *)t->max_x,

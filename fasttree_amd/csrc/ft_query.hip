@@ -876,28 +876,35 @@ static int launch_lane_lds(hipStream_t s, const DTree<T> &dt, const QSrc<T> &qs,
     return FT_OK;
 }
 
-// Device buffer for re-run lists, grown on demand (one per process).
+// Device buffers for the two re-run lists, grown on demand (one per process).
 struct RerunBuf {
-    int32_t *list = nullptr;
-    uint32_t *count = nullptr;  // [0] first pass, [1] stats accumulator scratch
+    int32_t *list[2] = {nullptr, nullptr};
+    uint32_t *count = nullptr;  // [0], [1]: list lengths
     size_t cap = 0;
+    void *block = nullptr;
 };
 static RerunBuf g_rerun;
 
 static int rerun_ensure(size_t n) {
-    if (n <= g_rerun.cap && g_rerun.list) return FT_OK;
-    if (g_rerun.list) (void)hipFree(g_rerun.list);
-    g_rerun.list = nullptr;
+    if (n <= g_rerun.cap && g_rerun.block) return FT_OK;
+    if (g_rerun.block) (void)hipFree(g_rerun.block);
+    g_rerun.block = nullptr;
     const size_t cap = n + n / 2 + 1024;
     void *p = nullptr;
-    if (hipMalloc(&p, cap * 4 + 256) != hipSuccess) return fail(FT_ENOMEM, "re-run list allocation failed");
-    g_rerun.list = (int32_t *)p;
-    g_rerun.count = (uint32_t *)((char *)p + cap * 4);
+    if (hipMalloc(&p, 2 * cap * 4 + 256) != hipSuccess)
+        return fail(FT_ENOMEM, "re-run list allocation failed");
+    g_rerun.block = p;
+    g_rerun.list[0] = (int32_t *)p;
+    g_rerun.list[1] = (int32_t *)p + cap;
+    g_rerun.count = (uint32_t *)((char *)p + 2 * cap * 4);
     g_rerun.cap = cap;
     return FT_OK;
 }
 
-__global__ void zero_u32_kernel(uint32_t *p) { *p = 0; }
+__global__ void zero_counts_kernel(uint32_t *p) {
+    p[0] = 0;
+    p[1] = 0;
+}
 __global__ void add_count_kernel(uint64_t *stats, const uint32_t *count) { stats[3] += *count; }
 
 static inline int bit_length(uint64_t v) {
@@ -917,6 +924,11 @@ static void launch_pk(hipStream_t s, dim3 g, const DTree<T> &dt, const QSrc<T> &
                        qs, a, seed, res, d);
 }
 
+// Runs the packet traversal and its exact re-runs:
+//   pass 1  direct, seeded or not, packed keys for integer metrics;
+//   pass 2  (packed only) unseeded packed run of the queries pass 1 listed;
+//   pass 3  unseeded lexicographic run of what is still listed -- exact.
+// The LISTED passes read list lengths on the device: no host round trip.
 template <typename T, int M>
 static int packet_run(hipStream_t s, const DTree<T> &dt, const QSrc<T> &qs, int nq, int k,
                       int excl, int bucket, const typename CoordTraits<T>::D *seed, uint64_t *res,
@@ -932,8 +944,8 @@ static int packet_run(hipStream_t s, const DTree<T> &dt, const QSrc<T> &qs, int 
     const bool need_list = seed != nullptr || INTD;
     if (need_list) {
         FT_TRY(rerun_ensure((size_t)nq));
-        hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(1), 0, s, g_rerun.count);
-        a.rerun = g_rerun.list;
+        hipLaunchKernelGGL(zero_counts_kernel, dim3(1), dim3(1), 0, s, g_rerun.count);
+        a.rerun = g_rerun.list[0];
         a.rerun_count = g_rerun.count;
     }
     const dim3 g(blocks_for(nq, 64));
@@ -945,24 +957,33 @@ static int packet_run(hipStream_t s, const DTree<T> &dt, const QSrc<T> &qs, int 
         else launch_pk<T, M, false, false, false>(s, g, dt, qs, a, seed, res, d);
     }
     FT_CHECK_LAUNCH();
-    if (need_list) {
-        // Exact re-run of the listed queries: unseeded, lexicographic keys, a
-        // fixed grid that walks the device-side list.
-        PacketArgs b = a;
-        b.list = g_rerun.list;
-        b.list_count = g_rerun.count;
-        b.rerun = nullptr;
-        b.rerun_count = nullptr;
-        b.stats = nullptr;
-        const unsigned waves = std::min<unsigned>(blocks_for(nq, 64), 2048u);
-        launch_pk<T, M, false, false, true>(s, dim3(waves), dt, qs, b, nullptr, res, d);
+    if (!need_list) return FT_OK;
+    if (stats) {
+        hipLaunchKernelGGL(add_count_kernel, dim3(1), dim3(1), 0, s, stats,
+                           (const uint32_t *)g_rerun.count);
         FT_CHECK_LAUNCH();
-        if (stats) {
-            hipLaunchKernelGGL(add_count_kernel, dim3(1), dim3(1), 0, s, stats,
-                               (const uint32_t *)g_rerun.count);
+    }
+    const unsigned waves = std::min<unsigned>(blocks_for(nq, 64), 2048u);
+    PacketArgs b = a;
+    b.stats = nullptr;
+    int lex_list = 0;
+    if constexpr (INTD) {
+        if (seed) {
+            b.list = g_rerun.list[0];
+            b.list_count = g_rerun.count;
+            b.rerun = g_rerun.list[1];
+            b.rerun_count = g_rerun.count + 1;
+            launch_pk<T, M, false, true, true>(s, dim3(waves), dt, qs, b, nullptr, res, d);
             FT_CHECK_LAUNCH();
+            lex_list = 1;
         }
     }
+    b.list = g_rerun.list[lex_list];
+    b.list_count = g_rerun.count + lex_list;
+    b.rerun = nullptr;
+    b.rerun_count = nullptr;
+    launch_pk<T, M, false, false, true>(s, dim3(waves), dt, qs, b, nullptr, res, d);
+    FT_CHECK_LAUNCH();
     return FT_OK;
 }
 
