@@ -86,6 +86,9 @@ EXPORTS = [
     "ft_scratch_create", "ft_scratch_destroy", "ft_scratch_set_profiling", "ft_scratch_stage_ms",
     "ft_compute_bbox", "ft_sfc_encode", "ft_build_tree", "ft_build_bvh", "ft_self_knn_query",
     "ft_self_knn_query_subset", "ft_knn_query", "ft_range_query", "ft_range_query_csr",
+    "ft_mlp_num_params", "ft_mlp_create", "ft_mlp_destroy", "ft_mlp_set", "ft_mlp_get",
+    "ft_knn_features", "ft_mlp_forward", "ft_mlp_grad", "ft_mlp_train_step",
+    "ft_mlp_last_loss", "ft_mlp_seed",
 ]
 
 _lib = None
@@ -177,6 +180,14 @@ class DeviceArray:
         out = np.empty(self.n, self.dtype)
         if self.n:
             _check(lib().ft_memcpy_d2h(out.ctypes.data, self.ptr, self.nbytes, stream))
+        return out
+
+    def to_host_range(self, start: int, count: int, stream=None):
+        """Copies elements [start, start + count) to the host."""
+        out = np.empty(count, self.dtype)
+        if count:
+            _check(lib().ft_memcpy_d2h(out.ctypes.data, self.ptr + start * self.dtype.itemsize,
+                                       out.nbytes, stream))
         return out
 
     def zero(self, stream=None):

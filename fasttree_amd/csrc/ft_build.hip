@@ -565,7 +565,7 @@ __global__ __launch_bounds__(256) void refit_kernel(int n, const int32_t *__rest
 // coalesced accesses; the climb synchronises at workgroup scope only.  A
 // climber stops below the first node whose range leaves the chunk and records
 // the node it completed (a "chunk top") for pass 2.
-constexpr int REFIT_CHUNK = 512;
+constexpr int REFIT_CHUNK = 1024;
 
 template <typename T>
 __global__ __launch_bounds__(REFIT_CHUNK) void refit_chunk_kernel(

@@ -202,6 +202,36 @@ int ft_range_query_csr(ft_stream s, const ft_tree *tree, const void *qx, const v
                        const void *qz, const void *r_min, const void *r_max, int num_queries,
                        int64_t *row_offsets, int32_t *cols, int64_t col_capacity);
 
+/* ---- learned search radius (no reference counterpart) --------------------- */
+/* A 12-64-64-64-1 ReLU MLP (bf16 weights, fp32 accumulation on MFMA) predicts     */
+/* log d2_k of a self query from tree-density features; its prediction plus a      */
+/* margin seeds ft_query_opts.seed_bound.  Results are unchanged by the seed.      */
+typedef struct ft_mlp ft_mlp;
+int ft_mlp_num_params(void);
+int ft_mlp_create(ft_mlp **model);
+int ft_mlp_destroy(ft_mlp *model);
+/* params: ft_mlp_num_params() floats; norm: 34 floats (feature mean[16],          */
+/* 1/std[16], target mean, target std) -- host memory; NULL keeps the current.     */
+int ft_mlp_set(ft_mlp *model, const float *params, const float *norm, double margin);
+int ft_mlp_get(ft_mlp *model, float *params, float *norm, double *margin);
+/* Feature rows float[nq][16] (device) for self queries (leaf_ids NULL = all).     */
+int ft_knn_features(ft_stream s, const ft_tree *tree, const int32_t *leaf_ids, int nq, int k,
+                    float *features);
+/* Predicted log d2_k, float[n] (device pointers).                                 */
+int ft_mlp_forward(ft_stream s, ft_mlp *model, const float *features, int n, float *pred);
+/* Gradient float[num_params] and loss of the mean squared error of the             */
+/* standardised target over n samples (device pointers).                           */
+int ft_mlp_grad(ft_stream s, ft_mlp *model, const float *features, const float *target, int n,
+                float *grad, float *loss);
+/* One Adam step over n samples (device pointers).                                 */
+int ft_mlp_train_step(ft_stream s, ft_mlp *model, const float *features, const float *target,
+                      int n, float lr);
+int ft_mlp_last_loss(ft_stream s, ft_mlp *model, float *loss);
+/* Seed bounds dist_t[nq] (device) for self queries: features + forward + margin.  */
+/* scratch_features: nq * 16 floats of device memory.                              */
+int ft_mlp_seed(ft_stream s, ft_mlp *model, const ft_tree *tree, const int32_t *leaf_ids,
+                int nq, int k, float *scratch_features, void *seed);
+
 #ifdef __cplusplus
 }
 #endif
