@@ -273,16 +273,22 @@ __device__ inline uint32_t block_excl_scan256(uint32_t v, uint32_t *s_tmp /*>= 4
 }
 
 // Exclusive scan of each pass's histogram -> digit base offsets; resets the
-// per-pass tile counters.
+// per-pass tile counters and advances the sort counter (epoch) that tags this
+// sort's look-back words.  Epochs run 1 .. 2^27-1 and then wrap to 1.
 __global__ __launch_bounds__(256) void radix_base_kernel(const uint32_t *__restrict__ hist,
                                                          int passes, uint32_t *__restrict__ base,
-                                                         uint32_t *__restrict__ tile_ctr) {
+                                                         uint32_t *__restrict__ tile_ctr,
+                                                         uint32_t *__restrict__ epoch) {
     __shared__ uint32_t tmp[4];
     for (int p = 0; p < passes; ++p) {
         uint32_t v = hist[p * 256 + threadIdx.x];
         base[p * 256 + threadIdx.x] = block_excl_scan256(v, tmp);
     }
     if (threadIdx.x < RS_MAX_PASSES) tile_ctr[threadIdx.x] = 0;
+    if (threadIdx.x == 0) {
+        const uint32_t e = *epoch + 1u;
+        *epoch = (e >= (1u << 27)) ? 1u : e;
+    }
 }
 
 __device__ inline uint64_t lanemask_lt() {
@@ -294,8 +300,8 @@ template <bool FIRST>
 __global__ __launch_bounds__(RS_BLOCK) void onesweep_kernel(
     const uint64_t *__restrict__ keys_in, const uint32_t *__restrict__ vals_in,
     uint64_t *__restrict__ keys_out, uint32_t *__restrict__ vals_out, int n, int shift,
-    const uint32_t *__restrict__ digit_base, uint64_t *__restrict__ lookback, uint32_t epoch,
-    uint32_t *__restrict__ tile_ctr) {
+    const uint32_t *__restrict__ digit_base, uint64_t *__restrict__ lookback,
+    const uint32_t *__restrict__ epoch, int pass, uint32_t *__restrict__ tile_ctr) {
     __shared__ uint64_t s_keys[RS_TILE];
     __shared__ uint32_t s_vals[RS_TILE];
     __shared__ uint32_t s_wave[RS_WAVES][256];
@@ -354,7 +360,7 @@ __global__ __launch_bounds__(RS_BLOCK) void onesweep_kernel(
         s_wave[w][dg] = count;
         count += c;
     }
-    const uint64_t tag = (uint64_t)(epoch & 0x3FFFFFFFu) << 34;
+    const uint64_t tag = (uint64_t)((*epoch * 8u + (uint32_t)pass) & 0x3FFFFFFFu) << 34;
     uint64_t *slot = lookback + (size_t)tile * 256 + dg;
     uint32_t excl = 0;
     if (tile == 0) {
@@ -713,6 +719,7 @@ static int scratch_ensure(Scratch &S, size_t n) {
     const size_t o_hist = off; off += al(RS_MAX_PASSES * 256 * 4 * 2);
     const size_t o_look = off; off += al(tiles * 256 * 8);
     const size_t o_tile = off; off += al(64);
+    const size_t o_epoch = off; off += al(64);
     const size_t o_topc = off; off += al(64);
     const size_t o_bbox = off; off += al(16 * 8);
     const size_t o_part = off; off += al(1024 * 6 * 8);
@@ -737,6 +744,7 @@ static int scratch_ensure(Scratch &S, size_t n) {
     S.lookback = (uint64_t *)(b + o_look);
     S.lookback_words = tiles * 256;
     S.tile_ctr = (uint32_t *)(b + o_tile);
+    S.epoch = (uint32_t *)(b + o_epoch);
     S.bbox_dev = (void *)(b + o_bbox);
     S.partials = (void *)(b + o_part);
     return FT_OK;
@@ -797,7 +805,7 @@ static int radix_sort(hipStream_t s, Scratch &S, const uint64_t *keys_in, size_t
                            keys_in, n, passes, S.hist);
     }
     hipLaunchKernelGGL(radix_base_kernel, dim3(1), dim3(256), 0, s, (const uint32_t *)S.hist,
-                       passes, base, S.tile_ctr);
+                       passes, base, S.tile_ctr, S.epoch);
     FT_CHECK_LAUNCH();
     const unsigned tiles = (unsigned)((n + RS_TILE - 1) / RS_TILE);
     const uint64_t *kin = keys_in;
@@ -806,20 +814,18 @@ static int radix_sort(hipStream_t s, Scratch &S, const uint64_t *keys_in, size_t
         const bool last = (p == passes - 1);
         uint64_t *kout = last ? sorted_out : ((p & 1) ? S.keys_b : S.keys_a);
         uint32_t *vout = last ? S.perm : ((p & 1) ? S.vals_b : S.vals_a);
-        const uint32_t ep = (uint32_t)(S.epoch++ & 0x3FFFFFFFu);
         if (p == 0)
             hipLaunchKernelGGL(onesweep_kernel<true>, dim3(tiles), dim3(RS_BLOCK), 0, s, kin, vin,
-                               kout, vout, (int)n, 8 * p, base + p * 256, S.lookback, ep,
-                               S.tile_ctr + p);
+                               kout, vout, (int)n, 8 * p, base + p * 256, S.lookback,
+                               (const uint32_t *)S.epoch, p, S.tile_ctr + p);
         else
             hipLaunchKernelGGL(onesweep_kernel<false>, dim3(tiles), dim3(RS_BLOCK), 0, s, kin, vin,
-                               kout, vout, (int)n, 8 * p, base + p * 256, S.lookback, ep,
-                               S.tile_ctr + p);
+                               kout, vout, (int)n, 8 * p, base + p * 256, S.lookback,
+                               (const uint32_t *)S.epoch, p, S.tile_ctr + p);
         FT_CHECK_LAUNCH();
         kin = kout;
         vin = vout;
     }
-    if (S.epoch >= 0x3FFFFFF0ull) S.epoch = 1;  // practically never; epochs stay unique
     return FT_OK;
 }
 

@@ -3,6 +3,7 @@
 #include "ft_internal.hpp"
 
 #include <climits>
+#include <cstdio>
 #include <limits>
 #include <mutex>
 #include <vector>
@@ -442,11 +443,17 @@ int ft_build_bvh(ft_stream s, ft_tree *tree, ft_scratch *scratch, const void *x,
 }
 
 // ---- queries -------------------------------------------------------------------
-static int check_k(int k, int max_k) {
+// The reference's message for k > _MAX_K_ names the entry point
+// (src/hlbvh.hpp:1942, :1970, :1445).
+static int check_k(int k, int max_k, const char *entry) {
     if (k < 0) return fail(FT_EINVAL, "k must be non-negative");
-    if (k > max_k)
-        return fail(FT_EINVAL,
-                    "k exceeds _MAX_K_ template parameter. Instantiate the query with larger _MAX_K_.");
+    if (k > max_k) {
+        char msg[160];
+        std::snprintf(msg, sizeof msg,
+                      "k exceeds _MAX_K_ template parameter. Instantiate %s with larger _MAX_K_.",
+                      entry);
+        return fail(FT_EINVAL, msg);
+    }
     return FT_OK;
 }
 
@@ -482,7 +489,7 @@ int ft_self_knn_query(ft_stream s, const ft_tree *tree, int k, uint64_t *results
     if (!tree) return fail(FT_EINVAL, "null tree");
     const size_t n = tree->num_leaves;
     if (n == 0) return FT_OK;
-    FT_TRY(check_k(k, (opts && opts->max_k > 0) ? opts->max_k : 32));
+    FT_TRY(check_k(k, (opts && opts->max_k > 0) ? opts->max_k : 32, "self_knn_query"));
     if (k == 0) return FT_OK;
     return knn_common(s, tree, nullptr, nullptr, nullptr, nullptr, (int)n, k, results, dists,
                       exclude_self ? 1 : 0, opts);
@@ -493,7 +500,7 @@ int ft_self_knn_query_subset(ft_stream s, const ft_tree *tree, const int32_t *le
                              const ft_query_opts *opts) {
     if (!tree) return fail(FT_EINVAL, "null tree");
     if (num_ids <= 0) return FT_OK;
-    FT_TRY(check_k(k, (opts && opts->max_k > 0) ? opts->max_k : 256));
+    FT_TRY(check_k(k, (opts && opts->max_k > 0) ? opts->max_k : 256, "self_knn_query_subset"));
     if (k == 0 || tree->num_leaves == 0) return FT_OK;
     if (!leaf_ids) return fail(FT_EINVAL, "null leaf_ids");
     return knn_common(s, tree, nullptr, nullptr, nullptr, leaf_ids, num_ids, k, results, dists,
@@ -505,7 +512,7 @@ int ft_knn_query(ft_stream s, const ft_tree *tree, const void *qx, const void *q
                  const ft_query_opts *opts) {
     if (!tree) return fail(FT_EINVAL, "null tree");
     if (tree->num_leaves == 0 || num_queries <= 0) return FT_OK;
-    FT_TRY(check_k(k, (opts && opts->max_k > 0) ? opts->max_k : 256));
+    FT_TRY(check_k(k, (opts && opts->max_k > 0) ? opts->max_k : 256, "knn_query"));
     if (k == 0) return FT_OK;
     if (!qx || !qy || !qz) return fail(FT_EINVAL, "null query coordinates");
     return knn_common(s, tree, qx, qy, qz, nullptr, num_queries, k, results, dists, 0, opts);

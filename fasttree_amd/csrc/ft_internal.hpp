@@ -270,9 +270,11 @@ struct Scratch {
     uint64_t *lookback = nullptr; // tiles x 256 status words
     size_t lookback_words = 0;
     uint32_t *tile_ctr = nullptr; // 8 passes
+    uint32_t *epoch = nullptr;    // sort counter on the device: look-back tags of a sort
+                                  // are 8 * epoch + pass, so no tag is reused and a
+                                  // captured graph that replays the sort stays correct
     void *bbox_dev = nullptr;     // 6 coords + 3 inverse extents (16 x 8 bytes)
     void *partials = nullptr;     // bbox partials (1024 x 6 x 8 bytes)
-    uint64_t epoch = 1;           // look-back epochs: never reused
     // staging of inputs (host pointers)
     void *stage_block = nullptr;
     size_t stage_bytes = 0;

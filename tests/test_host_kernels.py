@@ -17,3 +17,31 @@ def test_leaf_distance_equals_box_distance(tmp_path):
     out = subprocess.run([exe], capture_output=True, text=True, timeout=300)
     assert out.returncode == 0, out.stdout
     assert "total mismatches: 0" in out.stdout
+
+
+HEADER_CONFIGS = [
+    ["-DFASTTREE_INTEGER_COORDS", "-DPOSITIONS_IN_64BIT", "-DPERIODIC_BC"],
+    ["-DFASTTREE_INTEGER_COORDS", "-DPOSITIONS_IN_32BIT", "-DSFC_TYPE_MORTON"],
+    ["-DCOORDS_TYPE_DOUBLE", "-DSFC_TYPE_MORTON", "-DBITS_PER_DIMENSION=10"],
+    [],
+]
+
+
+@pytest.mark.parametrize("defs", HEADER_CONFIGS, ids=["u64-ph-periodic", "u32-morton",
+                                                      "f64-morton10", "f32-default"])
+def test_cxx_dropin_header_builds_and_maps_errors(tmp_path, defs):
+    """include/fasttree/hlbvh.hpp compiles under the reference's macro sets,
+    links against the library and maps k > _MAX_K_ to std::invalid_argument
+    with the reference's message."""
+    lib = os.path.join(ROOT, "fasttree_amd", "libfasttree_amd.so")
+    if not os.path.exists(lib):
+        pytest.skip("library not built")
+    src = os.path.join(ROOT, "tests", "host", "hlbvh_header_check.cpp")
+    exe = str(tmp_path / "hlbvh_header_check")
+    subprocess.check_call(["g++", "-std=c++17", "-Wall", "-Werror", "-D__HIP_PLATFORM_AMD__",
+                           "-I/opt/rocm/include", "-I" + os.path.join(ROOT, "include")] + defs +
+                          [src, "-L" + os.path.dirname(lib), "-lfasttree_amd",
+                           "-Wl,-rpath," + os.path.dirname(lib), "-o", exe])
+    out = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert ("Instantiate self_knn_query with larger _MAX_K_." in out.stdout)
