@@ -88,7 +88,9 @@ EXPORTS = [
     "ft_self_knn_query_subset", "ft_knn_query", "ft_range_query", "ft_range_query_csr",
     "ft_mlp_num_params", "ft_mlp_create", "ft_mlp_destroy", "ft_mlp_set", "ft_mlp_get",
     "ft_knn_features", "ft_mlp_forward", "ft_mlp_grad", "ft_mlp_train_step",
-    "ft_mlp_last_loss", "ft_mlp_seed",
+    "ft_mlp_last_loss", "ft_mlp_seed", "ft_batch_create", "ft_batch_destroy",
+    "ft_batch_set_trees", "ft_batch_set_queries", "ft_batch_set_stats", "ft_batch_tune",
+    "ft_batch_run", "ft_batch_capture", "ft_batch_launch", "ft_batch_tuning",
 ]
 
 _lib = None
@@ -192,6 +194,15 @@ class DeviceArray:
 
     def zero(self, stream=None):
         _check(lib().ft_memset(self.ptr, 0, self.nbytes, stream))
+        return self
+
+    def copy_from(self, a, stream=None):
+        """Overwrites the array with host data of the same length."""
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        if a.size != self.n:
+            raise ValueError("copy_from: length differs")
+        if a.size:
+            _check(lib().ft_memcpy_h2d(self.ptr, a.ctypes.data, a.nbytes, stream))
         return self
 
     def free(self):

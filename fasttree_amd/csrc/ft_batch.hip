@@ -1,0 +1,1010 @@
+// ft_batch.hip -- many trees of uneven size and query requests of mixed k in
+// one hipGraph launch (no reference counterpart; its closest analogue is the
+// compile-time width / heap-tier dispatch of src/hlbvh.hpp:1433-1453,
+// 1841-1927).
+//
+// One step of a batch:
+//   1. builds every tree (build_bvh) on parallel graph branches, each branch
+//      with its own build scratch;
+//   2. sorts all query entries by (request, Morton key of the query inside the
+//      tree's root box), so each wavefront's 64 queries are neighbours of one
+//      request;
+//   3. runs one persistent kernel per tier (register top-k of width M, LDS
+//      heap of capacity M, or per-lane LDS heaps) as parallel graph nodes;
+//      each drains its own work list of (request, packet) items over all
+//      trees, largest measured cost first;
+//   4. re-runs exactly, one lane per query, every query whose packed keys
+//      could not order its results.
+// The autotuner times candidate settings per (tree-size class, k tier) on the
+// bound data -- tier and leaf bucket for the queries, branch count for the
+// builds -- and keeps the fastest.  Results are canonical (ascending (d2,
+// index)), so every setting gives the same output.
+#include "ft_traverse.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <climits>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+namespace ft {
+
+template <typename T>
+int build_bvh_t(hipStream_t s, ft_tree *t, Scratch &S, const T *x, const T *y, const T *z,
+                const uint32_t *id, const int8_t *ghost, size_t n, const T *bbox6,
+                const uint64_t *keys);
+int radix_sort_u64(hipStream_t s, Scratch &S, const uint64_t *keys_in, size_t n,
+                   uint64_t *sorted_out);
+int scratch_ensure_public(Scratch &S, size_t n);
+
+// A request as the kernels see it (device copy).
+struct BatchReq {
+    int32_t tree, k;
+    int32_t entry_first, num;  // entries [entry_first, entry_first + num): sorted positions too
+    int32_t bucket, variant;   // tuned leaf bucket; tier kernel that runs it
+    int64_t query_first;       // first query point
+    int64_t result_first;      // first result slot
+};
+
+template <typename T> struct BatchView {
+    const DTree<T> *trees;
+    const BatchReq *reqs;
+    int nreq;
+    const uint32_t *order;  // sorted position -> entry index
+    const T *qx, *qy, *qz;
+    uint64_t *res;
+    typename CoordTraits<T>::D *dists;
+    int32_t *rerun;  // sorted positions to re-run exactly
+    uint32_t *rerun_count;
+    uint64_t *stats;
+};
+
+// Tier kernels ("variants").  PK_M: packet walk with the register top-k of
+// width M; HP_M: packet walk with an LDS heap of capacity M (integer metrics);
+// LANE: per-lane DFS with LDS heaps, one query per lane (any k, any metric).
+enum Variant : int {
+    V_PK1, V_PK4, V_PK8, V_PK16, V_PK32, V_HP32, V_HP64, V_HP128, V_LANE, V_COUNT
+};
+static const char *const kVariantName[V_COUNT] = {"packet-reg1",  "packet-reg4",  "packet-reg8",
+                                                  "packet-reg16", "packet-reg32", "packet-heap32",
+                                                  "packet-heap64", "packet-heap128", "lane-heap"};
+
+// ---------------------------------------------------------------------------
+// Kernels
+// ---------------------------------------------------------------------------
+// Entry -> composite sort key: request index on top, then the Morton key of the
+// query quantised to qb bits per axis inside the tree's root box.  Only the
+// grouping matters (queries of one packet should be spatial neighbours).
+template <typename T>
+__global__ __launch_bounds__(256) void batch_key_kernel(BatchView<T> b, int64_t n_entries,
+                                                        int rbits, int qb,
+                                                        uint64_t *__restrict__ keys) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n_entries) return;
+    int lo = 0, hi = b.nreq - 1;  // last request with entry_first <= e
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (b.reqs[mid].entry_first <= e) lo = mid;
+        else hi = mid - 1;
+    }
+    const BatchReq q = b.reqs[lo];
+    const DTree<T> &t = b.trees[q.tree];
+    const int64_t qi = q.query_first + (e - q.entry_first);
+    const double side = (double)((1u << qb) - 1);
+    auto quant = [&](T v, T lo_, T hi_) -> uint64_t {
+        const double ext = (double)hi_ - (double)lo_;
+        double u = ext > 0.0 ? ((double)v - (double)lo_) / ext : 0.0;
+        u = u < 0.0 ? 0.0 : (u > 1.0 ? 1.0 : u);
+        return (uint64_t)(u * side);
+    };
+    const uint64_t ix = quant(b.qx[qi], t.mnx[0], t.mxx[0]);
+    const uint64_t iy = quant(b.qy[qi], t.mny[0], t.mxy[0]);
+    const uint64_t iz = quant(b.qz[qi], t.mnz[0], t.mxz[0]);
+    uint64_t m = 0;
+    for (int bit = 0; bit < qb; ++bit) {
+        m |= ((ix >> bit) & 1ull) << (3 * bit + 2);
+        m |= ((iy >> bit) & 1ull) << (3 * bit + 1);
+        m |= ((iz >> bit) & 1ull) << (3 * bit);
+    }
+    keys[e] = ((uint64_t)lo << (64 - rbits)) | m;
+}
+
+__global__ void batch_reset_kernel(uint32_t *ctr, int n) {
+    if ((int)threadIdx.x < n) ctr[threadIdx.x] = 0u;
+}
+
+__device__ inline int grab_item(uint32_t *ctr) {
+    int it = 0;
+    if ((threadIdx.x & 63) == 0) it = (int)atomicAdd(ctr, 1u);
+    return uni(__shfl(it, 0, 64));
+}
+
+// Packet tiers.  HEAP: LDS heap of capacity M, else the register top-k.
+template <typename T, int M, bool HEAP>
+__global__ __launch_bounds__(64) void batch_packet_kernel(BatchView<T> b,
+                                                          const uint2 *__restrict__ items,
+                                                          int n_items, uint32_t *__restrict__ ctr) {
+    using D = typename CoordTraits<T>::D;
+    constexpr bool PACKED = !CoordTraits<T>::fp;
+    static_assert(!HEAP || PACKED, "LDS heap tier needs packed integer keys");
+    using H = typename std::conditional<HEAP, TopKHeapLds<M>,
+                                        typename TopKSel<D, M, PACKED>::type>::type;
+    __shared__ int32_t stk[PK_STACK];
+    [[maybe_unused]] uint64_t *heap_mem = nullptr;
+    if constexpr (HEAP) {
+        __shared__ uint64_t heap[M * 64];
+        heap_mem = heap;
+    }
+    const int lane = threadIdx.x;
+    PacketCounters c;
+    while (true) {
+        const int it = grab_item(ctr);
+        if (it >= n_items) break;
+        const uint2 w = items[it];
+        const BatchReq q = b.reqs[uni((int)w.x)];
+        const int j = uni((int)w.y);
+        const DTree<T> t = b.trees[q.tree];
+        const int k = q.k;
+        const int e = j * 64 + lane;
+        const bool active = e < q.num;
+        int local = 0;
+        T px = T(0), py = T(0), pz = T(0);
+        if (active) {
+            local = (int)b.order[q.entry_first + e] - q.entry_first;
+            const int64_t qi = q.query_first + local;
+            px = b.qx[qi];
+            py = b.qy[qi];
+            pz = b.qz[qi];
+        }
+        H h;
+        if constexpr (HEAP) h.key = heap_mem;
+        if constexpr (PACKED) h.init(k, ~0ull, 32 - __clz(t.n));
+        else h.init(k, dist_max<D>());
+        packet_walk<T, H>(t, q.bucket, active, px, py, pz, -1, h, stk, c);
+        if (active) {
+            bool redo = false;
+            if constexpr (PACKED) redo = h.needs_exact(k);
+            if (redo) {
+                const uint32_t s2 = atomicAdd(b.rerun_count, 1u);
+                b.rerun[s2] = q.entry_first + e;
+            } else {
+                h.store(k, b.res + q.result_first, b.dists + q.result_first, (size_t)local);
+            }
+        }
+    }
+    if (lane == 0 && b.stats) {
+        atomicAdd((unsigned long long *)&b.stats[0], (unsigned long long)c.nodes);
+        atomicAdd((unsigned long long *)&b.stats[1], (unsigned long long)c.leaves);
+    }
+    wave_add_stats(b.stats, 4, c.inserts);
+}
+
+// Per-lane tier: `lanes` queries per item (item j covers entries
+// [j*lanes, (j+1)*lanes) of its request), LDS heap rows of capacity kcap.
+template <typename T>
+__global__ __launch_bounds__(64) void batch_lane_kernel(BatchView<T> b,
+                                                        const uint2 *__restrict__ items,
+                                                        int n_items, uint32_t *__restrict__ ctr,
+                                                        int lanes, int kcap) {
+    using D = typename CoordTraits<T>::D;
+    extern __shared__ __align__(16) unsigned char lds[];
+    D *hd = (D *)lds;
+    int32_t *hi = (int32_t *)(hd + (size_t)kcap * lanes);
+    int32_t *stk = hi + (size_t)kcap * lanes;
+    const int lane = threadIdx.x;
+    uint64_t visits = 0, drops = 0, inserts = 0;
+    while (true) {
+        const int it = grab_item(ctr);
+        if (it >= n_items) break;
+        const uint2 w = items[it];
+        const BatchReq q = b.reqs[uni((int)w.x)];
+        const int j = uni((int)w.y);
+        const DTree<T> t = b.trees[q.tree];
+        const int e = j * lanes + lane;
+        const bool active = lane < lanes && e < q.num;
+        int local = 0;
+        T px = T(0), py = T(0), pz = T(0);
+        if (active) {
+            local = (int)b.order[q.entry_first + e] - q.entry_first;
+            const int64_t qi = q.query_first + local;
+            px = b.qx[qi];
+            py = b.qy[qi];
+            pz = b.qz[qi];
+        }
+        LdsHeap<D, false> h{hd, hi, lanes, lane, 0};
+        lane_heap_walk<T, false, true>(t, q.k, active, px, py, pz, -1, h, stk, lanes, lane,
+                                       visits, drops, inserts);
+        if (active) {
+            h.sort();
+            uint64_t *res = b.res + q.result_first + (size_t)local * q.k;
+            D *dst = b.dists + q.result_first + (size_t)local * q.k;
+            for (int s = 0; s < q.k; ++s) {
+                const bool ok = s < h.count && h.ii(s) >= 0;
+                res[s] = ok ? (uint64_t)(uint32_t)h.ii(s) : ~0ull;
+                dst[s] = ok ? h.dd(s) : missing_dist<D>();
+            }
+        }
+    }
+    wave_add_stats(b.stats, 0, visits);
+    wave_add_stats(b.stats, 4, inserts);
+}
+
+// Exact re-run of listed sorted positions: one lane per query, each lane on its
+// own tree (found by binary search over the requests), lexicographic LDS heap.
+template <typename T>
+__global__ __launch_bounds__(64) void batch_exact_kernel(BatchView<T> b, int lanes, int kcap) {
+    using D = typename CoordTraits<T>::D;
+    extern __shared__ __align__(16) unsigned char lds[];
+    D *hd = (D *)lds;
+    int32_t *hi = (int32_t *)(hd + (size_t)kcap * lanes);
+    int32_t *stk = hi + (size_t)kcap * lanes;
+    const int lane = threadIdx.x;
+    const int nlist = (int)*b.rerun_count;
+    uint64_t visits = 0, drops = 0, inserts = 0;
+    for (int base = blockIdx.x * lanes; base < nlist; base += gridDim.x * lanes) {
+        const int slot = base + lane;
+        const bool active = lane < lanes && slot < nlist;
+        int r = 0, local = 0;
+        if (active) {
+            const int pos = b.rerun[slot];
+            int lo = 0, hi2 = b.nreq - 1;
+            while (lo < hi2) {
+                const int mid = (lo + hi2 + 1) >> 1;
+                if (b.reqs[mid].entry_first <= pos) lo = mid;
+                else hi2 = mid - 1;
+            }
+            r = lo;
+            local = (int)b.order[pos] - b.reqs[r].entry_first;
+        }
+        const BatchReq q = b.reqs[r];
+        const DTree<T> t = b.trees[q.tree];
+        T px = T(0), py = T(0), pz = T(0);
+        if (active) {
+            const int64_t qi = q.query_first + local;
+            px = b.qx[qi];
+            py = b.qy[qi];
+            pz = b.qz[qi];
+        }
+        LdsHeap<D, false> h{hd, hi, lanes, lane, 0};
+        lane_heap_walk<T, false, false>(t, active ? q.k : 0, active, px, py, pz, -1, h, stk, lanes,
+                                        lane, visits, drops, inserts);
+        if (active) {
+            h.sort();
+            uint64_t *res = b.res + q.result_first + (size_t)local * q.k;
+            D *dst = b.dists + q.result_first + (size_t)local * q.k;
+            for (int s = 0; s < q.k; ++s) {
+                const bool ok = s < h.count && h.ii(s) >= 0;
+                res[s] = ok ? (uint64_t)(uint32_t)h.ii(s) : ~0ull;
+                dst[s] = ok ? h.dd(s) : missing_dist<D>();
+            }
+        }
+    }
+}
+
+__global__ void batch_count_kernel(uint64_t *stats, const uint32_t *count) {
+    if (stats) stats[3] += *count;
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+static inline unsigned cdiv(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
+
+static int size_class(size_t n) {
+    int c = 0;
+    for (size_t lim = 40000; c < 3 && n >= lim; lim *= 4) ++c;
+    return c;  // < 40k, < 160k, < 640k, larger
+}
+static int k_tier(int k) {
+    const int lim[8] = {1, 4, 8, 16, 32, 64, 128, 256};
+    for (int i = 0; i < 8; ++i)
+        if (k <= lim[i]) return i;
+    return 7;
+}
+static const int kTierCap[8] = {1, 4, 8, 16, 32, 64, 128, 256};
+constexpr int N_CLASS = 4, N_KTIER = 8;
+
+struct GroupTune {
+    int variant = -1, bucket = 32;
+    double ms_per_item = 0.0;  // measured cost of one item (load balancing)
+    bool tuned = false;
+};
+
+}  // namespace ft
+
+struct ft_batch {
+    int32_t coord_type = 0;
+    // trees
+    std::vector<ft_tree *> trees;
+    std::vector<const void *> bx, by, bz;
+    std::vector<size_t> bn;
+    bool build = false;
+    // requests
+    std::vector<ft_batch_request> reqs;
+    const void *qx = nullptr, *qy = nullptr, *qz = nullptr;
+    uint64_t *results = nullptr;
+    void *dists = nullptr;
+    int64_t n_entries = 0;
+    int kmax = 0;
+    uint64_t *stats = nullptr;
+    // tuning
+    ft::GroupTune tune[ft::N_CLASS][ft::N_KTIER];
+    int branches = 4;
+    bool tuned = false;
+    // device state
+    void *dev_block = nullptr;
+    size_t dev_bytes = 0;
+    void *dtrees = nullptr;
+    ft::BatchReq *dreqs = nullptr;
+    uint2 *ditems = nullptr;
+    uint32_t *dctr = nullptr;  // [V_COUNT] + rerun count
+    int32_t *drerun = nullptr;
+    uint64_t *dkeys = nullptr, *dsorted = nullptr;
+    int item_first[ft::V_COUNT] = {}, item_count[ft::V_COUNT] = {};
+    int lane_width = 64;
+    ft::Scratch sort_scratch;
+    std::vector<ft::Scratch> build_scratch;
+    std::vector<hipStream_t> side;
+    hipEvent_t ev_fork = nullptr;
+    std::vector<hipEvent_t> ev_join;
+    hipStream_t cap = nullptr;  // capture stream (the null stream cannot be captured)
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool dirty = true;  // device tables need a refresh
+};
+
+namespace ft {
+
+int fail(int code, const std::string &msg);
+
+// LDS lanes of the per-lane tier for heap capacity kcap (canonical walk).
+template <typename T> static int lane_tier_lanes(int kcap, size_t &lds) {
+    using D = typename CoordTraits<T>::D;
+    const size_t per_lane = (size_t)kcap * (sizeof(D) + 4) + (size_t)LANE_STACK_CANON * 4;
+    int lanes = 64;
+    while (lanes > 1 && (size_t)lanes * per_lane > 64 * 1024) lanes >>= 1;
+    lds = (size_t)lanes * per_lane;
+    return lds <= 64 * 1024 ? lanes : 0;
+}
+
+static void free_graph(ft_batch *b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+static int ensure_streams(ft_batch *b, int nb) {
+    while ((int)b->side.size() < nb) {
+        hipStream_t st;
+        FT_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        b->side.push_back(st);
+        hipEvent_t ev;
+        FT_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        b->ev_join.push_back(ev);
+    }
+    if (!b->ev_fork) FT_HIP(hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming));
+    return FT_OK;
+}
+
+// Candidate variants for a group (class, k tier) and metric.
+static std::vector<int> candidates(int ktier, bool fp) {
+    std::vector<int> v;
+    const int cap = kTierCap[ktier];
+    if (cap <= 32) v.push_back(V_PK1 + (ktier <= 4 ? ktier : 4));
+    if (!fp) {
+        if (cap <= 32) v.push_back(V_HP32);
+        else if (cap <= 64) v.push_back(V_HP64);
+        else if (cap <= 128) v.push_back(V_HP128);
+    }
+    v.push_back(V_LANE);
+    return v;
+}
+
+static bool is_packet(int v) { return v != V_LANE; }
+
+// Default settings: register packets up to k = 32, LDS heap packets up to 128
+// (integer metrics), per-lane heaps otherwise; bucket 32.
+static void default_tuning(ft_batch *b) {
+    const bool fp = b->coord_type == FT_COORD_F32 || b->coord_type == FT_COORD_F64;
+    for (int c = 0; c < N_CLASS; ++c)
+        for (int t = 0; t < N_KTIER; ++t) {
+            GroupTune &g = b->tune[c][t];
+            if (g.tuned) continue;
+            g.variant = candidates(t, fp).front();
+            g.bucket = 32;
+        }
+}
+
+// Device tables: trees, requests (with tuned settings), items per variant.
+template <typename T> static int upload_tables(ft_batch *b) {
+    using D = typename CoordTraits<T>::D;
+    (void)sizeof(D);
+    const int nt = (int)b->trees.size(), nr = (int)b->reqs.size();
+    std::vector<DTree<T>> dts(nt);
+    for (int i = 0; i < nt; ++i) dts[i] = make_dtree<T>(*b->trees[i]);
+    std::vector<BatchReq> rq(nr);
+    std::vector<std::vector<std::pair<double, uint2>>> items(V_COUNT);
+    int64_t entry = 0;
+    size_t lds = 0;
+    int kmax = 1;
+    for (const auto &r : b->reqs) kmax = std::max(kmax, (int)r.k);
+    b->kmax = kmax;
+    b->lane_width = lane_tier_lanes<T>(kmax, lds);
+    for (int i = 0; i < nr; ++i) {
+        const ft_batch_request &r = b->reqs[i];
+        const size_t n = b->build ? b->bn[r.tree] : b->trees[r.tree]->num_leaves;
+        GroupTune &g = b->tune[size_class(n)][k_tier(r.k)];
+        BatchReq &q = rq[i];
+        q.tree = r.tree;
+        q.k = r.k;
+        q.entry_first = (int32_t)entry;
+        q.num = (int32_t)r.num_queries;
+        q.bucket = g.bucket;
+        q.variant = g.variant;
+        q.query_first = r.query_first;
+        q.result_first = r.result_first;
+        entry += r.num_queries;
+        const int pw = is_packet(g.variant) ? 64 : b->lane_width;
+        const double cost = g.ms_per_item > 0 ? g.ms_per_item : (double)r.k * std::log2(2.0 + n);
+        if (n == 0) continue;  // an empty tree: results untouched, as knn_query
+        for (int j = 0; j * pw < r.num_queries; ++j)
+            items[g.variant].push_back({cost, make_uint2((unsigned)i, (unsigned)j)});
+    }
+    b->n_entries = entry;
+    std::vector<uint2> flat;
+    for (int v = 0; v < V_COUNT; ++v) {
+        auto &L = items[v];
+        std::stable_sort(L.begin(), L.end(),
+                         [](const std::pair<double, uint2> &a, const std::pair<double, uint2> &c) {
+                             return a.first > c.first;
+                         });
+        b->item_first[v] = (int)flat.size();
+        b->item_count[v] = (int)L.size();
+        for (auto &p : L) flat.push_back(p.second);
+    }
+    // one device block: trees | reqs | items | counters | rerun | keys | sorted keys
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t ne = (size_t)std::max<int64_t>(entry, 1);
+    const size_t o_t = 0, s_t = al(sizeof(DTree<T>) * std::max(nt, 1));
+    const size_t o_r = o_t + s_t, s_r = al(sizeof(BatchReq) * std::max(nr, 1));
+    const size_t o_i = o_r + s_r, s_i = al(sizeof(uint2) * std::max<size_t>(flat.size(), 1));
+    const size_t o_c = o_i + s_i, s_c = al(4 * (V_COUNT + 1));
+    const size_t o_l = o_c + s_c, s_l = al(4 * ne);
+    const size_t o_k = o_l + s_l, s_k = al(8 * ne);
+    const size_t o_s = o_k + s_k, s_s = al(8 * ne);
+    const size_t total = o_s + s_s;
+    if (total > b->dev_bytes) {
+        if (b->dev_block) FT_HIP(hipFree(b->dev_block));
+        b->dev_block = nullptr;
+        if (hipMalloc(&b->dev_block, total) != hipSuccess)
+            return fail(FT_ENOMEM, "batch table allocation failed");
+        b->dev_bytes = total;
+    }
+    char *base = (char *)b->dev_block;
+    b->dtrees = base + o_t;
+    b->dreqs = (BatchReq *)(base + o_r);
+    b->ditems = (uint2 *)(base + o_i);
+    b->dctr = (uint32_t *)(base + o_c);
+    b->drerun = (int32_t *)(base + o_l);
+    b->dkeys = (uint64_t *)(base + o_k);
+    b->dsorted = (uint64_t *)(base + o_s);
+    if (nt) FT_HIP(hipMemcpy(b->dtrees, dts.data(), sizeof(DTree<T>) * nt, hipMemcpyHostToDevice));
+    if (nr) FT_HIP(hipMemcpy(b->dreqs, rq.data(), sizeof(BatchReq) * nr, hipMemcpyHostToDevice));
+    if (!flat.empty())
+        FT_HIP(hipMemcpy(b->ditems, flat.data(), sizeof(uint2) * flat.size(),
+                         hipMemcpyHostToDevice));
+    FT_HIP(hipMemset(b->dctr, 0, 4 * (V_COUNT + 1)));
+    FT_TRY(scratch_ensure_public(b->sort_scratch, ne));
+    b->dirty = false;
+    return FT_OK;
+}
+
+template <typename T> static BatchView<T> view(ft_batch *b) {
+    BatchView<T> v;
+    v.trees = (const DTree<T> *)b->dtrees;
+    v.reqs = b->dreqs;
+    v.nreq = (int)b->reqs.size();
+    v.order = b->sort_scratch.perm;
+    v.qx = (const T *)b->qx;
+    v.qy = (const T *)b->qy;
+    v.qz = (const T *)b->qz;
+    v.res = b->results;
+    v.dists = (typename CoordTraits<T>::D *)b->dists;
+    v.rerun = b->drerun;
+    v.rerun_count = b->dctr + V_COUNT;
+    v.stats = b->stats;
+    return v;
+}
+
+// Launches one tier kernel over `n_items` items starting at `first`.
+template <typename T>
+static int launch_variant(ft_batch *b, hipStream_t s, int v, int first, int n_items,
+                          uint32_t *ctr) {
+    if (n_items <= 0) return FT_OK;
+    const BatchView<T> bv = view<T>(b);
+    const uint2 *it = b->ditems + first;
+    const unsigned waves = std::min<unsigned>((unsigned)n_items, 2048u);
+    constexpr bool FP = CoordTraits<T>::fp;
+    switch (v) {
+    case V_PK1: hipLaunchKernelGGL((batch_packet_kernel<T, 1, false>), dim3(waves), dim3(64), 0, s, bv, it, n_items, ctr); break;
+    case V_PK4: hipLaunchKernelGGL((batch_packet_kernel<T, 4, false>), dim3(waves), dim3(64), 0, s, bv, it, n_items, ctr); break;
+    case V_PK8: hipLaunchKernelGGL((batch_packet_kernel<T, 8, false>), dim3(waves), dim3(64), 0, s, bv, it, n_items, ctr); break;
+    case V_PK16: hipLaunchKernelGGL((batch_packet_kernel<T, 16, false>), dim3(waves), dim3(64), 0, s, bv, it, n_items, ctr); break;
+    case V_PK32: hipLaunchKernelGGL((batch_packet_kernel<T, 32, false>), dim3(waves), dim3(64), 0, s, bv, it, n_items, ctr); break;
+    case V_HP32:
+    case V_HP64:
+    case V_HP128:
+        if constexpr (FP) {
+            return fail(FT_EINVAL, "LDS heap packet tier needs integer coordinates");
+        } else {
+            if (v == V_HP32) hipLaunchKernelGGL((batch_packet_kernel<T, 32, true>), dim3(waves), dim3(64), 0, s, bv, it, n_items, ctr);
+            else if (v == V_HP64) hipLaunchKernelGGL((batch_packet_kernel<T, 64, true>), dim3(waves), dim3(64), 0, s, bv, it, n_items, ctr);
+            else hipLaunchKernelGGL((batch_packet_kernel<T, 128, true>), dim3(waves), dim3(64), 0, s, bv, it, n_items, ctr);
+        }
+        break;
+    case V_LANE: {
+        size_t lds = 0;
+        const int lanes = lane_tier_lanes<T>(b->kmax, lds);
+        if (!lanes) return fail(FT_ELDS, "k too large for local memory: reduce k");
+        hipLaunchKernelGGL((batch_lane_kernel<T>), dim3(waves), dim3(64), lds, s, bv, it, n_items,
+                           ctr, lanes, b->kmax);
+        break;
+    }
+    default: return fail(FT_EINVAL, "unknown batch tier");
+    }
+    FT_CHECK_LAUNCH();
+    return FT_OK;
+}
+
+template <typename T> static int launch_exact(ft_batch *b, hipStream_t s) {
+    size_t lds = 0;
+    const int lanes = lane_tier_lanes<T>(b->kmax, lds);
+    if (!lanes) return fail(FT_ELDS, "k too large for local memory: reduce k");
+    hipLaunchKernelGGL((batch_exact_kernel<T>), dim3(256), dim3(64), lds, s, view<T>(b), lanes,
+                       b->kmax);
+    FT_CHECK_LAUNCH();
+    if (b->stats) {
+        hipLaunchKernelGGL(batch_count_kernel, dim3(1), dim3(1), 0, s, b->stats,
+                           (const uint32_t *)(b->dctr + V_COUNT));
+        FT_CHECK_LAUNCH();
+    }
+    return FT_OK;
+}
+
+// Tree i goes to the branch with the least work so far, largest trees first.
+// Returns that order; br[i] = branch of tree i, maxn[t] = largest tree of t.
+static std::vector<int> assign_branches(ft_batch *b, int nb, std::vector<int> &br,
+                                        std::vector<size_t> &maxn) {
+    const int nt = (int)b->trees.size();
+    std::vector<int> order(nt);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return b->bn[a] > b->bn[c]; });
+    std::vector<size_t> load(nb, 0);
+    maxn.assign(nb, 0);
+    br.assign(nt, 0);
+    for (int i : order) {
+        const int t = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        br[i] = t;
+        load[t] += b->bn[i];
+        maxn[t] = std::max(maxn[t], b->bn[i]);
+    }
+    return order;
+}
+
+// Builds every tree on nb branches (nb = 1: in order on s).
+template <typename T> static int enqueue_builds(ft_batch *b, hipStream_t s, int nb) {
+    const int nt = (int)b->trees.size();
+    if (!b->build || nt == 0) return FT_OK;
+    nb = std::max(1, std::min(nb, nt));
+    FT_TRY(ensure_streams(b, nb));
+    while ((int)b->build_scratch.size() < nb) b->build_scratch.emplace_back();
+    std::vector<int> br;
+    std::vector<size_t> maxn;
+    const std::vector<int> order = assign_branches(b, nb, br, maxn);
+    for (int t = 0; t < nb; ++t) FT_TRY(scratch_ensure_public(b->build_scratch[t], maxn[t]));
+    if (nb > 1) {
+        FT_HIP(hipEventRecord(b->ev_fork, s));
+        for (int t = 0; t < nb; ++t) FT_HIP(hipStreamWaitEvent(b->side[t], b->ev_fork, 0));
+    }
+    for (int i : order) {
+        ft_tree *tr = b->trees[i];
+        const size_t n = b->bn[i];
+        if (n == 0) continue;
+        tr->num_leaves = n;
+        tr->num_internal = n - 1;
+        hipStream_t st = nb > 1 ? b->side[br[i]] : s;
+        FT_TRY(build_bvh_t<T>(st, tr, b->build_scratch[br[i]], (const T *)b->bx[i],
+                              (const T *)b->by[i], (const T *)b->bz[i], nullptr, nullptr, n,
+                              nullptr, nullptr));
+    }
+    if (nb > 1) {
+        for (int t = 0; t < nb; ++t) {
+            FT_HIP(hipEventRecord(b->ev_join[t], b->side[t]));
+            FT_HIP(hipStreamWaitEvent(s, b->ev_join[t], 0));
+        }
+    }
+    return FT_OK;
+}
+
+static int bit_len(uint64_t v) {
+    int r = 0;
+    while (v) {
+        ++r;
+        v >>= 1;
+    }
+    return r;
+}
+
+// Query ordering (composite-key sort).
+template <typename T> static int enqueue_sort(ft_batch *b, hipStream_t s) {
+    if (b->n_entries == 0) return FT_OK;
+    const int nreq = (int)b->reqs.size();
+    const int rbits = std::max(1, bit_len((uint64_t)(nreq - 1)));
+    const int qb = std::min(21, (64 - rbits) / 3);
+    hipLaunchKernelGGL((batch_key_kernel<T>), dim3(cdiv(b->n_entries, 256)), dim3(256), 0, s,
+                       view<T>(b), b->n_entries, rbits, qb, b->dkeys);
+    FT_CHECK_LAUNCH();
+    return radix_sort_u64(s, b->sort_scratch, b->dkeys, (size_t)b->n_entries, b->dsorted);
+}
+
+// Tier kernels on parallel branches (or in order on s), then the exact pass.
+template <typename T> static int enqueue_queries(ft_batch *b, hipStream_t s) {
+    if (b->n_entries == 0) return FT_OK;
+    hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(64), 0, s, b->dctr, V_COUNT + 1);
+    FT_CHECK_LAUNCH();
+    std::vector<int> used;
+    for (int v = 0; v < V_COUNT; ++v)
+        if (b->item_count[v] > 0) used.push_back(v);
+    const int nb = (int)used.size();
+    if (nb > 1) {
+        FT_TRY(ensure_streams(b, nb));
+        FT_HIP(hipEventRecord(b->ev_fork, s));
+        for (int i = 0; i < nb; ++i) FT_HIP(hipStreamWaitEvent(b->side[i], b->ev_fork, 0));
+    }
+    for (int i = 0; i < nb; ++i) {
+        const int v = used[i];
+        FT_TRY(launch_variant<T>(b, nb > 1 ? b->side[i] : s, v, b->item_first[v],
+                                 b->item_count[v], b->dctr + v));
+    }
+    if (nb > 1) {
+        for (int i = 0; i < nb; ++i) {
+            FT_HIP(hipEventRecord(b->ev_join[i], b->side[i]));
+            FT_HIP(hipStreamWaitEvent(s, b->ev_join[i], 0));
+        }
+    }
+    return launch_exact<T>(b, s);
+}
+
+template <typename T> static int enqueue_step(ft_batch *b, hipStream_t s) {
+    if (b->dirty) FT_TRY(upload_tables<T>(b));
+    FT_TRY(enqueue_builds<T>(b, s, b->branches));
+    FT_TRY(enqueue_sort<T>(b, s));
+    return enqueue_queries<T>(b, s);
+}
+
+static float elapsed(hipEvent_t a, hipEvent_t c) {
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, a, c);
+    return ms;
+}
+
+// Autotuner.  Builds once, sorts the queries once, then for every (size class,
+// k tier) group with work times each candidate tier kernel (and leaf bucket for
+// packet tiers) on that group's items alone, keeping the fastest; then times
+// the build phase at 1, 2, 4 and 8 branches.
+template <typename T> static int tune_t(ft_batch *b, hipStream_t s) {
+    const bool fp = CoordTraits<T>::fp;
+    default_tuning(b);
+    FT_TRY(upload_tables<T>(b));
+    FT_TRY(enqueue_builds<T>(b, s, b->branches));
+    FT_TRY(enqueue_sort<T>(b, s));
+    FT_HIP(hipStreamSynchronize(s));
+    hipEvent_t e0, e1;
+    FT_HIP(hipEventCreate(&e0));
+    FT_HIP(hipEventCreate(&e1));
+    // Items of every group, under a fixed packet width per tier (64 for
+    // packet tiers, lane_width for the per-lane tier).
+    const int nr = (int)b->reqs.size();
+    int rc = FT_OK;
+    for (int c = 0; c < N_CLASS && rc == FT_OK; ++c) {
+        for (int kt = 0; kt < N_KTIER && rc == FT_OK; ++kt) {
+            std::vector<int> members;
+            for (int i = 0; i < nr; ++i) {
+                const auto &r = b->reqs[i];
+                const size_t n = b->build ? b->bn[r.tree] : b->trees[r.tree]->num_leaves;
+                if (r.num_queries > 0 && size_class(n) == c && k_tier(r.k) == kt)
+                    members.push_back(i);
+            }
+            if (members.empty()) continue;
+            double best = 1e30;
+            int best_v = -1, best_bucket = 32, best_items = 1;
+            for (int v : candidates(kt, fp)) {
+                const int pw = is_packet(v) ? 64 : b->lane_width;
+                std::vector<uint2> its;
+                for (int i : members)
+                    for (int j = 0; j * pw < b->reqs[i].num_queries; ++j)
+                        its.push_back(make_uint2((unsigned)i, (unsigned)j));
+                // scratch item list at the end of the item table's region is not
+                // guaranteed large enough: use a temporary device buffer
+                uint2 *tmp = nullptr;
+                if (hipMalloc(&tmp, its.size() * sizeof(uint2)) != hipSuccess) {
+                    rc = fail(FT_ENOMEM, "tuner allocation failed");
+                    break;
+                }
+                (void)hipMemcpy(tmp, its.data(), its.size() * sizeof(uint2), hipMemcpyHostToDevice);
+                uint2 *saved = b->ditems;
+                const std::vector<int> buckets =
+                    is_packet(v) ? std::vector<int>{16, 32, 64} : std::vector<int>{32};
+                for (int bk : buckets) {
+                    // set the bucket of the group's requests on the device
+                    for (int i : members) {
+                        BatchReq q;
+                        (void)hipMemcpy(&q, b->dreqs + i, sizeof q, hipMemcpyDeviceToHost);
+                        q.bucket = bk;
+                        q.variant = v;
+                        (void)hipMemcpy(b->dreqs + i, &q, sizeof q, hipMemcpyHostToDevice);
+                    }
+                    b->ditems = tmp;
+                    float t_best = 1e30f;
+                    for (int rep = 0; rep < 2 && rc == FT_OK; ++rep) {
+                        hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(64), 0, s, b->dctr,
+                                           V_COUNT + 1);
+                        (void)hipEventRecord(e0, s);
+                        rc = launch_variant<T>(b, s, v, 0, (int)its.size(), b->dctr);
+                        (void)hipEventRecord(e1, s);
+                        (void)hipEventSynchronize(e1);
+                        t_best = std::min(t_best, elapsed(e0, e1));
+                    }
+                    b->ditems = saved;
+                    if (rc != FT_OK) break;
+                    if (t_best < best) {
+                        best = t_best;
+                        best_v = v;
+                        best_bucket = bk;
+                        best_items = (int)its.size();
+                    }
+                }
+                (void)hipFree(tmp);
+                if (rc != FT_OK) break;
+            }
+            if (rc != FT_OK) break;
+            GroupTune &g = b->tune[c][kt];
+            g.variant = best_v;
+            g.bucket = best_bucket;
+            g.ms_per_item = best / std::max(best_items, 1);
+            g.tuned = true;
+        }
+    }
+    // Build branches.
+    if (rc == FT_OK && b->build && b->trees.size() > 1) {
+        double best = 1e30;
+        int best_nb = 1;
+        for (int nb : {1, 2, 4, 8}) {
+            if (nb > (int)b->trees.size()) break;
+            float t_best = 1e30f;
+            for (int rep = 0; rep < 2 && rc == FT_OK; ++rep) {
+                (void)hipEventRecord(e0, s);
+                rc = enqueue_builds<T>(b, s, nb);
+                (void)hipEventRecord(e1, s);
+                (void)hipEventSynchronize(e1);
+                t_best = std::min(t_best, elapsed(e0, e1));
+            }
+            if (t_best < best) {
+                best = t_best;
+                best_nb = nb;
+            }
+        }
+        b->branches = best_nb;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    FT_TRY(rc);
+    b->tuned = true;
+    b->dirty = true;
+    free_graph(b);
+    return upload_tables<T>(b);
+}
+
+// Sizes every table, stream and scratch buffer the step uses, so that the
+// step itself (and a graph capture of it) allocates nothing.
+template <typename T> static int prepare_t(ft_batch *b) {
+    if (b->dirty) FT_TRY(upload_tables<T>(b));
+    FT_TRY(ensure_streams(b, std::max(b->branches, (int)V_COUNT)));
+    if (b->build && !b->trees.empty()) {
+        const int nb = std::max(1, std::min(b->branches, (int)b->trees.size()));
+        std::vector<int> br;
+        std::vector<size_t> maxn;
+        assign_branches(b, nb, br, maxn);
+        while ((int)b->build_scratch.size() < nb) b->build_scratch.emplace_back();
+        for (int t = 0; t < nb; ++t) FT_TRY(scratch_ensure_public(b->build_scratch[t], maxn[t]));
+    }
+    return FT_OK;
+}
+
+// Records the step on the batch's own capture stream; the graph is then
+// launched on any stream (also the null stream, which cannot be captured).
+template <typename T> static int capture_t(ft_batch *b, hipStream_t caller) {
+    free_graph(b);
+    if (!b->tuned) default_tuning(b);
+    FT_TRY(prepare_t<T>(b));
+    if (!b->cap) FT_HIP(hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking));
+    FT_HIP(hipStreamSynchronize(caller));  // table uploads done before recording
+    const hipStream_t s = b->cap;
+    FT_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+    const int rc = enqueue_step<T>(b, s);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(s, &g);
+    if (rc != FT_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess)
+        return fail(FT_EHIP, std::string("graph capture: ") + hipGetErrorString(e));
+    b->graph = g;
+    FT_HIP(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+    return FT_OK;
+}
+
+}  // namespace ft
+
+using namespace ft;
+
+#define FT_BATCH_DISPATCH(b, CALL)                                                   \
+    switch ((b)->coord_type) {                                                       \
+    case FT_COORD_F32: { using T = float; return CALL; }                             \
+    case FT_COORD_F64: { using T = double; return CALL; }                            \
+    case FT_COORD_U32: { using T = uint32_t; return CALL; }                          \
+    case FT_COORD_U64: { using T = uint64_t; return CALL; }                          \
+    default: return fail(FT_EINVAL, "unknown coordinate type");                      \
+    }
+
+extern "C" {
+
+int ft_batch_create(ft_batch **out, int32_t coord_type) {
+    if (!out) return fail(FT_EINVAL, "null batch");
+    if (coord_type < FT_COORD_F32 || coord_type > FT_COORD_U64)
+        return fail(FT_EINVAL, "unknown coordinate type");
+    ft_batch *b = new ft_batch;
+    b->coord_type = coord_type;
+    *out = b;
+    return FT_OK;
+}
+
+int ft_batch_destroy(ft_batch *b) {
+    if (!b) return FT_OK;
+    (void)hipDeviceSynchronize();
+    free_graph(b);
+    for (hipStream_t st : b->side) (void)hipStreamDestroy(st);
+    if (b->cap) (void)hipStreamDestroy(b->cap);
+    for (hipEvent_t ev : b->ev_join) (void)hipEventDestroy(ev);
+    if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
+    if (b->dev_block) (void)hipFree(b->dev_block);
+    if (b->sort_scratch.block) (void)hipFree(b->sort_scratch.block);
+    for (auto &S : b->build_scratch)
+        if (S.block) (void)hipFree(S.block);
+    delete b;
+    return FT_OK;
+}
+
+int ft_batch_set_trees(ft_batch *b, int num_trees, ft_tree *const *trees, const void *const *x,
+                       const void *const *y, const void *const *z, const size_t *n) {
+    if (!b || num_trees < 0 || (num_trees > 0 && !trees)) return fail(FT_EINVAL, "bad tree list");
+    const bool build = x != nullptr;
+    if (build && (!y || !z || !n)) return fail(FT_EINVAL, "build inputs need x, y, z and n");
+    b->trees.assign(trees, trees + num_trees);
+    b->build = build;
+    b->bx.clear(); b->by.clear(); b->bz.clear(); b->bn.clear();
+    for (int i = 0; i < num_trees; ++i) {
+        ft_tree *t = b->trees[i];
+        if (!t || !t->block) return fail(FT_EINVAL, "tree not allocated");
+        if (t->coord_type != b->coord_type)
+            return fail(FT_EINVAL, "tree coordinate type differs from the batch's");
+        if (build) {
+            if (n[i] > t->capacity)
+                return fail(FT_EINVAL, "tree capacity is smaller than the particle count");
+            if (n[i] > (size_t)(INT_MAX / 2))
+                return fail(FT_ERANGE, "Particle count n exceeds maximum supported tree size (INT_MAX / 2).");
+            b->bx.push_back(x[i]); b->by.push_back(y[i]); b->bz.push_back(z[i]);
+            b->bn.push_back(n[i]);
+            t->num_leaves = n[i];  // the batch builds it with this many particles
+            t->num_internal = n[i] ? n[i] - 1 : 0;
+        }
+    }
+    b->dirty = true;
+    free_graph(b);
+    return FT_OK;
+}
+
+int ft_batch_set_queries(ft_batch *b, int num_requests, const ft_batch_request *reqs,
+                         const void *qx, const void *qy, const void *qz, uint64_t *results,
+                         void *dists) {
+    if (!b || num_requests < 0 || (num_requests > 0 && !reqs))
+        return fail(FT_EINVAL, "bad request list");
+    int64_t total = 0;
+    for (int i = 0; i < num_requests; ++i) {
+        const ft_batch_request &r = reqs[i];
+        if (r.tree < 0 || r.tree >= (int)b->trees.size())
+            return fail(FT_EINVAL, "request names a tree outside the batch");
+        if (r.k < 1 || r.k > 256) return fail(FT_EINVAL, "batch requests need 1 <= k <= 256");
+        if (r.num_queries < 0 || r.query_first < 0 || r.result_first < 0)
+            return fail(FT_EINVAL, "negative query range");
+        total += r.num_queries;
+    }
+    if (total > (int64_t)INT_MAX / 2) return fail(FT_ERANGE, "too many batch queries");
+    if (total > 0 && (!qx || !qy || !qz || !results || !dists))
+        return fail(FT_EINVAL, "null query or result pointer");
+    b->reqs.assign(reqs, reqs + num_requests);
+    b->qx = qx; b->qy = qy; b->qz = qz;
+    b->results = results;
+    b->dists = dists;
+    b->dirty = true;
+    free_graph(b);
+    return FT_OK;
+}
+
+int ft_batch_set_stats(ft_batch *b, uint64_t *stats) {
+    if (!b) return fail(FT_EINVAL, "null batch");
+    b->stats = stats;
+    b->dirty = true;
+    free_graph(b);
+    return FT_OK;
+}
+
+int ft_batch_tune(ft_batch *b, ft_stream s) {
+    if (!b) return fail(FT_EINVAL, "null batch");
+    FT_BATCH_DISPATCH(b, tune_t<T>(b, as_stream(s)));
+}
+
+int ft_batch_run(ft_batch *b, ft_stream s) {
+    if (!b) return fail(FT_EINVAL, "null batch");
+    if (!b->tuned) default_tuning(b);
+    FT_BATCH_DISPATCH(b, enqueue_step<T>(b, as_stream(s)));
+}
+
+int ft_batch_capture(ft_batch *b, ft_stream s) {
+    if (!b) return fail(FT_EINVAL, "null batch");
+    FT_BATCH_DISPATCH(b, capture_t<T>(b, as_stream(s)));
+}
+
+int ft_batch_launch(ft_batch *b, ft_stream s) {
+    if (!b) return fail(FT_EINVAL, "null batch");
+    if (!b->exec) FT_TRY(ft_batch_capture(b, s));
+    FT_HIP(hipGraphLaunch(b->exec, as_stream(s)));
+    return FT_OK;
+}
+
+// Writes the chosen settings as JSON text: per group (size class, k tier) the
+// tier kernel, leaf bucket and measured cost per item; the build branch count.
+int ft_batch_tuning(ft_batch *b, char *buf, size_t len) {
+    if (!b || !buf || len == 0) return fail(FT_EINVAL, "null batch or buffer");
+    if (!b->tuned) default_tuning(b);
+    std::string out = "{\"tuned\": ";
+    out += b->tuned ? "true" : "false";
+    out += ", \"build_branches\": " + std::to_string(b->branches);
+    out += ", \"radix_bits\": 8, \"groups\": [";
+    bool first = true;
+    static const char *const cls[N_CLASS] = {"<40k", "<160k", "<640k", ">=640k"};
+    for (int c = 0; c < N_CLASS; ++c)
+        for (int t = 0; t < N_KTIER; ++t) {
+            const GroupTune &g = b->tune[c][t];
+            if (!g.tuned) continue;
+            char line[256];
+            std::snprintf(line, sizeof line,
+                          "%s{\"n_class\": \"%s\", \"k_max\": %d, \"tier\": \"%s\", "
+                          "\"bucket\": %d, \"ms_per_item\": %.6f}",
+                          first ? "" : ", ", cls[c], kTierCap[t], kVariantName[g.variant],
+                          g.bucket, g.ms_per_item);
+            out += line;
+            first = false;
+        }
+    out += "]}";
+    if (out.size() + 1 > len) return fail(FT_ERANGE, "tuning text does not fit the buffer");
+    std::memcpy(buf, out.c_str(), out.size() + 1);
+    return FT_OK;
+}
+
+}  // extern "C"

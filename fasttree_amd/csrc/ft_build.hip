@@ -829,6 +829,14 @@ static int radix_sort(hipStream_t s, Scratch &S, const uint64_t *keys_in, size_t
     return FT_OK;
 }
 
+// Stable sort of n 64-bit keys (all 64 bits) for other modules: sorted keys
+// to sorted_out, the permutation (sorted position -> input index) to S.perm.
+int radix_sort_u64(hipStream_t s, Scratch &S, const uint64_t *keys_in, size_t n,
+                   uint64_t *sorted_out) {
+    FT_TRY(scratch_ensure(S, n));
+    return radix_sort(s, S, keys_in, n, 64, sorted_out, false);
+}
+
 template <typename T> static int launch_topology(hipStream_t s, ft_tree *t, Scratch &S) {
     const int n = (int)t->num_leaves;
     if (n == 1) {

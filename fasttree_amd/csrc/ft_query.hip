@@ -14,7 +14,7 @@
 //
 // Canonical results: the k smallest (d2, returned index) pairs in ascending
 // order -- a unique answer, independent of traversal order.
-#include "ft_internal.hpp"
+#include "ft_traverse.hpp"
 
 #include <algorithm>
 #include <vector>
@@ -23,139 +23,6 @@ namespace ft {
 
 int scratch_ensure_public(Scratch &S, size_t n);
 
-template <typename D> __device__ inline D dist_max() {
-    if constexpr (std::is_floating_point<D>::value) return DBL_MAX;
-    else return ~0ull;
-}
-template <typename D> __device__ inline D dist_lowest() {
-    if constexpr (std::is_floating_point<D>::value) return -DBL_MAX;
-    else return 0ull;
-}
-
-__device__ inline uint32_t rl32(uint32_t v, int l) { return (uint32_t)__builtin_amdgcn_readlane((int)v, l); }
-__device__ inline uint64_t rl(uint64_t v, int l) {
-    return ((uint64_t)rl32((uint32_t)(v >> 32), l) << 32) | rl32((uint32_t)v, l);
-}
-__device__ inline uint32_t rl(uint32_t v, int l) { return rl32(v, l); }
-__device__ inline int32_t rl(int32_t v, int l) { return (int32_t)rl32((uint32_t)v, l); }
-__device__ inline double rl(double v, int l) { return __longlong_as_double((long long)rl((uint64_t)__double_as_longlong(v), l)); }
-__device__ inline float rl(float v, int l) { return __uint_as_float(rl32(__float_as_uint(v), l)); }
-
-__device__ inline int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
-
-// Per-lane source lane (ds_bpermute): lane i receives v from lane src_i.
-__device__ inline uint32_t shfl32(uint32_t v, int src) {
-    return (uint32_t)__builtin_amdgcn_ds_bpermute(src << 2, (int)v);
-}
-__device__ inline uint64_t shfl_any(uint64_t v, int src) {
-    return ((uint64_t)shfl32((uint32_t)(v >> 32), src) << 32) | shfl32((uint32_t)v, src);
-}
-__device__ inline uint32_t shfl_any(uint32_t v, int src) { return shfl32(v, src); }
-__device__ inline int32_t shfl_any(int32_t v, int src) { return (int32_t)shfl32((uint32_t)v, src); }
-__device__ inline double shfl_any(double v, int src) {
-    return __longlong_as_double((long long)shfl_any((uint64_t)__double_as_longlong(v), src));
-}
-__device__ inline float shfl_any(float v, int src) { return __uint_as_float(shfl32(__float_as_uint(v), src)); }
-
-__device__ inline void wave_add_stats(uint64_t *stats, int slot, uint64_t v) {
-    if (!stats) return;
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    if ((threadIdx.x & 63) == 0 && v) atomicAdd((unsigned long long *)&stats[slot], (unsigned long long)v);
-}
-
-// Query source: external points, or leaves of the tree (self queries).
-template <typename T> struct QSrc {
-    const T *qx, *qy, *qz;   // external points (null for self)
-    const int32_t *leaf_ids; // subset leaf ranks (null = all leaves)
-};
-
-template <typename T>
-__device__ inline void load_query(const DTree<T> &t, const QSrc<T> &qs, int q, T &px, T &py, T &pz,
-                                  int &self_rank) {
-    if (qs.qx) {
-        px = qs.qx[q]; py = qs.qy[q]; pz = qs.qz[q];
-        self_rank = -1;
-    } else {
-        const int r = qs.leaf_ids ? qs.leaf_ids[q] : q;
-        const int leaf = r + t.n - 1;
-        px = t.mnx[leaf]; py = t.mny[leaf]; pz = t.mnz[leaf];
-        self_rank = r;
-    }
-}
-
-template <typename T>
-__device__ inline typename CoordTraits<T>::D node_d2(const DTree<T> &t, T px, T py, T pz, int node) {
-    return box_d2<T>(t.m, px, py, pz, t.mnx[node], t.mxx[node], t.mny[node], t.mxy[node],
-                     t.mnz[node], t.mxz[node]);
-}
-
-// Returned index of a leaf: its rank, or orig_idx with RETURN_ORIG_INDICES.
-template <typename T> __device__ inline int32_t ret_index(const DTree<T> &t, int leaf_node) {
-    return t.orig_indices ? t.orig[leaf_node] : (leaf_node - (t.n - 1));
-}
-
-// ---------------------------------------------------------------------------
-// Canonical register top-k: ascending (d, idx) in M fixed registers, the live
-// k entries right-aligned so the worst is always slot M-1.  Slots below M-k
-// hold (-inf, -1) and never move.  Insertion is a fixed compare/shift network.
-// ---------------------------------------------------------------------------
-template <typename D, int M> struct TopK {
-    D d[M];
-    int32_t i[M];
-    __device__ inline void init(int k, D bound) {
-#pragma unroll
-        for (int s = 0; s < M; ++s) {
-            const bool live = s >= M - k;
-            d[s] = live ? bound : dist_lowest<D>();
-            i[s] = live ? INT32_MAX : -1;
-        }
-    }
-    __device__ inline D worst() const { return d[M - 1]; }
-    __device__ inline bool beats_worst(D cd, int32_t ci) const {
-        return (cd < d[M - 1]) | ((cd == d[M - 1]) & (ci < i[M - 1]));
-    }
-    // Predicated insertion as a compare-and-swap chain from the smallest slot:
-    // the carried entry swaps into every slot it is below, so it lands in order
-    // and the former worst drops off the top.  Every slot stays in its own
-    // registers; the comparisons are branch-free.
-    __device__ inline void insert(bool take, D cd, int32_t ci) {
-#pragma unroll
-        for (int s = 0; s < M; ++s) {
-            const bool lt = take & ((cd < d[s]) | ((cd == d[s]) & (ci < i[s])));
-            const D td = d[s];
-            const int32_t ti = i[s];
-            d[s] = lt ? cd : td;
-            i[s] = lt ? ci : ti;
-            cd = lt ? td : cd;
-            ci = lt ? ti : ci;
-        }
-    }
-    __device__ inline D worst_d() const { return d[M - 1]; }
-    // Number of live slots still holding the initial sentinel.
-    __device__ inline int missing(int k) const {
-        int c = 0;
-#pragma unroll
-        for (int s = 0; s < M; ++s) c += (s >= M - k && i[s] == INT32_MAX) ? 1 : 0;
-        return c;
-    }
-    template <typename DOut>
-    __device__ inline void store(int k, uint64_t *res, DOut *dists, size_t q) const {
-        const size_t o = q * (size_t)k;
-#pragma unroll
-        for (int s = 0; s < M; ++s) {
-            if (s >= M - k) {
-                const size_t j = o + (size_t)(s - (M - k));
-                const bool ok = i[s] != INT32_MAX;
-                res[j] = ok ? (uint64_t)(uint32_t)i[s] : ~0ull;
-                dists[j] = ok ? (DOut)d[s] : missing_dist<DOut>();
-            }
-        }
-    }
-};
-
-constexpr int LANE_BLOCK = 64;
-constexpr int STACK_DEPTH = 64;  // MAX_STACK_DEPTH (src/hlbvh.hpp:984)
 
 // ===========================================================================
 // Per-lane DFS with the canonical register top-k (k <= M <= 32).
@@ -169,7 +36,7 @@ __global__ __launch_bounds__(LANE_BLOCK) void lane_knn_reg_kernel(DTree<T> t, QS
                                                                   typename CoordTraits<T>::D *__restrict__ dists,
                                                                   uint64_t *__restrict__ stats) {
     using D = typename CoordTraits<T>::D;
-    __shared__ int32_t stk[STACK_DEPTH][LANE_BLOCK];
+    __shared__ int32_t stk[LANE_STACK_CANON][LANE_BLOCK];
     const int lane = threadIdx.x;
     const int q = blockIdx.x * LANE_BLOCK + lane;
     const bool active = q < nq;
@@ -203,7 +70,7 @@ __global__ __launch_bounds__(LANE_BLOCK) void lane_knn_reg_kernel(DTree<T> t, QS
             const int l = t.left[node], r = t.right[node];
             const D ld = node_d2(t, px, py, pz, l);
             const D rd = node_d2(t, px, py, pz, r);
-            if (sp < STACK_DEPTH - 2) {
+            if (sp < LANE_STACK_CANON - 2) {
                 if (ld <= rd) {
                     stk[sp++][lane] = r;
                     stk[sp++][lane] = l;
@@ -222,139 +89,47 @@ __global__ __launch_bounds__(LANE_BLOCK) void lane_knn_reg_kernel(DTree<T> t, QS
     wave_add_stats(stats, 4, inserts);
 }
 
-// ===========================================================================
-// Per-lane DFS with a max-heap row per lane in LDS (LocalMaxHeap,
-// src/maxheap.hpp:340-478).  REF = the reference's mechanics exactly (compare
-// d only, `>=` prune, sift rules, heapsort), so tie picks match; otherwise the
-// heap is keyed by (d, idx) and pruning is strict (canonical).  Layout
-// [slot][lane] so lanes at the same slot hit distinct banks.
-// ===========================================================================
-template <typename D, bool REF> struct LdsHeap {
-    D *dist;       // [cap][lanes]
-    int32_t *idx;  // [cap][lanes]
-    int lanes, lane, count;
-    __device__ inline D &dd(int s) { return dist[s * lanes + lane]; }
-    __device__ inline int32_t &ii(int s) { return idx[s * lanes + lane]; }
-    __device__ inline bool gt(int a, int b) {  // entry a > entry b
-        if (REF) return dd(a) > dd(b);
-        return dd(a) > dd(b) || (dd(a) == dd(b) && ii(a) > ii(b));
-    }
-    __device__ inline void swp(int a, int b) {
-        D td = dd(a); dd(a) = dd(b); dd(b) = td;
-        int32_t ti = ii(a); ii(a) = ii(b); ii(b) = ti;
-    }
-    __device__ inline bool prune(D d, int k) {
-        if (count < k) return false;
-        return REF ? (d >= dd(0)) : (d > dd(0));
-    }
-    __device__ inline void sift_up(int i) {
-        while (i > 0) {
-            const int p = (i - 1) >> 1;
-            if (!gt(i, p)) break;  // reference: dist[i] <= dist[p] -> stop
-            swp(i, p);
-            i = p;
-        }
-    }
-    __device__ inline void sift_down(int i, int size) {
-        while (true) {
-            int lg = i;
-            const int l = 2 * i + 1, r = 2 * i + 2;
-            if (l < size && gt(l, lg)) lg = l;
-            if (r < size && gt(r, lg)) lg = r;
-            if (lg == i) break;
-            swp(i, lg);
-            i = lg;
-        }
-    }
-    __device__ inline bool push(D d, int32_t i, int k) {
-        if (count < k) {
-            dd(count) = d;
-            ii(count) = i;
-            sift_up(count);
-            count++;
-            return true;
-        }
-        const bool better = REF ? (d < dd(0)) : (d < dd(0) || (d == dd(0) && i < ii(0)));
-        if (better) {
-            dd(0) = d;
-            ii(0) = i;
-            sift_down(0, k);
-            return true;
-        }
-        return false;
-    }
-    __device__ inline void sort() {
-        int hs = count;
-        while (hs > 1) {
-            swp(0, hs - 1);
-            hs--;
-            sift_down(0, hs);
-        }
-    }
-};
-
-template <typename T, bool REF, bool GROUPED>
-__global__ __launch_bounds__(LANE_BLOCK) void lane_knn_lds_kernel(DTree<T> t, QSrc<T> qs, int nq,
-                                                                  int k, int excl, int lanes,
-                                                                  uint64_t *__restrict__ res,
-                                                                  typename CoordTraits<T>::D *__restrict__ dists,
-                                                                  uint64_t *__restrict__ stats) {
+// Per-lane DFS with an LDS heap row per lane (LocalMaxHeap, src/maxheap.hpp:340-478):
+// REF = the reference's mechanics (ties as the reference picks them), else
+// canonical.  LISTED: the block's lanes take entries of a device list of query
+// positions (grid-stride; the list length is read on the device).
+template <typename T, bool REF, bool GROUPED, bool LISTED>
+__global__ __launch_bounds__(LANE_BLOCK) void lane_knn_lds_kernel(
+    DTree<T> t, QSrc<T> qs, int nq, int k, int excl, int lanes, const int32_t *__restrict__ list,
+    const uint32_t *__restrict__ list_count, uint64_t *__restrict__ res,
+    typename CoordTraits<T>::D *__restrict__ dists, uint64_t *__restrict__ stats) {
     using D = typename CoordTraits<T>::D;
+    constexpr int DEPTH = REF ? STACK_DEPTH : LANE_STACK_CANON;
     extern __shared__ __align__(16) unsigned char lds[];
     D *hd = (D *)lds;                                   // [k][lanes]
     int32_t *hi = (int32_t *)(hd + (size_t)k * lanes);  // [k][lanes]
-    int32_t *stk = hi + (size_t)k * lanes;              // [64][lanes]
+    int32_t *stk = hi + (size_t)k * lanes;              // [DEPTH][lanes]
+    (void)DEPTH;
     const int lane = threadIdx.x;
     const bool lane_ok = lane < lanes;
-    const int q = blockIdx.x * lanes + lane;
-    const bool active = lane_ok && q < nq;
-    const int n = t.n;
-    T px = T(0), py = T(0), pz = T(0);
-    int self = -1;
-    if (active) load_query(t, qs, q, px, py, pz, self);
-    LdsHeap<D, REF> h{hd, hi, lanes, lane, 0};
+    const int nlist = LISTED ? (int)*list_count : nq;
     uint64_t visits = 0, drops = 0, inserts = 0;
-    int sp = 0;
-    if (active) stk[(sp++) * lanes + lane] = 0;
-    while (sp > 0) {
-        const int node = stk[(--sp) * lanes + lane];
-        const D d2 = node_d2(t, px, py, pz, node);
-        ++visits;
-        const bool pruned = h.prune(d2, k);
-        if (GROUPED) {
-            if (!__any(!pruned)) continue;
-        }
-        if (pruned) continue;
-        if (node >= n - 1 || n == 1) {
-            const int leaf = (n == 1) ? 0 : node;
-            const int rank = leaf - (n - 1);
-            if (excl && rank == self) continue;
-            if (h.push(d2, ret_index(t, leaf), k)) ++inserts;
-        } else {
-            const int l = t.left[node], r = t.right[node];
-            const D ld = node_d2(t, px, py, pz, l);
-            const D rd = node_d2(t, px, py, pz, r);
-            if (sp < STACK_DEPTH - 2) {
-                if (ld <= rd) {
-                    stk[(sp++) * lanes + lane] = r;
-                    stk[(sp++) * lanes + lane] = l;
-                } else {
-                    stk[(sp++) * lanes + lane] = l;
-                    stk[(sp++) * lanes + lane] = r;
-                }
-            } else {
-                ++drops;
+    for (int base = blockIdx.x * lanes; base < nlist; base += (LISTED ? gridDim.x * lanes : nlist)) {
+        const int slot = base + lane;
+        const bool active = lane_ok && slot < nlist;
+        const int q = active ? (LISTED ? list[slot] : slot) : -1;
+        T px = T(0), py = T(0), pz = T(0);
+        int self = -1;
+        if (active) load_query(t, qs, q, px, py, pz, self);
+        if (!excl) self = -1;
+        LdsHeap<D, REF> h{hd, hi, lanes, lane, 0};
+        lane_heap_walk<T, REF, GROUPED>(t, k, active, px, py, pz, self, h, stk, lanes, lane,
+                                        visits, drops, inserts);
+        if (active) {
+            h.sort();
+            const size_t o = (size_t)q * k;
+            for (int s = 0; s < k; ++s) {
+                const bool ok = s < h.count && h.ii(s) >= 0;
+                res[o + s] = ok ? (uint64_t)(uint32_t)h.ii(s) : ~0ull;
+                dists[o + s] = ok ? h.dd(s) : missing_dist<D>();
             }
         }
-    }
-    if (active) {
-        h.sort();
-        const size_t o = (size_t)q * k;
-        for (int s = 0; s < k; ++s) {
-            const bool ok = s < h.count && h.ii(s) >= 0;
-            res[o + s] = ok ? (uint64_t)(uint32_t)h.ii(s) : ~0ull;
-            dists[o + s] = ok ? h.dd(s) : missing_dist<D>();
-        }
+        if (!LISTED) break;
     }
     wave_add_stats(stats, 0, visits);
     wave_add_stats(stats, 2, drops);
@@ -487,70 +262,6 @@ __global__ __launch_bounds__(64) void merge_knn_kernel(DTree<T> t, QSrc<T> qs, i
     }
 }
 
-// ---------------------------------------------------------------------------
-// Packed top-k for integer distances: key = min(d2, SAT) << ib | index, one
-// 64-bit compare orders by (d2, index).  ib = bit length of n, so every index
-// is below 2^ib - 1 and an all-ones index marks a sentinel.  A key whose d2
-// part saturated cannot be ordered exactly; the traversal reports such queries
-// (and queries left with sentinels) for an exact re-run.
-// ---------------------------------------------------------------------------
-template <int M> struct TopKPacked {
-    uint64_t key[M];
-    int ib;
-    uint64_t sat, imask;
-    __device__ inline void init(int k, uint64_t bound, int ibits) {
-        ib = ibits;
-        sat = (1ull << (64 - ib)) - 1;
-        imask = (1ull << ib) - 1;
-        const uint64_t sent = ((bound < sat ? bound : sat) << ib) | imask;
-#pragma unroll
-        for (int s = 0; s < M; ++s) key[s] = (s >= M - k) ? sent : 0ull;
-    }
-    __device__ inline uint64_t make(uint64_t d, int32_t i) const {
-        return ((d < sat ? d : sat) << ib) | (uint64_t)(uint32_t)i;
-    }
-    __device__ inline uint64_t worst_d() const { return key[M - 1] >> ib; }
-    __device__ inline bool beats_worst(uint64_t d, int32_t i) const { return make(d, i) < key[M - 1]; }
-    __device__ inline void insert(bool take, uint64_t d, int32_t i) {
-        uint64_t c = make(d, i);
-#pragma unroll
-        for (int s = 0; s < M; ++s) {
-            const bool lt = take & (c < key[s]);
-            const uint64_t t = key[s];
-            key[s] = lt ? c : t;
-            c = lt ? t : c;
-        }
-    }
-    // Sentinels left in live slots, or a saturated k-th distance.
-    __device__ inline bool needs_exact(int k) const {
-        bool bad = (key[M - 1] >> ib) >= sat;
-#pragma unroll
-        for (int s = 0; s < M; ++s) bad |= (s >= M - k) & ((key[s] & imask) == imask);
-        return bad;
-    }
-    __device__ inline bool missing(int k) const {
-        bool bad = false;
-#pragma unroll
-        for (int s = 0; s < M; ++s) bad |= (s >= M - k) & ((key[s] & imask) == imask);
-        return bad;
-    }
-    template <typename DOut>
-    __device__ inline void store(int k, uint64_t *res, DOut *dists, size_t q) const {
-        const size_t o = q * (size_t)k;
-#pragma unroll
-        for (int s = 0; s < M; ++s) {
-            if (s >= M - k) {
-                const size_t j = o + (size_t)(s - (M - k));
-                const bool ok = (key[s] & imask) != imask;
-                res[j] = ok ? (key[s] & imask) : ~0ull;
-                dists[j] = ok ? (DOut)(key[s] >> ib) : missing_dist<DOut>();
-            }
-        }
-    }
-};
-
-template <typename D, int M, bool PACKED> struct TopKSel { using type = TopK<D, M>; };
-template <int M> struct TopKSel<uint64_t, M, true> { using type = TopKPacked<M>; };
 
 // ===========================================================================
 // Packet traversal: a wavefront of 64 queries shares one DFS.  A node is
@@ -568,8 +279,6 @@ template <int M> struct TopKSel<uint64_t, M, true> { using type = TopKPacked<M>;
 // exactly are appended to the re-run list too.  The re-run is LISTED,
 // unseeded and lexicographic, so its results are exact.
 // ===========================================================================
-constexpr int PK_STACK = 128;
-
 struct PacketArgs {
     int nq, k, excl, bucket, ibits, pad;
     const int32_t *list;        // LISTED: query positions
@@ -585,16 +294,17 @@ __global__ __launch_bounds__(64) void packet_knn_kernel(
     uint64_t *__restrict__ res, typename CoordTraits<T>::D *__restrict__ dists) {
     using D = typename CoordTraits<T>::D;
     using H = typename TopKSel<D, M, PACKED>::type;
+    static_assert(M <= 32 || PACKED, "LDS heap tier needs packed integer keys");
     __shared__ int32_t stk[PK_STACK];
+    [[maybe_unused]] uint64_t *heap_mem = nullptr;
+    if constexpr (M > 32) {
+        __shared__ uint64_t heap[M * 64];
+        heap_mem = heap;
+    }
     const int lane = threadIdx.x;
-    const int n = t.n;
-    const int off = n - 1;
-    const T *lx = t.mnx + off;
-    const T *ly = t.mny + off;
-    const T *lz = t.mnz + off;
     const int k = a.k;
     const int nlist = LISTED ? (int)*a.list_count : a.nq;
-    uint64_t nodes = 0, leaves = 0, inserts = 0, overflow = 0;
+    PacketCounters c;
 
     for (int base = blockIdx.x * 64; base < nlist; base += (LISTED ? gridDim.x * 64 : nlist)) {
         const int slot = base + lane;
@@ -605,101 +315,10 @@ __global__ __launch_bounds__(64) void packet_knn_kernel(
         if (active) load_query(t, qs, q, px, py, pz, self);
         if (!a.excl) self = -1;
         H h;
+        if constexpr (M > 32) h.key = heap_mem;
         if constexpr (PACKED) h.init(k, (SEEDED && active) ? (uint64_t)seed[q] : ~0ull, a.ibits);
         else h.init(k, (SEEDED && active) ? seed[q] : dist_max<D>());
-
-        // Scans leaf ranks [first, first+cnt) (cnt <= 64).  One coalesced load
-        // puts leaf first+j in lane j.  Phase 1 broadcasts each leaf and marks,
-        // per lane, the leaves not farther than its current k-th distance;
-        // phase 2 inserts the marked leaves in rounds (each lane fetches its own
-        // next candidate from the holding lane), so the insertion chain runs
-        // once per round instead of once per leaf.
-        auto scan_range = [&](int first, int cnt) {
-            T bx = T(0), by = T(0), bz = T(0);
-            int32_t bo = 0;
-            if (lane < cnt) {
-                bx = lx[first + lane];
-                by = ly[first + lane];
-                bz = lz[first + lane];
-                bo = t.orig_indices ? t.orig[off + first + lane] : first + lane;
-            }
-            const D w = (D)h.worst_d();
-            uint64_t cand = 0;
-            for (int j = 0; j < cnt; ++j) {
-                const D d = leaf_d2<T>(t.m, px, py, pz, rl(bx, j), rl(by, j), rl(bz, j));
-                cand |= (uint64_t)(d <= w) << j;
-            }
-            if (!active) cand = 0;
-            const int sr = self - first;
-            if (sr >= 0 && sr < cnt) cand &= ~(1ull << sr);
-            leaves += (uint64_t)cnt;
-            while (__any(cand != 0)) {
-                const bool has = cand != 0;
-                const int j = has ? (__ffsll((long long)cand) - 1) : 0;
-                cand &= cand - 1;
-                const T x = shfl_any(bx, j), y = shfl_any(by, j), z = shfl_any(bz, j);
-                const int32_t ri = shfl_any(bo, j);
-                const D d = leaf_d2<T>(t.m, px, py, pz, x, y, z);
-                const bool take = has && h.beats_worst(d, ri);
-                h.insert(take, d, ri);
-                inserts += take ? 1 : 0;
-            }
-        };
-
-        int sp = 0;
-        int node = 0;  // an internal node some lane needs (or the single leaf when n == 1)
-        while (true) {
-            ++nodes;
-            int scan_first = 0, scan_cnt = 0, next = -1;
-            if (n == 1) {
-                scan_cnt = 1;
-            } else {
-                const int first = uni(t.rfirst[node]);
-                const int last = uni(t.rlast[node]);
-                if (last - first + 1 <= a.bucket) {
-                    scan_first = first;
-                    scan_cnt = last - first + 1;
-                } else {
-                    const int l = uni(t.left[node]);
-                    const int r = uni(t.right[node]);
-                    const D ld = node_d2(t, px, py, pz, l);
-                    const D rd = node_d2(t, px, py, pz, r);
-                    const D w = (D)h.worst_d();
-                    const bool nl = active && ld <= w;
-                    const bool nr = active && rd <= w;
-                    const bool anyl = __any(nl), anyr = __any(nr);
-                    // With more than `bucket` >= 2 leaves below, at most one child is a leaf.
-                    if (l >= off) {
-                        if (anyl) { scan_first = l - off; scan_cnt = 1; }
-                    } else if (r >= off) {
-                        if (anyr) { scan_first = r - off; scan_cnt = 1; }
-                    }
-                    const bool gl = l < off && anyl;
-                    const bool gr = r < off && anyr;
-                    if (gl && gr) {
-                        // Nearer child first, by vote of the lanes needing either.
-                        const uint64_t needing = __ballot(nl || nr);
-                        const uint64_t lfirst = __ballot((nl || nr) && ld <= rd);
-                        const bool left_first = 2 * __popcll(lfirst) >= __popcll(needing);
-                        next = left_first ? l : r;
-                        if (sp < PK_STACK) stk[sp++] = left_first ? r : l;
-                        else ++overflow;
-                    } else if (gl) {
-                        next = l;
-                    } else if (gr) {
-                        next = r;
-                    }
-                }
-            }
-            if (scan_cnt > 0) scan_range(scan_first, scan_cnt);
-            while (next < 0 && sp > 0) {
-                const int c = uni(stk[--sp]);
-                const D dc = node_d2(t, px, py, pz, c);
-                if (__any(active && dc <= (D)h.worst_d())) next = c;
-            }
-            if (next < 0) break;
-            node = next;
-        }
+        packet_walk<T, H>(t, a.bucket, active, px, py, pz, self, h, stk, c);
         if (active) {
             bool redo = false;
             if constexpr (PACKED) redo = h.needs_exact(k);
@@ -714,11 +333,11 @@ __global__ __launch_bounds__(64) void packet_knn_kernel(
         if (!LISTED) break;
     }
     if (lane == 0 && a.stats) {
-        atomicAdd((unsigned long long *)&a.stats[0], (unsigned long long)nodes);
-        atomicAdd((unsigned long long *)&a.stats[1], (unsigned long long)leaves);
-        if (overflow) atomicAdd((unsigned long long *)&a.stats[2], (unsigned long long)overflow);
+        atomicAdd((unsigned long long *)&a.stats[0], (unsigned long long)c.nodes);
+        atomicAdd((unsigned long long *)&a.stats[1], (unsigned long long)c.leaves);
+        if (c.overflow) atomicAdd((unsigned long long *)&a.stats[2], (unsigned long long)c.overflow);
     }
-    wave_add_stats(a.stats, 4, inserts);
+    wave_add_stats(a.stats, 4, c.inserts);
 }
 
 // ===========================================================================
@@ -858,20 +477,36 @@ static int launch_lane_reg(hipStream_t s, const DTree<T> &dt, const QSrc<T> &qs,
     return FT_OK;
 }
 
-template <typename T, bool REF, bool GROUPED>
-static int launch_lane_lds(hipStream_t s, const DTree<T> &dt, const QSrc<T> &qs, int nq, int k,
-                           int excl, uint64_t *res, typename CoordTraits<T>::D *d,
-                           uint64_t *stats) {
+// Lanes per block for an LDS heap of k entries: as many as fit 64 KiB.
+template <typename T, bool REF> static int lds_lanes(int k, size_t &lds) {
     using D = typename CoordTraits<T>::D;
-    const size_t per_lane = (size_t)k * (sizeof(D) + 4) + STACK_DEPTH * 4;
+    const size_t per_lane =
+        (size_t)k * (sizeof(D) + 4) + (size_t)(REF ? STACK_DEPTH : LANE_STACK_CANON) * 4;
     int lanes = 64;
     const size_t budget = 64 * 1024;
     while (lanes > 1 && (size_t)lanes * per_lane > budget) lanes >>= 1;
-    if ((size_t)lanes * per_lane > budget)
-        return fail(FT_ELDS, "k too large for local memory: reduce k");
-    const size_t lds = (size_t)lanes * per_lane;
-    hipLaunchKernelGGL((lane_knn_lds_kernel<T, REF, GROUPED>), dim3(blocks_for(nq, lanes)),
-                       dim3(LANE_BLOCK), lds, s, dt, qs, nq, k, excl, lanes, res, d, stats);
+    lds = (size_t)lanes * per_lane;
+    return lds <= budget ? lanes : 0;
+}
+
+template <typename T, bool REF, bool GROUPED>
+static int launch_lane_lds(hipStream_t s, const DTree<T> &dt, const QSrc<T> &qs, int nq, int k,
+                           int excl, uint64_t *res, typename CoordTraits<T>::D *d,
+                           uint64_t *stats, const int32_t *list = nullptr,
+                           const uint32_t *list_count = nullptr) {
+    size_t lds = 0;
+    const int lanes = lds_lanes<T, REF>(k, lds);
+    if (!lanes) return fail(FT_ELDS, "k too large for local memory: reduce k");
+    if (list) {
+        const unsigned g = std::min<unsigned>(blocks_for(nq, lanes), 4096u);
+        hipLaunchKernelGGL((lane_knn_lds_kernel<T, REF, GROUPED, true>), dim3(g), dim3(LANE_BLOCK),
+                           lds, s, dt, qs, nq, k, excl, lanes, list, list_count, res, d, stats);
+    } else {
+        hipLaunchKernelGGL((lane_knn_lds_kernel<T, REF, GROUPED, false>),
+                           dim3(blocks_for(nq, lanes)), dim3(LANE_BLOCK), lds, s, dt, qs, nq, k,
+                           excl, lanes, (const int32_t *)nullptr, (const uint32_t *)nullptr, res,
+                           d, stats);
+    }
     FT_CHECK_LAUNCH();
     return FT_OK;
 }
@@ -982,7 +617,13 @@ static int packet_run(hipStream_t s, const DTree<T> &dt, const QSrc<T> &qs, int 
     b.list_count = g_rerun.count + lex_list;
     b.rerun = nullptr;
     b.rerun_count = nullptr;
-    launch_pk<T, M, false, false, true>(s, dim3(waves), dt, qs, b, nullptr, res, d);
+    if constexpr (M > 32) {
+        // exact: per-lane lexicographic LDS heap over the listed queries
+        return launch_lane_lds<T, false, true>(s, dt, qs, nq, k, excl, res, d, nullptr, b.list,
+                                               b.list_count);
+    } else {
+        launch_pk<T, M, false, false, true>(s, dim3(waves), dt, qs, b, nullptr, res, d);
+    }
     FT_CHECK_LAUNCH();
     return FT_OK;
 }
@@ -995,7 +636,13 @@ static int launch_packet(hipStream_t s, const DTree<T> &dt, const QSrc<T> &qs, i
     if (k <= 4) return packet_run<T, 4>(s, dt, qs, nq, k, excl, bucket, seed, res, d, stats);
     if (k <= 8) return packet_run<T, 8>(s, dt, qs, nq, k, excl, bucket, seed, res, d, stats);
     if (k <= 16) return packet_run<T, 16>(s, dt, qs, nq, k, excl, bucket, seed, res, d, stats);
-    return packet_run<T, 32>(s, dt, qs, nq, k, excl, bucket, seed, res, d, stats);
+    if (k <= 32) return packet_run<T, 32>(s, dt, qs, nq, k, excl, bucket, seed, res, d, stats);
+    if constexpr (!CoordTraits<T>::fp) {
+        if (k <= 64) return packet_run<T, 64>(s, dt, qs, nq, k, excl, bucket, seed, res, d, stats);
+        if (k <= 128)
+            return packet_run<T, 128>(s, dt, qs, nq, k, excl, bucket, seed, res, d, stats);
+    }
+    return fail(FT_EINVAL, "packet traversal: k above the heap tiers");
 }
 
 template <typename T>
@@ -1029,9 +676,13 @@ static int knn_dispatch(hipStream_t s, const ft_tree *tree, const QSrc<T> &qs, i
         return launch_lane_lds<T, true, true>(s, dt, qs, nq, k, excl, res, dists, stats);
     }
     // Canonical mode.
-    if (algo == FT_ALGO_AUTO) algo = (k <= 32) ? FT_ALGO_PACKET : FT_ALGO_GROUPED;
-    if (o.seed_bound) algo = FT_ALGO_PACKET;
-    if (algo == FT_ALGO_PACKET && k <= 32) {
+    // Packet tiers: registers for k <= 32; an LDS heap per lane for integer
+    // metrics up to k = 128.  Larger k (or FP metrics above 32) take the
+    // per-lane LDS heap walk.
+    const bool packet_ok = k <= 32 || (!CoordTraits<T>::fp && k <= 128);
+    if (algo == FT_ALGO_AUTO) algo = packet_ok ? FT_ALGO_PACKET : FT_ALGO_GROUPED;
+    if (o.seed_bound && packet_ok) algo = FT_ALGO_PACKET;
+    if (algo == FT_ALGO_PACKET && packet_ok) {
         const int bucket = o.leaf_bucket > 0 ? std::min(o.leaf_bucket, 64) : 32;
         return launch_packet<T>(s, dt, qs, nq, k, excl, bucket, (const D *)o.seed_bound, res,
                                 dists, stats);

@@ -1,0 +1,547 @@
+// ft_traverse.hpp -- device-side pieces shared by the query kernels
+// (ft_query.hip) and the batched multi-tree kernels (ft_batch.hip): lane
+// shuffles, query sources, node distances, the canonical top-k containers
+// (register, packed 64-bit, LDS heap) and the packet traversal walk.
+#pragma once
+
+#include "ft_internal.hpp"
+
+namespace ft {
+
+template <typename D> __device__ inline D dist_max() {
+    if constexpr (std::is_floating_point<D>::value) return DBL_MAX;
+    else return ~0ull;
+}
+template <typename D> __device__ inline D dist_lowest() {
+    if constexpr (std::is_floating_point<D>::value) return -DBL_MAX;
+    else return 0ull;
+}
+
+__device__ inline uint32_t rl32(uint32_t v, int l) { return (uint32_t)__builtin_amdgcn_readlane((int)v, l); }
+__device__ inline uint64_t rl(uint64_t v, int l) {
+    return ((uint64_t)rl32((uint32_t)(v >> 32), l) << 32) | rl32((uint32_t)v, l);
+}
+__device__ inline uint32_t rl(uint32_t v, int l) { return rl32(v, l); }
+__device__ inline int32_t rl(int32_t v, int l) { return (int32_t)rl32((uint32_t)v, l); }
+__device__ inline double rl(double v, int l) { return __longlong_as_double((long long)rl((uint64_t)__double_as_longlong(v), l)); }
+__device__ inline float rl(float v, int l) { return __uint_as_float(rl32(__float_as_uint(v), l)); }
+
+__device__ inline int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// Per-lane source lane (ds_bpermute): lane i receives v from lane src_i.
+__device__ inline uint32_t shfl32(uint32_t v, int src) {
+    return (uint32_t)__builtin_amdgcn_ds_bpermute(src << 2, (int)v);
+}
+__device__ inline uint64_t shfl_any(uint64_t v, int src) {
+    return ((uint64_t)shfl32((uint32_t)(v >> 32), src) << 32) | shfl32((uint32_t)v, src);
+}
+__device__ inline uint32_t shfl_any(uint32_t v, int src) { return shfl32(v, src); }
+__device__ inline int32_t shfl_any(int32_t v, int src) { return (int32_t)shfl32((uint32_t)v, src); }
+__device__ inline double shfl_any(double v, int src) {
+    return __longlong_as_double((long long)shfl_any((uint64_t)__double_as_longlong(v), src));
+}
+__device__ inline float shfl_any(float v, int src) { return __uint_as_float(shfl32(__float_as_uint(v), src)); }
+
+__device__ inline void wave_add_stats(uint64_t *stats, int slot, uint64_t v) {
+    if (!stats) return;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63) == 0 && v) atomicAdd((unsigned long long *)&stats[slot], (unsigned long long)v);
+}
+
+// Query source: external points, or leaves of the tree (self queries).
+template <typename T> struct QSrc {
+    const T *qx, *qy, *qz;   // external points (null for self)
+    const int32_t *leaf_ids; // subset leaf ranks (null = all leaves)
+};
+
+template <typename T>
+__device__ inline void load_query(const DTree<T> &t, const QSrc<T> &qs, int q, T &px, T &py, T &pz,
+                                  int &self_rank) {
+    if (qs.qx) {
+        px = qs.qx[q]; py = qs.qy[q]; pz = qs.qz[q];
+        self_rank = -1;
+    } else {
+        const int r = qs.leaf_ids ? qs.leaf_ids[q] : q;
+        const int leaf = r + t.n - 1;
+        px = t.mnx[leaf]; py = t.mny[leaf]; pz = t.mnz[leaf];
+        self_rank = r;
+    }
+}
+
+template <typename T>
+__device__ inline typename CoordTraits<T>::D node_d2(const DTree<T> &t, T px, T py, T pz, int node) {
+    return box_d2<T>(t.m, px, py, pz, t.mnx[node], t.mxx[node], t.mny[node], t.mxy[node],
+                     t.mnz[node], t.mxz[node]);
+}
+
+// Returned index of a leaf: its rank, or orig_idx with RETURN_ORIG_INDICES.
+template <typename T> __device__ inline int32_t ret_index(const DTree<T> &t, int leaf_node) {
+    return t.orig_indices ? t.orig[leaf_node] : (leaf_node - (t.n - 1));
+}
+
+// ---------------------------------------------------------------------------
+// Canonical register top-k: ascending (d, idx) in M fixed registers, the live
+// k entries right-aligned so the worst is always slot M-1.  Slots below M-k
+// hold (-inf, -1) and never move.  Insertion is a fixed compare/shift network.
+// ---------------------------------------------------------------------------
+template <typename D, int M> struct TopK {
+    D d[M];
+    int32_t i[M];
+    __device__ inline void init(int k, D bound) {
+#pragma unroll
+        for (int s = 0; s < M; ++s) {
+            const bool live = s >= M - k;
+            d[s] = live ? bound : dist_lowest<D>();
+            i[s] = live ? INT32_MAX : -1;
+        }
+    }
+    __device__ inline D worst() const { return d[M - 1]; }
+    __device__ inline bool beats_worst(D cd, int32_t ci) const {
+        return (cd < d[M - 1]) | ((cd == d[M - 1]) & (ci < i[M - 1]));
+    }
+    // Predicated insertion as a compare-and-swap chain from the smallest slot:
+    // the carried entry swaps into every slot it is below, so it lands in order
+    // and the former worst drops off the top.  Every slot stays in its own
+    // registers; the comparisons are branch-free.
+    __device__ inline void insert(bool take, D cd, int32_t ci) {
+#pragma unroll
+        for (int s = 0; s < M; ++s) {
+            const bool lt = take & ((cd < d[s]) | ((cd == d[s]) & (ci < i[s])));
+            const D td = d[s];
+            const int32_t ti = i[s];
+            d[s] = lt ? cd : td;
+            i[s] = lt ? ci : ti;
+            cd = lt ? td : cd;
+            ci = lt ? ti : ci;
+        }
+    }
+    __device__ inline D worst_d() const { return d[M - 1]; }
+    // Number of live slots still holding the initial sentinel.
+    __device__ inline int missing(int k) const {
+        int c = 0;
+#pragma unroll
+        for (int s = 0; s < M; ++s) c += (s >= M - k && i[s] == INT32_MAX) ? 1 : 0;
+        return c;
+    }
+    template <typename DOut>
+    __device__ inline void store(int k, uint64_t *res, DOut *dists, size_t q) const {
+        const size_t o = q * (size_t)k;
+#pragma unroll
+        for (int s = 0; s < M; ++s) {
+            if (s >= M - k) {
+                const size_t j = o + (size_t)(s - (M - k));
+                const bool ok = i[s] != INT32_MAX;
+                res[j] = ok ? (uint64_t)(uint32_t)i[s] : ~0ull;
+                dists[j] = ok ? (DOut)d[s] : missing_dist<DOut>();
+            }
+        }
+    }
+};
+
+constexpr int LANE_BLOCK = 64;
+constexpr int STACK_DEPTH = 64;  // MAX_STACK_DEPTH (src/hlbvh.hpp:984)
+// Canonical per-lane walks: a DFS that pushes both children holds at most one
+// pending sibling per level plus the current pair, and a Karras tree over
+// (64-bit key, 32-bit index) is less than 97 levels deep, so 128 entries never
+// overflow (the reference's 64-entry stack silently drops deeper nodes).
+constexpr int LANE_STACK_CANON = 128;
+
+// ===========================================================================
+// Per-lane DFS with a max-heap row per lane in LDS (LocalMaxHeap,
+// src/maxheap.hpp:340-478).  REF = the reference's mechanics exactly (compare
+// d only, `>=` prune, sift rules, heapsort), so tie picks match; otherwise the
+// heap is keyed by (d, idx) and pruning is strict (canonical).  Layout
+// [slot][lane] so lanes at the same slot hit distinct banks.
+// ===========================================================================
+template <typename D, bool REF> struct LdsHeap {
+    D *dist;       // [cap][lanes]
+    int32_t *idx;  // [cap][lanes]
+    int lanes, lane, count;
+    __device__ inline D &dd(int s) { return dist[s * lanes + lane]; }
+    __device__ inline int32_t &ii(int s) { return idx[s * lanes + lane]; }
+    __device__ inline bool gt(int a, int b) {  // entry a > entry b
+        if (REF) return dd(a) > dd(b);
+        return dd(a) > dd(b) || (dd(a) == dd(b) && ii(a) > ii(b));
+    }
+    __device__ inline void swp(int a, int b) {
+        D td = dd(a); dd(a) = dd(b); dd(b) = td;
+        int32_t ti = ii(a); ii(a) = ii(b); ii(b) = ti;
+    }
+    __device__ inline bool prune(D d, int k) {
+        if (count < k) return false;
+        return REF ? (d >= dd(0)) : (d > dd(0));
+    }
+    __device__ inline void sift_up(int i) {
+        while (i > 0) {
+            const int p = (i - 1) >> 1;
+            if (!gt(i, p)) break;  // reference: dist[i] <= dist[p] -> stop
+            swp(i, p);
+            i = p;
+        }
+    }
+    __device__ inline void sift_down(int i, int size) {
+        while (true) {
+            int lg = i;
+            const int l = 2 * i + 1, r = 2 * i + 2;
+            if (l < size && gt(l, lg)) lg = l;
+            if (r < size && gt(r, lg)) lg = r;
+            if (lg == i) break;
+            swp(i, lg);
+            i = lg;
+        }
+    }
+    __device__ inline bool push(D d, int32_t i, int k) {
+        if (count < k) {
+            dd(count) = d;
+            ii(count) = i;
+            sift_up(count);
+            count++;
+            return true;
+        }
+        const bool better = REF ? (d < dd(0)) : (d < dd(0) || (d == dd(0) && i < ii(0)));
+        if (better) {
+            dd(0) = d;
+            ii(0) = i;
+            sift_down(0, k);
+            return true;
+        }
+        return false;
+    }
+    __device__ inline void sort() {
+        int hs = count;
+        while (hs > 1) {
+            swp(0, hs - 1);
+            hs--;
+            sift_down(0, hs);
+        }
+    }
+};
+
+
+// ---------------------------------------------------------------------------
+// Packed top-k for integer distances: key = min(d2, SAT) << ib | index, one
+// 64-bit compare orders by (d2, index).  ib = bit length of n, so every index
+// is below 2^ib - 1 and an all-ones index marks a sentinel.  A key whose d2
+// part saturated cannot be ordered exactly; the traversal reports such queries
+// (and queries left with sentinels) for an exact re-run.
+// ---------------------------------------------------------------------------
+template <int M> struct TopKPacked {
+    uint64_t key[M];
+    int ib;
+    uint64_t sat, imask;
+    __device__ inline void init(int k, uint64_t bound, int ibits) {
+        ib = ibits;
+        sat = (1ull << (64 - ib)) - 1;
+        imask = (1ull << ib) - 1;
+        const uint64_t sent = ((bound < sat ? bound : sat) << ib) | imask;
+#pragma unroll
+        for (int s = 0; s < M; ++s) key[s] = (s >= M - k) ? sent : 0ull;
+    }
+    __device__ inline uint64_t make(uint64_t d, int32_t i) const {
+        return ((d < sat ? d : sat) << ib) | (uint64_t)(uint32_t)i;
+    }
+    __device__ inline uint64_t worst_d() const { return key[M - 1] >> ib; }
+    __device__ inline bool beats_worst(uint64_t d, int32_t i) const { return make(d, i) < key[M - 1]; }
+    __device__ inline void insert(bool take, uint64_t d, int32_t i) {
+        uint64_t c = make(d, i);
+#pragma unroll
+        for (int s = 0; s < M; ++s) {
+            const bool lt = take & (c < key[s]);
+            const uint64_t t = key[s];
+            key[s] = lt ? c : t;
+            c = lt ? t : c;
+        }
+    }
+    // Sentinels left in live slots, or a saturated k-th distance.
+    __device__ inline bool needs_exact(int k) const {
+        bool bad = (key[M - 1] >> ib) >= sat;
+#pragma unroll
+        for (int s = 0; s < M; ++s) bad |= (s >= M - k) & ((key[s] & imask) == imask);
+        return bad;
+    }
+    __device__ inline bool missing(int k) const {
+        bool bad = false;
+#pragma unroll
+        for (int s = 0; s < M; ++s) bad |= (s >= M - k) & ((key[s] & imask) == imask);
+        return bad;
+    }
+    template <typename DOut>
+    __device__ inline void store(int k, uint64_t *res, DOut *dists, size_t q) const {
+        const size_t o = q * (size_t)k;
+#pragma unroll
+        for (int s = 0; s < M; ++s) {
+            if (s >= M - k) {
+                const size_t j = o + (size_t)(s - (M - k));
+                const bool ok = (key[s] & imask) != imask;
+                res[j] = ok ? (key[s] & imask) : ~0ull;
+                dists[j] = ok ? (DOut)(key[s] >> ib) : missing_dist<DOut>();
+            }
+        }
+    }
+};
+
+
+
+// ===========================================================================
+// Packet walk: the 64 lanes of a wavefront share one DFS over tree t.  A node
+// is expanded when some active lane's box distance does not exceed that
+// lane's current k-th distance (h.worst_d()); a subtree with at most `bucket`
+// leaves is scanned as one contiguous run of leaf slots.  Node data is
+// wave-uniform and the stack (PK_STACK entries) lives in LDS.  Each lane's
+// result depends only on its own pruning tests: a child box lies inside its
+// parent's box and every bound only shrinks, so sharing the walk never drops a
+// node a lane needs.  The depth of a Karras tree over (64-bit key, 32-bit
+// index) is below 96, so the stack (one entry per level) cannot overflow.
+//
+// h must be initialised by the caller (k, bound); self = leaf rank excluded
+// from the lane's results (-1: none); inactive lanes take no candidates.
+// ===========================================================================
+constexpr int PK_STACK = 128;
+
+struct PacketCounters {
+    uint64_t nodes = 0, leaves = 0, inserts = 0, overflow = 0;
+};
+
+template <typename T, typename H>
+__device__ __forceinline__ void packet_walk(const DTree<T> &t, int bucket, bool active, T px, T py,
+                                            T pz, int self, H &h, int32_t *stk,
+                                            PacketCounters &c) {
+    using D = typename CoordTraits<T>::D;
+    const int lane = threadIdx.x & 63;
+    const int n = t.n;
+    const int off = n - 1;
+    const T *lx = t.mnx + off;
+    const T *ly = t.mny + off;
+    const T *lz = t.mnz + off;
+
+    // Scans leaf ranks [first, first+cnt) (cnt <= 64).  One coalesced load
+    // puts leaf first+j in lane j.  Phase 1 broadcasts each leaf and marks,
+    // per lane, the leaves not farther than its current k-th distance;
+    // phase 2 inserts the marked leaves in rounds (each lane fetches its own
+    // next candidate from the holding lane), so the insertion chain runs
+    // once per round instead of once per leaf.
+    auto scan_range = [&](int first, int cnt) {
+        T bx = T(0), by = T(0), bz = T(0);
+        int32_t bo = 0;
+        if (lane < cnt) {
+            bx = lx[first + lane];
+            by = ly[first + lane];
+            bz = lz[first + lane];
+            bo = t.orig_indices ? t.orig[off + first + lane] : first + lane;
+        }
+        const D w = (D)h.worst_d();
+        uint64_t cand = 0;
+        for (int j = 0; j < cnt; ++j) {
+            const D d = leaf_d2<T>(t.m, px, py, pz, rl(bx, j), rl(by, j), rl(bz, j));
+            cand |= (uint64_t)(d <= w) << j;
+        }
+        if (!active) cand = 0;
+        const int sr = self - first;
+        if (sr >= 0 && sr < cnt) cand &= ~(1ull << sr);
+        c.leaves += (uint64_t)cnt;
+        while (__any(cand != 0)) {
+            const bool has = cand != 0;
+            const int j = has ? (__ffsll((long long)cand) - 1) : 0;
+            cand &= cand - 1;
+            const T x = shfl_any(bx, j), y = shfl_any(by, j), z = shfl_any(bz, j);
+            const int32_t ri = shfl_any(bo, j);
+            const D d = leaf_d2<T>(t.m, px, py, pz, x, y, z);
+            const bool take = has && h.beats_worst(d, ri);
+            h.insert(take, d, ri);
+            c.inserts += take ? 1 : 0;
+        }
+    };
+
+    int sp = 0;
+    int node = 0;  // an internal node some lane needs (or the single leaf when n == 1)
+    while (true) {
+        ++c.nodes;
+        int scan_first = 0, scan_cnt = 0, next = -1;
+        if (n == 1) {
+            scan_cnt = 1;
+        } else {
+            const int first = uni(t.rfirst[node]);
+            const int last = uni(t.rlast[node]);
+            if (last - first + 1 <= bucket) {
+                scan_first = first;
+                scan_cnt = last - first + 1;
+            } else {
+                const int l = uni(t.left[node]);
+                const int r = uni(t.right[node]);
+                const D ld = node_d2(t, px, py, pz, l);
+                const D rd = node_d2(t, px, py, pz, r);
+                const D w = (D)h.worst_d();
+                const bool nl = active && ld <= w;
+                const bool nr = active && rd <= w;
+                const bool anyl = __any(nl), anyr = __any(nr);
+                // With more than `bucket` >= 2 leaves below, at most one child is a leaf.
+                if (l >= off) {
+                    if (anyl) { scan_first = l - off; scan_cnt = 1; }
+                } else if (r >= off) {
+                    if (anyr) { scan_first = r - off; scan_cnt = 1; }
+                }
+                const bool gl = l < off && anyl;
+                const bool gr = r < off && anyr;
+                if (gl && gr) {
+                    // Nearer child first, by vote of the lanes needing either.
+                    const uint64_t needing = __ballot(nl || nr);
+                    const uint64_t lfirst = __ballot((nl || nr) && ld <= rd);
+                    const bool left_first = 2 * __popcll(lfirst) >= __popcll(needing);
+                    next = left_first ? l : r;
+                    if (sp < PK_STACK) stk[sp++] = left_first ? r : l;
+                    else ++c.overflow;
+                } else if (gl) {
+                    next = l;
+                } else if (gr) {
+                    next = r;
+                }
+            }
+        }
+        if (scan_cnt > 0) scan_range(scan_first, scan_cnt);
+        while (next < 0 && sp > 0) {
+            const int cn = uni(stk[--sp]);
+            const D dc = node_d2(t, px, py, pz, cn);
+            if (__any(active && dc <= (D)h.worst_d())) next = cn;
+        }
+        if (next < 0) break;
+        node = next;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Packed (d2, index) keys in a max-heap per lane in LDS, layout [slot][64
+// lanes] so a wave's accesses to one slot hit 64 distinct banks: the top-k
+// container for k above the register tiers (k <= KCAP).  Same key form and
+// re-run rule as TopKPacked; the root (the current k-th key) is mirrored in a
+// register so bound tests never touch LDS.  Insertion replaces the root and
+// sifts down (log2 k levels); store() heapsorts in place.
+// ---------------------------------------------------------------------------
+template <int KCAP> struct TopKHeapLds {
+    uint64_t *key;  // [KCAP][64], owned by the kernel
+    uint64_t root, sat, imask;
+    int ib, k, lane;
+    __device__ inline uint64_t &K(int s) { return key[s * 64 + lane]; }
+    __device__ inline void init(int k_, uint64_t bound, int ibits) {
+        k = k_;
+        ib = ibits;
+        lane = threadIdx.x & 63;
+        sat = (1ull << (64 - ib)) - 1;
+        imask = (1ull << ib) - 1;
+        root = ((bound < sat ? bound : sat) << ib) | imask;
+        for (int s = 0; s < k; ++s) K(s) = root;
+    }
+    __device__ inline uint64_t make(uint64_t d, int32_t i) const {
+        return ((d < sat ? d : sat) << ib) | (uint64_t)(uint32_t)i;
+    }
+    __device__ inline uint64_t worst_d() const { return root >> ib; }
+    __device__ inline bool beats_worst(uint64_t d, int32_t i) const { return make(d, i) < root; }
+    // Sift c down from the root of a heap of `size` entries.
+    __device__ inline void sift_root(uint64_t c, int size) {
+        int pos = 0;
+        while (true) {
+            const int l = 2 * pos + 1;
+            if (l >= size) break;
+            uint64_t kb = K(l);
+            int b = l;
+            if (l + 1 < size) {
+                const uint64_t kr = K(l + 1);
+                if (kr > kb) {
+                    kb = kr;
+                    b = l + 1;
+                }
+            }
+            if (kb <= c) break;
+            K(pos) = kb;
+            pos = b;
+        }
+        K(pos) = c;
+    }
+    __device__ inline void insert(bool take, uint64_t d, int32_t i) {
+        if (take) {
+            sift_root(make(d, i), k);
+            root = K(0);
+        }
+    }
+    __device__ inline bool needs_exact(int) const {
+        return ((root >> ib) >= sat) | ((root & imask) == imask);
+    }
+    __device__ inline bool missing(int) const { return (root & imask) == imask; }
+    template <typename DOut>
+    __device__ inline void store(int, uint64_t *res, DOut *dists, size_t q) {
+        for (int hs = k; hs > 1; --hs) {
+            const uint64_t top = K(0);
+            sift_root(K(hs - 1), hs - 1);
+            K(hs - 1) = top;
+        }
+        const size_t o = q * (size_t)k;
+        for (int s = 0; s < k; ++s) {
+            const uint64_t v = K(s);
+            const bool ok = (v & imask) != imask;
+            res[o + s] = ok ? (v & imask) : ~0ull;
+            dists[o + s] = ok ? (DOut)(v >> ib) : missing_dist<DOut>();
+        }
+    }
+};
+
+// Top-k container of a packet traversal: registers for M <= 32 (canonical
+// (d, idx) pairs, or packed keys for integer metrics), an LDS heap above that
+// (integer metrics only).
+template <typename D, int M, bool PACKED> struct TopKSel { using type = TopK<D, M>; };
+template <int M> struct TopKSel<uint64_t, M, true> {
+    using type = typename std::conditional<(M > 32), TopKHeapLds<M>, TopKPacked<M>>::type;
+};
+
+// ---------------------------------------------------------------------------
+// Per-lane DFS with an LDS heap row per lane (LdsHeap).  REF: the reference's
+// walk exactly -- 64-entry stack, children pushed while sp < 62 (deeper ones
+// dropped and counted), `>=` prune.  Otherwise canonical: strict prune, a
+// 128-entry stack that cannot overflow.  GROUPED: the wave skips a node unless
+// some lane still needs it (results do not depend on it).  stk: [depth][lanes].
+// ---------------------------------------------------------------------------
+template <typename T, bool REF, bool GROUPED>
+__device__ __forceinline__ void lane_heap_walk(const DTree<T> &t, int k, bool active, T px, T py,
+                                               T pz, int self,
+                                               LdsHeap<typename CoordTraits<T>::D, REF> &h,
+                                               int32_t *stk, int lanes, int lane,
+                                               uint64_t &visits, uint64_t &drops,
+                                               uint64_t &inserts) {
+    using D = typename CoordTraits<T>::D;
+    constexpr int DEPTH = REF ? STACK_DEPTH : LANE_STACK_CANON;
+    const int n = t.n;
+    int sp = 0;
+    if (active) stk[(sp++) * lanes + lane] = 0;
+    while (sp > 0) {
+        const int node = stk[(--sp) * lanes + lane];
+        const D d2 = node_d2(t, px, py, pz, node);
+        ++visits;
+        const bool pruned = h.prune(d2, k);
+        if (GROUPED) {
+            if (!__any(!pruned)) continue;
+        }
+        if (pruned) continue;
+        if (node >= n - 1 || n == 1) {
+            const int leaf = (n == 1) ? 0 : node;
+            const int rank = leaf - (n - 1);
+            if (rank == self) continue;
+            if (h.push(d2, ret_index(t, leaf), k)) ++inserts;
+        } else {
+            const int l = t.left[node], r = t.right[node];
+            const D ld = node_d2(t, px, py, pz, l);
+            const D rd = node_d2(t, px, py, pz, r);
+            if (sp < DEPTH - 2) {
+                if (ld <= rd) {
+                    stk[(sp++) * lanes + lane] = r;
+                    stk[(sp++) * lanes + lane] = l;
+                } else {
+                    stk[(sp++) * lanes + lane] = l;
+                    stk[(sp++) * lanes + lane] = r;
+                }
+            } else {
+                ++drops;
+            }
+        }
+    }
+}
+
+}  // namespace ft

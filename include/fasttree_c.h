@@ -232,6 +232,49 @@ int ft_mlp_last_loss(ft_stream s, ft_mlp *model, float *loss);
 int ft_mlp_seed(ft_stream s, ft_mlp *model, const ft_tree *tree, const int32_t *leaf_ids,
                 int nq, int k, float *scratch_features, void *seed);
 
+/* ---- batched trees and mixed-k queries in one graph (no reference counterpart) ---- */
+/* Many trees of uneven size, and query requests of mixed k over them, run as one    */
+/* step: every tree is built (build_bvh), the queries are ordered along each tree's  */
+/* curve, and persistent kernels per heap tier drain a work list over all trees.     */
+/* The step can be captured in a hipGraph and replayed with one launch.  Its closest */
+/* reference analogue is the compile-time group-width / heap-tier dispatch          */
+/* (src/hlbvh.hpp:1433-1453, 1841-1927).  Results equal knn_query's canonical        */
+/* results on each tree.  Pointers given to a batch must stay valid while it is used.*/
+typedef struct ft_batch ft_batch;
+typedef struct ft_batch_request {
+    int32_t tree;         /* index of the tree the queries search                    */
+    int32_t k;            /* neighbours per query, 1..256                            */
+    int64_t query_first;  /* first query point (index into qx/qy/qz)                 */
+    int64_t num_queries;
+    int64_t result_first; /* first result slot (element index into results/dists)    */
+} ft_batch_request;
+int ft_batch_create(ft_batch **batch, int32_t coord_type);
+int ft_batch_destroy(ft_batch *batch);
+/* trees: allocated trees of the batch's coordinate type.  x/y/z/n: per-tree build    */
+/* inputs (device pointers); x = NULL takes the trees as built (queries only).        */
+int ft_batch_set_trees(ft_batch *batch, int num_trees, ft_tree *const *trees,
+                       const void *const *x, const void *const *y, const void *const *z,
+                       const size_t *n);
+/* Query points (device, coord_t) and outputs (device): results uint64_t, dists      */
+/* dist_t, row-major per query at each request's result_first.                        */
+int ft_batch_set_queries(ft_batch *batch, int num_requests, const ft_batch_request *reqs,
+                         const void *qx, const void *qy, const void *qz, uint64_t *results,
+                         void *dists);
+/* Optional device uint64_t[8] counters as in ft_query_opts.stats.                    */
+int ft_batch_set_stats(ft_batch *batch, uint64_t *stats);
+/* Autotuner: runs the step's parts on the bound data and keeps, per (tree-size      */
+/* class, k tier), the fastest tier kernel and leaf bucket, and the fastest build    */
+/* branch count.  Synchronises the stream.                                            */
+int ft_batch_tune(ft_batch *batch, ft_stream s);
+/* One step, launched kernel by kernel (no graph).                                    */
+int ft_batch_run(ft_batch *batch, ft_stream s);
+/* Records the step into a hipGraph (allocates nothing while recording).             */
+int ft_batch_capture(ft_batch *batch, ft_stream s);
+/* Replays the recorded step: one graph launch (captures first if needed).           */
+int ft_batch_launch(ft_batch *batch, ft_stream s);
+/* The settings in use, as JSON text.                                                  */
+int ft_batch_tuning(ft_batch *batch, char *buf, size_t len);
+
 #ifdef __cplusplus
 }
 #endif

@@ -103,7 +103,8 @@ class Tree:
         s = _Tree()
         for name in ("min_x", "max_x", "min_y", "max_y", "min_z", "max_z", "left", "right",
                      "parent", "id", "ghost", "orig"):
-            setattr(s, name, a[name].ctypes.data)
+            # a metric-only tree (no arrays) serves point_d2
+            setattr(s, name, a[name].ctypes.data if name in a else None)
         s.n, s.ct, s.sfc, s.bits = self.n, self.ct, self.sfc, self.bits
         s.periodic, s.orig_indices, s.box = self.periodic, self.orig_indices, self.box
         return s

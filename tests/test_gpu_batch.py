@@ -1,0 +1,163 @@
+"""GPU checks of the batched multi-tree path (ft_batch_*): every request's
+results equal knn_query's canonical results on its own tree (which the kNN
+tests pin to the oracle's brute force), a sample is checked against the oracle
+directly, and graph replays -- after retuning, with other GPU work between
+them, and after the query points change -- keep giving those results."""
+import numpy as np
+import pytest
+
+import oracle as O
+from fasttree_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def make_batch(ft, ct, sizes, ks, seed, periodic=False):
+    from fasttree_amd.batch import Batch, plan_requests
+    rng = np.random.default_rng(seed)
+    dt = O.COORD_DTYPES[ct]
+    fp = ct in (O.CT_F32, O.CT_F64)
+    cfg = ft.Config(sfc=ft.SFC_MORTON, bits_per_dim=21, periodic=periodic,
+                    box_length=100.0 if (fp and periodic) else 0.0)
+    trees, parts, host = [], [], []
+    for i, n in enumerate(sizes):
+        if n >= 64:
+            x, y, z = synth.nfw_clustered(n, seed * 100 + i, box=100.0, n_halos=max(2, n // 2000))
+        else:
+            x, y, z = synth.uniform(n, seed * 100 + i, 100.0)
+        if fp:
+            pts = tuple(a.astype(dt) for a in (x, y, z))
+        else:
+            pts = tuple(synth.to_fixed(a, 100.0, 21, dt) for a in (x, y, z))
+        host.append(pts)
+        trees.append(ft.TreeSoA(max(n, 1), ct, cfg))
+        parts.append(tuple(ft.DeviceArray.from_host(a) for a in pts))
+    # two requests per tree (one may be empty), k from the mixed set
+    nq, kk, tt = [], [], []
+    for i in range(len(sizes)):
+        for _ in range(2):
+            nq.append(int(rng.choice([0, 1, 63, 64, 65, 700, 2500])))
+            kk.append(int(rng.choice(ks)))
+            tt.append(i)
+    reqs, total_q, total_r = plan_requests(nq, kk, tt)
+    # queries: half near tree points, half uniform in the box (random order)
+    qs = []
+    for (t, k, qf, n, rf) in reqs:
+        src = host[t]
+        m = len(src[0])
+        pick = rng.integers(0, m, n)
+        near = [src[a][pick].astype(np.float64) + rng.normal(0, 0.3 if fp else 2000.0, n)
+                for a in range(3)]
+        far = [rng.uniform(0, 100.0 if fp else float(1 << 21), n) for _ in range(3)]
+        sel = rng.random(n) < 0.5
+        q = [np.where(sel, near[a], far[a]) for a in range(3)]
+        if fp:
+            q = [a.astype(dt) for a in q]
+        else:
+            q = [np.clip(a, 0, (1 << 21) - 1).astype(dt) for a in q]
+        qs.append(q)
+    qx = np.concatenate([q[0] for q in qs]) if qs else np.zeros(0, dt)
+    qy = np.concatenate([q[1] for q in qs]) if qs else np.zeros(0, dt)
+    qz = np.concatenate([q[2] for q in qs]) if qs else np.zeros(0, dt)
+    dq = [ft.DeviceArray.from_host(a) for a in (qx, qy, qz)]
+    res = ft.DeviceArray(max(total_r, 1), np.uint64)
+    dst = ft.DeviceArray(max(total_r, 1), ft.dist_dtype(ct))
+    b = Batch(ct)
+    b.set_trees(trees, parts)
+    b.set_queries(reqs, *dq, res, dst)
+    return b, trees, reqs, (qx, qy, qz), dq, res, dst, host
+
+
+def expected(ft, trees, reqs, q):
+    out = []
+    for (t, k, qf, n, rf) in reqs:
+        if n == 0:
+            out.append(None)
+            continue
+        sl = slice(qf, qf + n)
+        r, d = ft.knn_query(trees[t], q[0][sl], q[1][sl], q[2][sl], k)
+        out.append((r, d))
+    return out
+
+
+def check(reqs, exp, res, dst, tag):
+    r_all = res.to_host()
+    d_all = dst.to_host()
+    for (t, k, qf, n, rf), e in zip(reqs, exp):
+        if e is None:
+            continue
+        r = r_all[rf:rf + n * k].reshape(n, k)
+        d = d_all[rf:rf + n * k].reshape(n, k)
+        assert np.array_equal(d, e[1]), f"{tag}: d2 tree={t} k={k} n={n}"
+        assert np.array_equal(r, e[0]), f"{tag}: idx tree={t} k={k} n={n}"
+
+
+@pytest.mark.parametrize("ct", [O.CT_U32, O.CT_F64])
+def test_batch_run_tune_and_graph_replays(ft, orc, ct):
+    sizes = [3000, 17000, 1, 2, 45000, 9000, 130]
+    b, trees, reqs, q, dq, res, dst, host = make_batch(ft, ct, sizes, (1, 8, 32, 128), 3)
+    print(f"[batch] ct={ct} eager run", flush=True)
+    b.run()
+    exp = expected(ft, trees, reqs, q)
+    check(reqs, exp, res, dst, "eager")
+
+    # oracle brute force on a sample of each non-empty request
+    for (t, k, qf, n, rf), e in zip(reqs, exp):
+        if e is None:
+            continue
+        ot = O.build_bvh(ct, *host[t], sfc=O.SFC_MORTON, bits=21)
+        m = min(n, 40)
+        sl = slice(qf, qf + m)
+        rb, db = O.knn_bruteforce(ot, k, queries=(q[0][sl], q[1][sl], q[2][sl]))
+        assert np.array_equal(e[1][:m], db) and np.array_equal(e[0][:m], rb), (t, k)
+
+    print("[batch] tune", flush=True)
+    b.tune()
+    tun = b.tuning()
+    assert tun["tuned"] and tun["groups"]
+    print(f"[batch] tuning {tun}", flush=True)
+    for rep in range(3):
+        res.zero()
+        dst.zero()
+        print(f"[batch] graph replay {rep}", flush=True)
+        b.launch()
+        # unrelated eager GPU work between replays
+        ft.self_knn_query(trees[0], 4)
+        check(reqs, exp, res, dst, f"replay {rep}")
+
+    # new query coordinates, same layout: the graph reads the new points
+    rng = np.random.default_rng(99)
+    perm = rng.permutation(len(q[0]))
+    q2 = tuple(np.ascontiguousarray(a[perm]) for a in q)
+    for dev, host_new in zip(dq, q2):
+        dev.copy_from(host_new)
+    b.launch()
+    check(reqs, expected(ft, trees, reqs, q2), res, dst, "moved queries")
+
+
+def test_batch_queries_only_on_prebuilt_trees(ft, orc):
+    """Trees built beforehand (no build inputs): the step only queries."""
+    from fasttree_amd.batch import Batch, plan_requests
+    ct = O.CT_U64
+    cfg = ft.Config(sfc=ft.SFC_PEANO_HILBERT, bits_per_dim=21, periodic=True)
+    trees, hosts = [], []
+    for i, n in enumerate((5000, 20000)):
+        pts = synth.clustered_fixed(n, 40 + i, n_halos=20)
+        t = ft.TreeSoA(n, ct, cfg)
+        ft.build_bvh(ft.particles(*pts), t)
+        trees.append(t)
+        hosts.append(pts)
+    reqs, total_q, total_r = plan_requests([300, 500, 200], [16, 64, 1], [0, 1, 1])
+    rng = np.random.default_rng(5)
+    q = tuple(rng.integers(0, 1 << 21, total_q, dtype=np.uint64) for _ in range(3))
+    dq = [ft.DeviceArray.from_host(a) for a in q]
+    res = ft.DeviceArray(total_r, np.uint64)
+    dst = ft.DeviceArray(total_r, np.uint64)
+    stats = ft.DeviceArray(8, np.uint64).zero()
+    b = Batch(ct)
+    b.set_trees(trees)
+    b.set_queries(reqs, *dq, res, dst)
+    b.set_stats(stats)
+    b.launch()
+    check(reqs, expected(ft, trees, reqs, q), res, dst, "prebuilt")
+    assert stats.to_host()[0] > 0

@@ -53,7 +53,9 @@ DATASETS = [
 def test_self_knn_canonical(ft, orc, ds, algo):
     name, ct, sfc, bits, periodic, gen = ds
     tree, t = make(ft, ct, sfc, bits, periodic, gen())
-    ks = (1, 5, 16, 32) if algo == "packet" else (1, 16, 32, 33, 64)
+    # packet: register tiers (k <= 32) and, for integer metrics, the LDS heap
+    # tier (32 < k <= 128); FP metrics above 32 take the per-lane heap walk.
+    ks = (1, 5, 16, 32, 33, 64, 128) if algo == "packet" else (1, 16, 32, 33, 64)
     for k in ks:
         for excl in (True, False):
             print(f"[knn] {name} {algo} k={k} excl={excl}", flush=True)
@@ -145,7 +147,7 @@ def test_edge_sizes_and_sentinels(ft, orc, n):
         tree, t = make(ft, ct, O.SFC_PH, 21, 0, (x, y, z))
         for k in (1, 2, 4, 32, 33, 64):
             for excl in (True, False):
-                for algo in (("packet", "grouped") if k <= 32 else ("grouped",)):
+                for algo in ("packet", "grouped"):
                     r, d = ft.self_knn_query(tree, k, exclude_self=excl, max_k=256,
                                              algorithm=ALGOS[algo])
                     rb, db = O.knn_bruteforce(t, k, excl)
@@ -173,14 +175,15 @@ def test_seeded_bound_never_changes_results(ft, orc):
     gives exactly the unbounded results; too-small seeds are re-run."""
     pts = nfw_points(40000, 17, O.CT_U64)
     tree, t = make(ft, O.CT_U64, O.SFC_PH, 21, 1, pts)
-    k = 16
-    rb, db = O.knn_bruteforce(t, k, True)
-    true_r2 = db[:, -1].astype(np.float64)
-    rng = np.random.default_rng(2)
-    scale = rng.choice([0.25, 0.9, 1.0, 1.5, 4.0], size=len(true_r2))
-    seed = np.floor(true_r2 * scale).astype(np.uint64)
-    stats = ft.DeviceArray(8, np.uint64).zero()
-    r, d = ft.self_knn_query(tree, k, seed_bound=ft.DeviceArray.from_host(seed), stats=stats)
-    assert np.array_equal(d, db) and np.array_equal(r, rb)
-    st = stats.to_host()
-    assert st[3] == int(np.sum(seed < db[:, -1]))  # exactly the too-small seeds re-ran
+    for k in (16, 64):  # register tier, LDS heap tier
+        rb, db = O.knn_bruteforce(t, k, True)
+        true_r2 = db[:, -1].astype(np.float64)
+        rng = np.random.default_rng(2)
+        scale = rng.choice([0.25, 0.9, 1.0, 1.5, 4.0], size=len(true_r2))
+        seed = np.floor(true_r2 * scale).astype(np.uint64)
+        stats = ft.DeviceArray(8, np.uint64).zero()
+        r, d = ft.self_knn_query(tree, k, seed_bound=ft.DeviceArray.from_host(seed), stats=stats,
+                                 max_k=256)
+        assert np.array_equal(d, db) and np.array_equal(r, rb), k
+        st = stats.to_host()
+        assert st[3] == int(np.sum(seed < db[:, -1]))  # exactly the too-small seeds re-ran
