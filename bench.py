@@ -42,12 +42,19 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--bucket", type=int, default=0, help="packet leaf bucket (0 = auto)")
     ap.add_argument("--sweep-buckets", action="store_true")
+    ap.add_argument("--topk", type=int, default=0,
+                    help="packet top-k container: 0 auto, 1 registers, 2 LDS heap")
     ap.add_argument("--model", default=os.path.join(ROOT, "fasttree_amd", "models",
                                                     "nfw10m_int64_ph21_k32.npz"),
                     help="trained learned-radius model (npz); the learned step is skipped "
                          "when it is missing")
     ap.add_argument("--seed-probe", action="store_true",
                     help="time the packet kernel seeded with scaled exact k-th distances")
+    ap.add_argument("--pmc-probe", action="store_true",
+                    help="one build, one self-kNN and one 1 GiB copy kernel, then exit (for "
+                         "hardware counter runs); prints no JSON")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="skip the batched multi-tree workload (64 trees, 4M queries)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -77,10 +84,20 @@ def main():
         ft.build_bvh(p, tree, scratch=scratch, stream=stream.handle)
         ev[1].record(stream)
         ft.self_knn_query(tree, k, results=res, dists=dst, exclude_self=True,
-                          stream=stream.handle, algorithm=algorithm, leaf_bucket=args.bucket)
+                          stream=stream.handle, algorithm=algorithm, leaf_bucket=args.bucket,
+                          topk=args.topk)
         ev[2].record(stream)
         stream.synchronize()
         return ev[0].elapsed_ms(ev[1]), ev[1].elapsed_ms(ev[2])
+
+    if args.pmc_probe:
+        b, q = step()
+        cb = 1 << 30
+        src, dstc = ft.DeviceArray(cb // 8, np.uint64), ft.DeviceArray(cb // 8, np.uint64)
+        ft._check(ft.lib().ft_copy_kernel(dstc.ptr, src.ptr, cb, stream.handle))
+        stream.synchronize()
+        log(f"pmc probe: build {b:.3f} ms, self-kNN {q:.3f} ms, copy kernel {cb} bytes")
+        return
 
     for w in range(args.warmup):
         b, q = step()
@@ -132,7 +149,8 @@ def main():
             model.seed(tree, k, seeds, feats, stream=stream.handle)
             ev[1].record(stream)
             ft.self_knn_query(tree, k, results=res2, dists=dst2, stream=stream.handle,
-                              seed_bound=seeds, stats=lst, leaf_bucket=args.bucket)
+                              seed_bound=seeds, stats=lst, leaf_bucket=args.bucket,
+                              topk=args.topk)
             ev[2].record(stream)
             stream.synchronize()
             if rep:
@@ -154,13 +172,14 @@ def main():
             f"identical sample {same}")
         del res2, dst2, feats, seeds
     if args.sweep_buckets:
-        for bk in (8, 16, 24, 32, 48, 64):
-            ev[0].record(stream)
-            ft.self_knn_query(tree, k, results=res, dists=dst, stream=stream.handle,
-                              algorithm=ft.ALGO_PACKET, leaf_bucket=bk)
-            ev[1].record(stream)
-            stream.synchronize()
-            log(f"bucket {bk}: {ev[0].elapsed_ms(ev[1]):.2f} ms")
+        for tk in (ft.TOPK_REGISTERS, ft.TOPK_LDS_HEAP):
+            for bk in (8, 16, 24, 32, 48, 64):
+                ev[0].record(stream)
+                ft.self_knn_query(tree, k, results=res, dists=dst, stream=stream.handle,
+                                  algorithm=ft.ALGO_PACKET, leaf_bucket=bk, topk=tk)
+                ev[1].record(stream)
+                stream.synchronize()
+                log(f"topk {tk} bucket {bk}: {ev[0].elapsed_ms(ev[1]):.2f} ms")
     if args.seed_probe:
         dk = dst.to_host().reshape(n, k)[:, k - 1].astype(np.float64)
         for f in (1.0, 1.2, 1.5, 2.0, 0.9):
@@ -201,6 +220,11 @@ def main():
     if not args.no_cpu:
         cpu = cpu_baseline(ft, tree, n, k, args.cpu_seconds)
 
+    batched = None
+    if not args.no_batched:
+        del res, dst, tree, scratch, p
+        batched = batched_bench(ft, args.steps, args.warmup, args.seed)
+
     out = {
         "metric": "self-kNN queries/s, N=10M k=32, learned-bound vs plain; BVH build ms",
         "value": round(ladder.get("learned_qps", qps), 1),
@@ -229,8 +253,102 @@ def main():
                       "inserts_per_query": float(st[4]) / n, "stack_overflows": int(st[2])},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "batched": batched,
     }
     print(json.dumps(out), flush=True)
+
+
+def batched_bench(ft, steps, warmup, seed):
+    """64 trees (N log-uniform in [10k, 2M], clustered, int32 21-bit Morton,
+    non-periodic), 4M external queries in 256 requests with k in {1, 8, 32,
+    128}: one step = build every tree + answer every request, recorded in one
+    hipGraph after autotuning.  Beside it: the same step launched eagerly with
+    the untuned defaults, and a plain loop of one build_bvh per tree and one
+    knn_query per request on one stream."""
+    import ctypes as C
+    from fasttree_amd import synth
+    from fasttree_amd.batch import Batch
+
+    t0 = time.time()
+    sizes, pts, reqs, q = synth.batched_workload(seed)
+    log(f"batched workload: {len(sizes)} trees, {int(sizes.sum())} points, {len(q[0])} queries "
+        f"in {len(reqs)} requests, generated in {time.time() - t0:.1f}s")
+    cfg = ft.Config(sfc=ft.SFC_MORTON, bits_per_dim=21, periodic=False)
+    trees = [ft.TreeSoA(int(m), ft.COORD_U32, cfg) for m in sizes]
+    parts = [tuple(ft.DeviceArray.from_host(a) for a in pp) for pp in pts]
+    dq = [ft.DeviceArray.from_host(a) for a in q]
+    total_r = sum(r[1] * r[3] for r in reqs)
+    res = ft.DeviceArray(total_r, np.uint64)
+    dst = ft.DeviceArray(total_r, np.uint64)
+    stream = ft.Stream()
+    e0, e1 = ft.Event(), ft.Event()
+
+    def timed(fn):
+        e0.record(stream)
+        fn()
+        e1.record(stream)
+        stream.synchronize()
+        return e0.elapsed_ms(e1)
+
+    b = Batch(ft.COORD_U32)
+    b.set_trees(trees, parts)
+    b.set_queries(reqs, *dq, res, dst)
+    b.run(stream)
+    stream.synchronize()
+    eager_ms = min(timed(lambda: b.run(stream)) for _ in range(2))
+    t0 = time.time()
+    b.tune(stream)
+    tune_s = time.time() - t0
+    b.capture(stream)
+    for _ in range(max(warmup, 1)):
+        b.launch(stream)
+    stream.synchronize()
+    graph = [timed(lambda: b.launch(stream)) for _ in range(max(steps, 1))]
+    graph_ms = float(np.median(graph))
+    log(f"batched: graph step {graph_ms:.2f} ms (eager untuned {eager_ms:.2f} ms, "
+        f"tuning {tune_s:.1f}s)")
+
+    # plain loop: one build_bvh per tree, one knn_query per request
+    L = ft.lib()
+    res2 = ft.DeviceArray(total_r, np.uint64)
+    dst2 = ft.DeviceArray(total_r, np.uint64)
+    scratch = ft.BvhScratch()
+
+    def loop():
+        for t, tr in enumerate(trees):
+            ft.build_bvh(ft.particles(*parts[t]), tr, scratch=scratch, stream=stream.handle)
+        for (t, kk, qf, m, rf) in reqs:
+            if m == 0:
+                continue
+            o = ft.QueryOpts(256, 0, 0, 0, 0, 0, None, None)
+            ft._check(L.ft_knn_query(stream.handle, C.byref(trees[t].c), dq[0].ptr + 4 * qf,
+                                     dq[1].ptr + 4 * qf, dq[2].ptr + 4 * qf, int(kk), int(m),
+                                     res2.ptr + 8 * rf, dst2.ptr + 8 * rf, C.byref(o)))
+
+    loop()
+    stream.synchronize()
+    loop_ms = min(timed(loop) for _ in range(2))
+    same = True
+    rng = np.random.default_rng(seed)
+    for i in rng.choice(len(reqs), 24, replace=False):
+        t, kk, qf, m, rf = reqs[i]
+        if m == 0:
+            continue
+        same &= bool(np.array_equal(res.to_host_range(rf, m * kk), res2.to_host_range(rf, m * kk)))
+        same &= bool(np.array_equal(dst.to_host_range(rf, m * kk), dst2.to_host_range(rf, m * kk)))
+    log(f"batched: per-tree loop {loop_ms:.2f} ms, identical sample {same}")
+    nq = int(len(q[0]))
+    return {"workload": "64 clustered trees, N log-uniform in [10k, 2M], int32 21-bit Morton, "
+                        "non-periodic; 4M external queries in 256 requests, k in {1,8,32,128}; "
+                        "step = build every tree + answer every request",
+            "trees": len(sizes), "points": int(sizes.sum()), "queries": nq,
+            "requests": len(reqs), "graph_step_ms": round(graph_ms, 3),
+            "queries_per_s_incl_builds": round(nq / (graph_ms * 1e-3), 1),
+            "eager_untuned_step_ms": round(eager_ms, 3),
+            "per_tree_loop_step_ms": round(loop_ms, 3),
+            "speedup_vs_loop": round(loop_ms / graph_ms, 3),
+            "identical_to_loop_on_24_sampled_requests": same,
+            "tuning_s": round(tune_s, 2), "tuning": b.tuning()}
 
 
 def cpu_baseline(ft, tree, n, k, seconds):

@@ -25,6 +25,7 @@ COORD_F32, COORD_F64, COORD_U32, COORD_U64 = 0, 1, 2, 3
 SFC_MORTON, SFC_PEANO_HILBERT = 0, 1
 MODE_CANONICAL, MODE_REFERENCE = 0, 1
 ALGO_AUTO, ALGO_PLAIN, ALGO_GROUPED, ALGO_PACKET = 0, 1, 2, 3
+TOPK_AUTO, TOPK_REGISTERS, TOPK_LDS_HEAP = 0, 1, 2
 
 COORD_DTYPES = {COORD_F32: np.float32, COORD_F64: np.float64, COORD_U32: np.uint32,
                 COORD_U64: np.uint64}
@@ -74,13 +75,13 @@ class _Tree(C.Structure):
 
 class QueryOpts(C.Structure):
     _fields_ = [("max_k", C.c_int32), ("mode", C.c_int32), ("algorithm", C.c_int32),
-                ("group_width", C.c_int32), ("leaf_bucket", C.c_int32), ("reserved1", C.c_int32),
+                ("group_width", C.c_int32), ("leaf_bucket", C.c_int32), ("topk", C.c_int32),
                 ("seed_bound", C.c_void_p), ("stats", C.c_void_p)]
 
 
 EXPORTS = [
     "ft_last_error", "ft_version", "ft_device_count", "ft_malloc", "ft_free", "ft_memcpy_h2d",
-    "ft_memcpy_d2h", "ft_memcpy_d2d", "ft_memset", "ft_stream_create", "ft_stream_destroy",
+    "ft_memcpy_d2h", "ft_memcpy_d2d", "ft_memset", "ft_copy_kernel", "ft_stream_create", "ft_stream_destroy",
     "ft_stream_synchronize", "ft_event_create", "ft_event_destroy", "ft_event_record",
     "ft_event_elapsed_ms", "ft_tree_alloc", "ft_tree_ensure", "ft_tree_free",
     "ft_scratch_create", "ft_scratch_destroy", "ft_scratch_set_profiling", "ft_scratch_stage_ms",
@@ -111,6 +112,7 @@ def lib():
         for f in ("ft_memcpy_h2d", "ft_memcpy_d2h", "ft_memcpy_d2d"):
             getattr(L, f).argtypes = [vp, vp, sz, vp]
         L.ft_memset.argtypes = [vp, i32, sz, vp]
+        L.ft_copy_kernel.argtypes = [vp, vp, sz, vp]
         L.ft_stream_create.argtypes = [C.POINTER(C.c_void_p)]
         L.ft_stream_destroy.argtypes = [vp]
         L.ft_stream_synchronize.argtypes = [vp]
@@ -186,9 +188,10 @@ class DeviceArray:
 
     def to_host_range(self, start: int, count: int, stream=None):
         """Copies elements [start, start + count) to the host."""
-        out = np.empty(count, self.dtype)
+        out = np.empty(int(count), self.dtype)
         if count:
-            _check(lib().ft_memcpy_d2h(out.ctypes.data, self.ptr + start * self.dtype.itemsize,
+            _check(lib().ft_memcpy_d2h(out.ctypes.data,
+                                       int(self.ptr) + int(start) * self.dtype.itemsize,
                                        out.nbytes, stream))
         return out
 
@@ -456,9 +459,9 @@ def build_bvh(p: particles, tree: TreeSoA, scratch: BvhScratch | None = None, bb
 # Queries
 # ---------------------------------------------------------------------------
 def _opts(max_k=0, mode=MODE_CANONICAL, algorithm=ALGO_AUTO, group_width=0, leaf_bucket=0,
-          seed_bound=None, stats=None):
+          seed_bound=None, stats=None, topk=0):
     return QueryOpts(int(max_k), int(mode), int(algorithm), int(group_width), int(leaf_bucket),
-                     0, _ptr(seed_bound), _ptr(stats))
+                     int(topk), _ptr(seed_bound), _ptr(stats))
 
 
 def _outputs(tree, nq, k, results, dists):

@@ -2,6 +2,7 @@
 // reference's error behaviour, host-pointer staging, tree / scratch lifetime.
 #include "ft_internal.hpp"
 
+#include <algorithm>
 #include <climits>
 #include <cstdio>
 #include <limits>
@@ -155,6 +156,27 @@ int ft_memcpy_d2d(void *dst, const void *src, size_t bytes, ft_stream s) {
 }
 int ft_memset(void *dst, int value, size_t bytes, ft_stream s) {
     FT_HIP(hipMemsetAsync(dst, value, bytes, as_stream(s)));
+    return FT_OK;
+}
+
+// A plain streaming copy kernel (16 bytes per lane per step): the known-size
+// reference for hardware byte counters (bytes read = bytes written = n).
+__global__ __launch_bounds__(256) void copy16_kernel(const uint4 *__restrict__ src,
+                                                     uint4 *__restrict__ dst, size_t n16) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16;
+         i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+int ft_copy_kernel(void *dst, const void *src, size_t bytes, ft_stream s) {
+    if (bytes % 16 || ((uintptr_t)dst | (uintptr_t)src) % 16)
+        return fail(FT_EINVAL, "ft_copy_kernel: 16-byte aligned sizes and pointers only");
+    const size_t n16 = bytes / 16;
+    if (!n16) return FT_OK;
+    const size_t blocks = std::min<size_t>((n16 + 255) / 256, 8192);
+    hipLaunchKernelGGL(copy16_kernel, dim3((unsigned)blocks), dim3(256), 0, as_stream(s),
+                       (const uint4 *)src, (uint4 *)dst, n16);
+    FT_CHECK_LAUNCH();
     return FT_OK;
 }
 int ft_stream_create(ft_stream *s) {

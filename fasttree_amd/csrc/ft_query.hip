@@ -288,16 +288,18 @@ struct PacketArgs {
     uint64_t *stats;
 };
 
-template <typename T, int M, bool SEEDED, bool PACKED, bool LISTED>
+template <typename T, int M, bool HEAP, bool SEEDED, bool PACKED, bool LISTED>
 __global__ __launch_bounds__(64) void packet_knn_kernel(
     DTree<T> t, QSrc<T> qs, PacketArgs a, const typename CoordTraits<T>::D *__restrict__ seed,
     uint64_t *__restrict__ res, typename CoordTraits<T>::D *__restrict__ dists) {
     using D = typename CoordTraits<T>::D;
-    using H = typename TopKSel<D, M, PACKED>::type;
-    static_assert(M <= 32 || PACKED, "LDS heap tier needs packed integer keys");
+    using H = typename std::conditional<HEAP, TopKHeapLds<M>,
+                                        typename TopKSel<D, M, PACKED>::type>::type;
+    static_assert(!HEAP || PACKED, "LDS heap tier needs packed integer keys");
+    static_assert(HEAP || M <= 32, "register tiers hold at most 32 entries");
     __shared__ int32_t stk[PK_STACK];
     [[maybe_unused]] uint64_t *heap_mem = nullptr;
-    if constexpr (M > 32) {
+    if constexpr (HEAP) {
         __shared__ uint64_t heap[M * 64];
         heap_mem = heap;
     }
@@ -315,7 +317,7 @@ __global__ __launch_bounds__(64) void packet_knn_kernel(
         if (active) load_query(t, qs, q, px, py, pz, self);
         if (!a.excl) self = -1;
         H h;
-        if constexpr (M > 32) h.key = heap_mem;
+        if constexpr (HEAP) h.key = heap_mem;
         if constexpr (PACKED) h.init(k, (SEEDED && active) ? (uint64_t)seed[q] : ~0ull, a.ibits);
         else h.init(k, (SEEDED && active) ? seed[q] : dist_max<D>());
         packet_walk<T, H>(t, a.bucket, active, px, py, pz, self, h, stk, c);
@@ -551,11 +553,11 @@ static inline int bit_length(uint64_t v) {
     return b;
 }
 
-template <typename T, int M, bool SEEDED, bool PACKED, bool LISTED>
+template <typename T, int M, bool HEAP, bool SEEDED, bool PACKED, bool LISTED>
 static void launch_pk(hipStream_t s, dim3 g, const DTree<T> &dt, const QSrc<T> &qs,
                       const PacketArgs &a, const typename CoordTraits<T>::D *seed, uint64_t *res,
                       typename CoordTraits<T>::D *d) {
-    hipLaunchKernelGGL((packet_knn_kernel<T, M, SEEDED, PACKED, LISTED>), g, dim3(64), 0, s, dt,
+    hipLaunchKernelGGL((packet_knn_kernel<T, M, HEAP, SEEDED, PACKED, LISTED>), g, dim3(64), 0, s, dt,
                        qs, a, seed, res, d);
 }
 
@@ -564,7 +566,7 @@ static void launch_pk(hipStream_t s, dim3 g, const DTree<T> &dt, const QSrc<T> &
 //   pass 2  (packed only) unseeded packed run of the queries pass 1 listed;
 //   pass 3  unseeded lexicographic run of what is still listed -- exact.
 // The LISTED passes read list lengths on the device: no host round trip.
-template <typename T, int M>
+template <typename T, int M, bool HEAP>
 static int packet_run(hipStream_t s, const DTree<T> &dt, const QSrc<T> &qs, int nq, int k,
                       int excl, int bucket, const typename CoordTraits<T>::D *seed, uint64_t *res,
                       typename CoordTraits<T>::D *d, uint64_t *stats) {
@@ -585,11 +587,11 @@ static int packet_run(hipStream_t s, const DTree<T> &dt, const QSrc<T> &qs, int 
     }
     const dim3 g(blocks_for(nq, 64));
     if (seed) {
-        if constexpr (INTD) launch_pk<T, M, true, true, false>(s, g, dt, qs, a, seed, res, d);
-        else launch_pk<T, M, true, false, false>(s, g, dt, qs, a, seed, res, d);
+        if constexpr (INTD) launch_pk<T, M, HEAP, true, true, false>(s, g, dt, qs, a, seed, res, d);
+        else launch_pk<T, M, false, true, false, false>(s, g, dt, qs, a, seed, res, d);
     } else {
-        if constexpr (INTD) launch_pk<T, M, false, true, false>(s, g, dt, qs, a, seed, res, d);
-        else launch_pk<T, M, false, false, false>(s, g, dt, qs, a, seed, res, d);
+        if constexpr (INTD) launch_pk<T, M, HEAP, false, true, false>(s, g, dt, qs, a, seed, res, d);
+        else launch_pk<T, M, false, false, false, false>(s, g, dt, qs, a, seed, res, d);
     }
     FT_CHECK_LAUNCH();
     if (!need_list) return FT_OK;
@@ -608,7 +610,7 @@ static int packet_run(hipStream_t s, const DTree<T> &dt, const QSrc<T> &qs, int 
             b.list_count = g_rerun.count;
             b.rerun = g_rerun.list[1];
             b.rerun_count = g_rerun.count + 1;
-            launch_pk<T, M, false, true, true>(s, dim3(waves), dt, qs, b, nullptr, res, d);
+            launch_pk<T, M, HEAP, false, true, true>(s, dim3(waves), dt, qs, b, nullptr, res, d);
             FT_CHECK_LAUNCH();
             lex_list = 1;
         }
@@ -617,12 +619,12 @@ static int packet_run(hipStream_t s, const DTree<T> &dt, const QSrc<T> &qs, int 
     b.list_count = g_rerun.count + lex_list;
     b.rerun = nullptr;
     b.rerun_count = nullptr;
-    if constexpr (M > 32) {
+    if constexpr (HEAP) {
         // exact: per-lane lexicographic LDS heap over the listed queries
         return launch_lane_lds<T, false, true>(s, dt, qs, nq, k, excl, res, d, nullptr, b.list,
                                                b.list_count);
     } else {
-        launch_pk<T, M, false, false, true>(s, dim3(waves), dt, qs, b, nullptr, res, d);
+        launch_pk<T, M, false, false, false, true>(s, dim3(waves), dt, qs, b, nullptr, res, d);
     }
     FT_CHECK_LAUNCH();
     return FT_OK;
@@ -631,16 +633,21 @@ static int packet_run(hipStream_t s, const DTree<T> &dt, const QSrc<T> &qs, int 
 template <typename T>
 static int launch_packet(hipStream_t s, const DTree<T> &dt, const QSrc<T> &qs, int nq, int k,
                          int excl, int bucket, const typename CoordTraits<T>::D *seed,
-                         uint64_t *res, typename CoordTraits<T>::D *d, uint64_t *stats) {
-    if (k <= 1) return packet_run<T, 1>(s, dt, qs, nq, k, excl, bucket, seed, res, d, stats);
-    if (k <= 4) return packet_run<T, 4>(s, dt, qs, nq, k, excl, bucket, seed, res, d, stats);
-    if (k <= 8) return packet_run<T, 8>(s, dt, qs, nq, k, excl, bucket, seed, res, d, stats);
-    if (k <= 16) return packet_run<T, 16>(s, dt, qs, nq, k, excl, bucket, seed, res, d, stats);
-    if (k <= 32) return packet_run<T, 32>(s, dt, qs, nq, k, excl, bucket, seed, res, d, stats);
-    if constexpr (!CoordTraits<T>::fp) {
-        if (k <= 64) return packet_run<T, 64>(s, dt, qs, nq, k, excl, bucket, seed, res, d, stats);
-        if (k <= 128)
-            return packet_run<T, 128>(s, dt, qs, nq, k, excl, bucket, seed, res, d, stats);
+                         uint64_t *res, typename CoordTraits<T>::D *d, uint64_t *stats,
+                         int topk) {
+    constexpr bool INTD = !CoordTraits<T>::fp;
+    const bool heap = INTD && (k > 32 || topk == FT_TOPK_LDS_HEAP);
+    if (!heap) {
+        if (k <= 1) return packet_run<T, 1, false>(s, dt, qs, nq, k, excl, bucket, seed, res, d, stats);
+        if (k <= 4) return packet_run<T, 4, false>(s, dt, qs, nq, k, excl, bucket, seed, res, d, stats);
+        if (k <= 8) return packet_run<T, 8, false>(s, dt, qs, nq, k, excl, bucket, seed, res, d, stats);
+        if (k <= 16) return packet_run<T, 16, false>(s, dt, qs, nq, k, excl, bucket, seed, res, d, stats);
+        if (k <= 32) return packet_run<T, 32, false>(s, dt, qs, nq, k, excl, bucket, seed, res, d, stats);
+    }
+    if constexpr (INTD) {
+        if (k <= 32) return packet_run<T, 32, true>(s, dt, qs, nq, k, excl, bucket, seed, res, d, stats);
+        if (k <= 64) return packet_run<T, 64, true>(s, dt, qs, nq, k, excl, bucket, seed, res, d, stats);
+        if (k <= 128) return packet_run<T, 128, true>(s, dt, qs, nq, k, excl, bucket, seed, res, d, stats);
     }
     return fail(FT_EINVAL, "packet traversal: k above the heap tiers");
 }
@@ -685,7 +692,7 @@ static int knn_dispatch(hipStream_t s, const ft_tree *tree, const QSrc<T> &qs, i
     if (algo == FT_ALGO_PACKET && packet_ok) {
         const int bucket = o.leaf_bucket > 0 ? std::min(o.leaf_bucket, 64) : 32;
         return launch_packet<T>(s, dt, qs, nq, k, excl, bucket, (const D *)o.seed_bound, res,
-                                dists, stats);
+                                dists, stats, o.topk);
     }
     if (k <= 32) {
         if (algo == FT_ALGO_PLAIN)

@@ -111,3 +111,60 @@ def clustered_fixed(n: int, seed: int, bits: int = 21, box: float = 100.0, dtype
                     n_halos: int = 2000) -> tuple:
     x, y, z = nfw_clustered(n, seed, box=box, n_halos=n_halos)
     return tuple(to_fixed(a, box, bits, dtype) for a in (x, y, z))
+
+
+def batched_workload(seed: int = 0, n_trees: int = 64, n_min: int = 10_000,
+                     n_max: int = 2_000_000, n_queries: int = 4_000_000, ks=(1, 8, 32, 128),
+                     requests_per_tree: int = 4, bits: int = 21, box: float = 100.0,
+                     dtype=np.uint32):
+    """Uneven multi-tree workload.
+
+    Tree sizes are log-uniform in [n_min, n_max].  Each tree is a clustered
+    snapshot whose halo count keeps the 10M snapshot's 3,500 particles per
+    halo, in fixed-point coordinates of `bits` bits per axis.  n_queries
+    external queries are split over n_trees * requests_per_tree requests by a
+    flat Dirichlet draw (largest remainders rounded up), each request with k
+    drawn from `ks`; a query is a random particle of its tree displaced by a
+    Gaussian of sigma = box / 1000, so queries follow the tree's density.
+
+    Returns (sizes, points per tree, requests (tree, k, query_first,
+    num_queries, result_first), (qx, qy, qz)).
+    """
+    u = uniform01(seed, 40, n_trees)
+    sizes = np.floor(np.exp(np.log(n_min) + u * (np.log(n_max) - np.log(n_min)))).astype(np.int64)
+    pts = []
+    for t in range(n_trees):
+        n = int(sizes[t])
+        halos = max(4, int(round(n / 3500.0)))
+        pts.append(clustered_fixed(n, seed * 1000 + t + 1, bits=bits, box=box, dtype=dtype,
+                                   n_halos=halos))
+    nr = n_trees * requests_per_tree
+    g = -np.log(np.maximum(uniform01(seed, 41, nr), 1e-300))
+    share = g / g.sum() * n_queries
+    counts = np.floor(share).astype(np.int64)
+    short = n_queries - int(counts.sum())
+    if short > 0:
+        order = np.argsort(-(share - counts), kind="stable")
+        counts[order[:short]] += 1
+    kidx = np.floor(uniform01(seed, 42, nr) * len(ks)).astype(np.int64)
+    reqs, qs = [], [[], [], []]
+    qf = rf = 0
+    top = float((1 << bits) - 1)
+    scale = float(1 << bits) / box
+    for r in range(nr):
+        t = r // requests_per_tree
+        m = int(counts[r])
+        k = int(ks[kidx[r]])
+        reqs.append((t, k, qf, m, rf))
+        src = pts[t]
+        pick = np.floor(uniform01(seed, 1000 + r, m) * len(src[0])).astype(np.int64)
+        for d in range(3):
+            z1 = uniform01(seed, 100000 + 3 * r + d, m)
+            z2 = uniform01(seed, 200000 + 3 * r + d, m)
+            gauss = np.sqrt(-2.0 * np.log(np.maximum(z1, 1e-300))) * np.cos(2.0 * np.pi * z2)
+            v = src[d][pick].astype(np.float64) + gauss * (box / 1000.0) * scale
+            qs[d].append(np.clip(np.floor(v), 0, top).astype(dtype))
+        qf += m
+        rf += m * k
+    q = tuple(np.concatenate(a) if a else np.zeros(0, dtype) for a in qs)
+    return sizes, pts, reqs, q

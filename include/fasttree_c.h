@@ -59,6 +59,10 @@ extern "C" {
 #define FT_ALGO_GROUPED 2 /* SFC-grouped lockstep DFS with a wave vote (reference self-kNN)    */
 #define FT_ALGO_PACKET 3  /* one shared traversal per wavefront of SFC-adjacent queries        */
 
+#define FT_TOPK_AUTO 0      /* registers for k <= 32, an LDS heap above (integer metrics) */
+#define FT_TOPK_REGISTERS 1 /* sorted (d2, index) entries in registers, k <= 32          */
+#define FT_TOPK_LDS_HEAP 2  /* a max-heap of packed keys per lane in LDS (integer metrics)*/
+
 /* Stream handle: a hipStream_t passed as an opaque pointer.  NULL = null stream. */
 typedef void *ft_stream;
 
@@ -113,7 +117,7 @@ typedef struct ft_query_opts {
     int32_t group_width; /* lanes that share a traversal (FT_ALGO_PACKET: 64): 0 = auto   */
     int32_t leaf_bucket; /* FT_ALGO_PACKET: subtrees with at most this many leaves are    */
                          /* scanned as one contiguous run of leaf slots (1..64, 0 = auto) */
-    int32_t reserved1;
+    int32_t topk;        /* FT_TOPK_*: top-k container of the packet traversal            */
     const void *seed_bound; /* optional per-query squared-distance bound (dist_t[num_q]):*/
                             /* the search is first limited to it; queries that find     */
                             /* fewer than k neighbours inside it are re-run unbounded.  */
@@ -133,6 +137,9 @@ int ft_memcpy_h2d(void *dst, const void *src, size_t bytes, ft_stream s);
 int ft_memcpy_d2h(void *dst, const void *src, size_t bytes, ft_stream s);
 int ft_memcpy_d2d(void *dst, const void *src, size_t bytes, ft_stream s);
 int ft_memset(void *dst, int value, size_t bytes, ft_stream s);
+/* Device-to-device copy by a plain streaming kernel (16-byte aligned): the       */
+/* known-size reference against which hardware byte counters are scaled.         */
+int ft_copy_kernel(void *dst, const void *src, size_t bytes, ft_stream s);
 int ft_stream_create(ft_stream *s);
 int ft_stream_destroy(ft_stream s);
 int ft_stream_synchronize(ft_stream s);

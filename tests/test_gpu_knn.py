@@ -35,7 +35,8 @@ def nfw_points(n, seed, ct, halos=200):
     return tuple(a.astype(O.COORD_DTYPES[ct]) for a in (x, y, z))
 
 
-ALGOS = {"packet": 3, "grouped": 2, "plain": 1}
+ALGOS = {"packet": 3, "grouped": 2, "plain": 1, "packet-heap": 3}
+TOPK = {"packet-heap": 2}  # LDS heap container also for k <= 32 (integer metrics)
 
 DATASETS = [
     ("ref-uniform-u64-periodic", O.CT_U64, O.SFC_PH, 21, 1, lambda: ref_points(O.GEN_UNIFORM, 1000, 42, O.CT_U64)),
@@ -55,12 +56,12 @@ def test_self_knn_canonical(ft, orc, ds, algo):
     tree, t = make(ft, ct, sfc, bits, periodic, gen())
     # packet: register tiers (k <= 32) and, for integer metrics, the LDS heap
     # tier (32 < k <= 128); FP metrics above 32 take the per-lane heap walk.
-    ks = (1, 5, 16, 32, 33, 64, 128) if algo == "packet" else (1, 16, 32, 33, 64)
+    ks = (1, 5, 16, 32, 33, 64, 128) if algo.startswith("packet") else (1, 16, 32, 33, 64)
     for k in ks:
         for excl in (True, False):
             print(f"[knn] {name} {algo} k={k} excl={excl}", flush=True)
             r, d = ft.self_knn_query(tree, k, exclude_self=excl, max_k=256,
-                                     algorithm=ALGOS[algo])
+                                     algorithm=ALGOS[algo], topk=TOPK.get(algo, 0))
             rb, db = O.knn_bruteforce(t, k, excl)
             assert np.array_equal(d, db), f"d2 k={k} excl={excl}"
             assert np.array_equal(r, rb), f"idx k={k} excl={excl}"
@@ -165,8 +166,8 @@ def test_large_sample_against_bruteforce(ft, orc):
     tree, t = make(ft, O.CT_U64, O.SFC_PH, 21, 1, pts)
     ids = np.random.default_rng(5).choice(300_000, 3000, replace=False).astype(np.int32)
     rb, db = O.knn_bruteforce(t, 32, True, leaf_ids=ids)
-    for algo in ("packet", "grouped", "plain"):
-        r, d = ft.self_knn_query(tree, 32, algorithm=ALGOS[algo])
+    for algo in ("packet", "packet-heap", "grouped", "plain"):
+        r, d = ft.self_knn_query(tree, 32, algorithm=ALGOS[algo], topk=TOPK.get(algo, 0))
         assert np.array_equal(d[ids], db) and np.array_equal(r[ids], rb), algo
 
 
