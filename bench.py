@@ -211,9 +211,12 @@ def main():
     knn_bytes = n * (3 * s_b + 16 * k) + (12 * s_b + 8) * n
     build_bytes = 492 * n
     achieved = knn_bytes / (query_ms * 1e-3) / 1e9
+    traffic = measured_traffic()
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
-            "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 5), "traffic": None,
-            "kernel": "packet_knn_kernel<uint64_t,32>", "algorithmic_bytes": knn_bytes}
+            "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 5), "traffic": traffic,
+            "kernel": "packet_knn_kernel<uint64_t,32> (unseeded, packed keys)",
+            "algorithmic_bytes": knn_bytes,
+            "traffic_source": TRAFFIC_PROFILE if traffic is not None else None}
     build_gbs = build_bytes / (build_ms * 1e-3) / 1e9
 
     cpu = None
@@ -256,6 +259,21 @@ def main():
         "batched": batched,
     }
     print(json.dumps(out), flush=True)
+
+
+TRAFFIC_PROFILE = "profiles/c2_hbm_traffic_counters.md"
+
+
+def measured_traffic():
+    """HBM bytes per self-kNN launch from the committed FETCH_SIZE / WRITE_SIZE
+    counter summary (scaled by the copy-kernel calibration), or None."""
+    import re
+    try:
+        text = open(os.path.join(ROOT, TRAFFIC_PROFILE)).read()
+    except OSError:
+        return None
+    m = re.search(r"Self-kNN traffic: ([0-9.]+) GB per launch", text)
+    return float(m.group(1)) * 1e9 if m else None
 
 
 def batched_bench(ft, steps, warmup, seed):

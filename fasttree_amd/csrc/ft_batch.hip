@@ -123,7 +123,7 @@ __device__ inline int grab_item(uint32_t *ctr) {
 }
 
 // Packet tiers.  HEAP: LDS heap of capacity M, else the register top-k.
-template <typename T, int M, bool HEAP>
+template <typename T, int M, bool HEAP, int MK>
 __global__ __launch_bounds__(64) void batch_packet_kernel(BatchView<T> b,
                                                           const uint2 *__restrict__ items,
                                                           int n_items, uint32_t *__restrict__ ctr) {
@@ -163,7 +163,7 @@ __global__ __launch_bounds__(64) void batch_packet_kernel(BatchView<T> b,
         if constexpr (HEAP) h.key = heap_mem;
         if constexpr (PACKED) h.init(k, ~0ull, 32 - __clz(t.n));
         else h.init(k, dist_max<D>());
-        packet_walk<T, H>(t, q.bucket, active, px, py, pz, -1, h, stk, c);
+        packet_walk<T, H, MK>(t, q.bucket, active, px, py, pz, -1, h, stk, c);
         if (active) {
             bool redo = false;
             if constexpr (PACKED) redo = h.needs_exact(k);
@@ -345,6 +345,7 @@ struct ft_batch {
     uint64_t *dkeys = nullptr, *dsorted = nullptr;
     int item_first[ft::V_COUNT] = {}, item_count[ft::V_COUNT] = {};
     int lane_width = 64;
+    int mk = 0;  // metric kind shared by every tree (MK_ANY when they differ)
     ft::Scratch sort_scratch;
     std::vector<ft::Scratch> build_scratch;
     std::vector<hipStream_t> side;
@@ -426,6 +427,11 @@ template <typename T> static int upload_tables(ft_batch *b) {
     const int nt = (int)b->trees.size(), nr = (int)b->reqs.size();
     std::vector<DTree<T>> dts(nt);
     for (int i = 0; i < nt; ++i) dts[i] = make_dtree<T>(*b->trees[i]);
+    b->mk = nt ? metric_kind<T>(dts[0].m) : MK_ANY;
+    for (int i = 1; i < nt; ++i) {
+        const Metric &a0 = dts[0].m, &ai = dts[i].m;
+        if (metric_kind<T>(ai) != b->mk || ai.bits != a0.bits || ai.box != a0.box) b->mk = MK_ANY;
+    }
     std::vector<BatchReq> rq(nr);
     std::vector<std::vector<std::pair<double, uint2>>> items(V_COUNT);
     int64_t entry = 0;
@@ -521,29 +527,31 @@ template <typename T> static BatchView<T> view(ft_batch *b) {
 }
 
 // Launches one tier kernel over `n_items` items starting at `first`.
-template <typename T>
-static int launch_variant(ft_batch *b, hipStream_t s, int v, int first, int n_items,
-                          uint32_t *ctr) {
-    if (n_items <= 0) return FT_OK;
+template <typename T, int MK>
+static int launch_variant_mk(ft_batch *b, hipStream_t s, int v, int first, int n_items,
+                             uint32_t *ctr) {
     const BatchView<T> bv = view<T>(b);
     const uint2 *it = b->ditems + first;
     const unsigned waves = std::min<unsigned>((unsigned)n_items, 2048u);
     constexpr bool FP = CoordTraits<T>::fp;
+#define FT_BPK(M_, HEAP_)                                                                     \
+    hipLaunchKernelGGL((batch_packet_kernel<T, M_, HEAP_, MK>), dim3(waves), dim3(64), 0, s, bv, \
+                       it, n_items, ctr)
     switch (v) {
-    case V_PK1: hipLaunchKernelGGL((batch_packet_kernel<T, 1, false>), dim3(waves), dim3(64), 0, s, bv, it, n_items, ctr); break;
-    case V_PK4: hipLaunchKernelGGL((batch_packet_kernel<T, 4, false>), dim3(waves), dim3(64), 0, s, bv, it, n_items, ctr); break;
-    case V_PK8: hipLaunchKernelGGL((batch_packet_kernel<T, 8, false>), dim3(waves), dim3(64), 0, s, bv, it, n_items, ctr); break;
-    case V_PK16: hipLaunchKernelGGL((batch_packet_kernel<T, 16, false>), dim3(waves), dim3(64), 0, s, bv, it, n_items, ctr); break;
-    case V_PK32: hipLaunchKernelGGL((batch_packet_kernel<T, 32, false>), dim3(waves), dim3(64), 0, s, bv, it, n_items, ctr); break;
+    case V_PK1: FT_BPK(1, false); break;
+    case V_PK4: FT_BPK(4, false); break;
+    case V_PK8: FT_BPK(8, false); break;
+    case V_PK16: FT_BPK(16, false); break;
+    case V_PK32: FT_BPK(32, false); break;
     case V_HP32:
     case V_HP64:
     case V_HP128:
         if constexpr (FP) {
             return fail(FT_EINVAL, "LDS heap packet tier needs integer coordinates");
         } else {
-            if (v == V_HP32) hipLaunchKernelGGL((batch_packet_kernel<T, 32, true>), dim3(waves), dim3(64), 0, s, bv, it, n_items, ctr);
-            else if (v == V_HP64) hipLaunchKernelGGL((batch_packet_kernel<T, 64, true>), dim3(waves), dim3(64), 0, s, bv, it, n_items, ctr);
-            else hipLaunchKernelGGL((batch_packet_kernel<T, 128, true>), dim3(waves), dim3(64), 0, s, bv, it, n_items, ctr);
+            if (v == V_HP32) FT_BPK(32, true);
+            else if (v == V_HP64) FT_BPK(64, true);
+            else FT_BPK(128, true);
         }
         break;
     case V_LANE: {
@@ -556,8 +564,18 @@ static int launch_variant(ft_batch *b, hipStream_t s, int v, int first, int n_it
     }
     default: return fail(FT_EINVAL, "unknown batch tier");
     }
+#undef FT_BPK
     FT_CHECK_LAUNCH();
     return FT_OK;
+}
+
+template <typename T>
+static int launch_variant(ft_batch *b, hipStream_t s, int v, int first, int n_items,
+                          uint32_t *ctr) {
+    if (n_items <= 0) return FT_OK;
+    if (b->mk == MK_WRAP) return launch_variant_mk<T, MK_WRAP>(b, s, v, first, n_items, ctr);
+    if (b->mk == MK_OPEN) return launch_variant_mk<T, MK_OPEN>(b, s, v, first, n_items, ctr);
+    return launch_variant_mk<T, MK_ANY>(b, s, v, first, n_items, ctr);
 }
 
 template <typename T> static int launch_exact(ft_batch *b, hipStream_t s) {

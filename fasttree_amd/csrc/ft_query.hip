@@ -288,7 +288,7 @@ struct PacketArgs {
     uint64_t *stats;
 };
 
-template <typename T, int M, bool HEAP, bool SEEDED, bool PACKED, bool LISTED>
+template <typename T, int M, bool HEAP, bool SEEDED, bool PACKED, bool LISTED, int MK>
 __global__ __launch_bounds__(64) void packet_knn_kernel(
     DTree<T> t, QSrc<T> qs, PacketArgs a, const typename CoordTraits<T>::D *__restrict__ seed,
     uint64_t *__restrict__ res, typename CoordTraits<T>::D *__restrict__ dists) {
@@ -320,7 +320,7 @@ __global__ __launch_bounds__(64) void packet_knn_kernel(
         if constexpr (HEAP) h.key = heap_mem;
         if constexpr (PACKED) h.init(k, (SEEDED && active) ? (uint64_t)seed[q] : ~0ull, a.ibits);
         else h.init(k, (SEEDED && active) ? seed[q] : dist_max<D>());
-        packet_walk<T, H>(t, a.bucket, active, px, py, pz, self, h, stk, c);
+        packet_walk<T, H, MK>(t, a.bucket, active, px, py, pz, self, h, stk, c);
         if (active) {
             bool redo = false;
             if constexpr (PACKED) redo = h.needs_exact(k);
@@ -557,8 +557,20 @@ template <typename T, int M, bool HEAP, bool SEEDED, bool PACKED, bool LISTED>
 static void launch_pk(hipStream_t s, dim3 g, const DTree<T> &dt, const QSrc<T> &qs,
                       const PacketArgs &a, const typename CoordTraits<T>::D *seed, uint64_t *res,
                       typename CoordTraits<T>::D *d) {
-    hipLaunchKernelGGL((packet_knn_kernel<T, M, HEAP, SEEDED, PACKED, LISTED>), g, dim3(64), 0, s, dt,
-                       qs, a, seed, res, d);
+    switch (metric_kind<T>(dt.m)) {
+    case MK_WRAP:
+        hipLaunchKernelGGL((packet_knn_kernel<T, M, HEAP, SEEDED, PACKED, LISTED, MK_WRAP>), g,
+                           dim3(64), 0, s, dt, qs, a, seed, res, d);
+        break;
+    case MK_OPEN:
+        hipLaunchKernelGGL((packet_knn_kernel<T, M, HEAP, SEEDED, PACKED, LISTED, MK_OPEN>), g,
+                           dim3(64), 0, s, dt, qs, a, seed, res, d);
+        break;
+    default:
+        hipLaunchKernelGGL((packet_knn_kernel<T, M, HEAP, SEEDED, PACKED, LISTED, MK_ANY>), g,
+                           dim3(64), 0, s, dt, qs, a, seed, res, d);
+        break;
+    }
 }
 
 // Runs the packet traversal and its exact re-runs:

@@ -75,6 +75,124 @@ __device__ inline typename CoordTraits<T>::D node_d2(const DTree<T> &t, T px, T 
                      t.mnz[node], t.mxz[node]);
 }
 
+// ---------------------------------------------------------------------------
+// Compile-time metric of a walk.  MK_ANY reads the Metric at run time; MK_WRAP
+// (periodic) and MK_OPEN (non-periodic) fix the flag so inner loops carry no
+// metric branches.  Integer MK_WRAP needs B <= 31 (every valid config): the
+// wrapped difference is then the sign-extended low B bits of a 32-bit
+// difference, and _DIST_SHIFT is 0.
+// ---------------------------------------------------------------------------
+enum : int { MK_ANY = 0, MK_WRAP = 1, MK_OPEN = 2 };
+
+template <typename T> inline int metric_kind(const Metric &m) {
+    if (!m.periodic) return MK_OPEN;
+    if (!CoordTraits<T>::fp && m.bits > 31) return MK_ANY;
+    return MK_WRAP;
+}
+
+// Signed B-bit wrapped difference a - b (sh = 32 - B).
+__host__ __device__ __forceinline__ int32_t wrap_diff32(uint32_t a, uint32_t b, int sh) {
+    return (int32_t)((a - b) << sh) >> sh;
+}
+__host__ __device__ __forceinline__ uint64_t wrap_abs32(uint32_t a, uint32_t b, int sh) {
+    const int32_t v = wrap_diff32(a, b, sh);
+    return (uint64_t)(uint32_t)(v < 0 ? -v : v);
+}
+
+// Float32 pre-filter of the integer periodic metric (B <= 31).  The float sum
+// of the three squared wrapped gaps (conversions, one product, two fmas) stays
+// within 5 * 2^-24 (relative) of the exact sum, and the bound is raised by
+// 2^-20 after its own conversion, so wrap_filter_near() is true for every leaf
+// whose exact squared distance is at most w; leaves it passes are then tested
+// exactly.
+__host__ __device__ __forceinline__ float wrap_filter_bound(uint64_t w) {
+    return (float)w * (1.0f + 0x1p-20f);
+}
+__host__ __device__ __forceinline__ bool wrap_filter_near(uint32_t qx, uint32_t qy, uint32_t qz,
+                                                          uint32_t x, uint32_t y, uint32_t z,
+                                                          int sh, float wf) {
+    const float fx = (float)wrap_diff32(qx, x, sh);
+    const float fy = (float)wrap_diff32(qy, y, sh);
+    const float fz = (float)wrap_diff32(qz, z, sh);
+    float d = fx * fx;
+    d = __builtin_fmaf(fy, fy, d);
+    d = __builtin_fmaf(fz, fz, d);
+    return d <= wf;
+}
+
+// leaf_d2 / box_d2 with the metric kind fixed (same values).
+template <typename T, int MK>
+__device__ __forceinline__ typename CoordTraits<T>::D leaf_d2_k(const Metric &m, T px, T py, T pz,
+                                                               T x, T y, T z) {
+    if constexpr (MK == MK_ANY) {
+        return leaf_d2<T>(m, px, py, pz, x, y, z);
+    } else if constexpr (CoordTraits<T>::fp) {
+        T dx = ffabs(px - x), dy = ffabs(py - y), dz = ffabs(pz - z);
+        if constexpr (MK == MK_WRAP) {
+            const T L = (T)m.box;
+            dx = ffmin(dx, L - dx);
+            dy = ffmin(dy, L - dy);
+            dz = ffmin(dz, L - dz);
+        }
+        T s = dx * dx;
+        s = s + dy * dy;
+        s = s + dz * dz;
+        return (double)s;
+    } else if constexpr (MK == MK_WRAP) {
+        const int sh = 32 - m.bits;
+        const uint64_t gx = wrap_abs32((uint32_t)px, (uint32_t)x, sh);
+        const uint64_t gy = wrap_abs32((uint32_t)py, (uint32_t)y, sh);
+        const uint64_t gz = wrap_abs32((uint32_t)pz, (uint32_t)z, sh);
+        return gx * gx + gy * gy + gz * gz;
+    } else {
+        auto g = [&m](T p, T c) -> uint64_t {
+            return (uint64_t)(p < c ? (T)(c - p) : (T)(p - c)) >> m.dshift;
+        };
+        const uint64_t gx = g(px, x), gy = g(py, y), gz = g(pz, z);
+        return gx * gx + gy * gy + gz * gz;
+    }
+}
+
+template <typename T, int MK>
+__device__ __forceinline__ typename CoordTraits<T>::D box_d2_k(const Metric &m, T px, T py, T pz,
+                                                              T x0, T x1, T y0, T y1, T z0,
+                                                              T z1) {
+    if constexpr (MK == MK_ANY) {
+        return box_d2<T>(m, px, py, pz, x0, x1, y0, y1, z0, z1);
+    } else if constexpr (CoordTraits<T>::fp) {
+        Metric mm = m;
+        mm.periodic = (MK == MK_WRAP);
+        return box_d2<T>(mm, px, py, pz, x0, x1, y0, y1, z0, z1);
+    } else if constexpr (MK == MK_WRAP) {
+        const int sh = 32 - m.bits;
+        auto g = [&](T p, T lo, T hi) -> uint64_t {
+            T v = p;
+            if constexpr (sizeof(T) >= 8) v &= (T)((T(1) << m.bits) - T(1));
+            const uint64_t al = wrap_abs32((uint32_t)v, (uint32_t)lo, sh);
+            const uint64_t ah = wrap_abs32((uint32_t)v, (uint32_t)hi, sh);
+            const uint64_t w = al < ah ? al : ah;
+            return (v >= lo && v <= hi) ? 0ull : w;
+        };
+        const uint64_t gx = g(px, x0, x1), gy = g(py, y0, y1), gz = g(pz, z0, z1);
+        return gx * gx + gy * gy + gz * gz;
+    } else {
+        auto g = [&m](T p, T lo, T hi) -> uint64_t {
+            const uint64_t a = (uint64_t)(T)(lo - p) >> m.dshift;
+            const uint64_t b = (uint64_t)(T)(p - hi) >> m.dshift;
+            return p < lo ? a : (p > hi ? b : 0ull);
+        };
+        const uint64_t gx = g(px, x0, x1), gy = g(py, y0, y1), gz = g(pz, z0, z1);
+        return gx * gx + gy * gy + gz * gz;
+    }
+}
+
+template <typename T, int MK>
+__device__ __forceinline__ typename CoordTraits<T>::D node_d2_k(const DTree<T> &t, T px, T py,
+                                                               T pz, int node) {
+    return box_d2_k<T, MK>(t.m, px, py, pz, t.mnx[node], t.mxx[node], t.mny[node], t.mxy[node],
+                           t.mnz[node], t.mxz[node]);
+}
+
 // Returned index of a leaf: its rank, or orig_idx with RETURN_ORIG_INDICES.
 template <typename T> __device__ inline int32_t ret_index(const DTree<T> &t, int leaf_node) {
     return t.orig_indices ? t.orig[leaf_node] : (leaf_node - (t.n - 1));
@@ -243,12 +361,14 @@ template <int M> struct TopKPacked {
     }
     __device__ inline uint64_t worst_d() const { return key[M - 1] >> ib; }
     __device__ inline bool beats_worst(uint64_t d, int32_t i) const { return make(d, i) < key[M - 1]; }
+    // A lane that takes nothing carries the all-ones key, which no slot is
+    // above, so the chain needs no predicate.
     __device__ inline void insert(bool take, uint64_t d, int32_t i) {
-        uint64_t c = make(d, i);
+        uint64_t c = take ? make(d, i) : ~0ull;
 #pragma unroll
         for (int s = 0; s < M; ++s) {
-            const bool lt = take & (c < key[s]);
             const uint64_t t = key[s];
+            const bool lt = c < t;
             key[s] = lt ? c : t;
             c = lt ? t : c;
         }
@@ -303,7 +423,7 @@ struct PacketCounters {
     uint64_t nodes = 0, leaves = 0, inserts = 0, overflow = 0;
 };
 
-template <typename T, typename H>
+template <typename T, typename H, int MK>
 __device__ __forceinline__ void packet_walk(const DTree<T> &t, int bucket, bool active, T px, T py,
                                             T pz, int self, H &h, int32_t *stk,
                                             PacketCounters &c) {
@@ -331,11 +451,28 @@ __device__ __forceinline__ void packet_walk(const DTree<T> &t, int bucket, bool 
             bo = t.orig_indices ? t.orig[off + first + lane] : first + lane;
         }
         const D w = (D)h.worst_d();
-        uint64_t cand = 0;
-        for (int j = 0; j < cnt; ++j) {
-            const D d = leaf_d2<T>(t.m, px, py, pz, rl(bx, j), rl(by, j), rl(bz, j));
-            cand |= (uint64_t)(d <= w) << j;
+        uint32_t clo = 0, chi = 0;
+        const int c0 = cnt < 32 ? cnt : 32;
+        if constexpr (!CoordTraits<T>::fp && MK == MK_WRAP) {
+            // float32 pre-filter (wrap_filter_near): never rejects a leaf within
+            // w; the insertion below tests every passing leaf exactly.
+            const float wf = wrap_filter_bound((uint64_t)w);
+            const int sh = 32 - t.m.bits;
+            const uint32_t qx = (uint32_t)px, qy = (uint32_t)py, qz = (uint32_t)pz;
+            const uint32_t ex = (uint32_t)bx, ey = (uint32_t)by, ez = (uint32_t)bz;
+            auto near = [&](int j) -> bool {
+                return wrap_filter_near(qx, qy, qz, rl(ex, j), rl(ey, j), rl(ez, j), sh, wf);
+            };
+            for (int j = 0; j < c0; ++j) clo |= near(j) ? (1u << j) : 0u;
+            for (int j = 32; j < cnt; ++j) chi |= near(j) ? (1u << (j - 32)) : 0u;
+        } else {
+            auto near = [&](int j) -> bool {
+                return leaf_d2_k<T, MK>(t.m, px, py, pz, rl(bx, j), rl(by, j), rl(bz, j)) <= w;
+            };
+            for (int j = 0; j < c0; ++j) clo |= near(j) ? (1u << j) : 0u;
+            for (int j = 32; j < cnt; ++j) chi |= near(j) ? (1u << (j - 32)) : 0u;
         }
+        uint64_t cand = ((uint64_t)chi << 32) | clo;
         if (!active) cand = 0;
         const int sr = self - first;
         if (sr >= 0 && sr < cnt) cand &= ~(1ull << sr);
@@ -346,7 +483,7 @@ __device__ __forceinline__ void packet_walk(const DTree<T> &t, int bucket, bool 
             cand &= cand - 1;
             const T x = shfl_any(bx, j), y = shfl_any(by, j), z = shfl_any(bz, j);
             const int32_t ri = shfl_any(bo, j);
-            const D d = leaf_d2<T>(t.m, px, py, pz, x, y, z);
+            const D d = leaf_d2_k<T, MK>(t.m, px, py, pz, x, y, z);
             const bool take = has && h.beats_worst(d, ri);
             h.insert(take, d, ri);
             c.inserts += take ? 1 : 0;
@@ -369,8 +506,8 @@ __device__ __forceinline__ void packet_walk(const DTree<T> &t, int bucket, bool 
             } else {
                 const int l = uni(t.left[node]);
                 const int r = uni(t.right[node]);
-                const D ld = node_d2(t, px, py, pz, l);
-                const D rd = node_d2(t, px, py, pz, r);
+                const D ld = node_d2_k<T, MK>(t, px, py, pz, l);
+                const D rd = node_d2_k<T, MK>(t, px, py, pz, r);
                 const D w = (D)h.worst_d();
                 const bool nl = active && ld <= w;
                 const bool nr = active && rd <= w;
@@ -401,7 +538,7 @@ __device__ __forceinline__ void packet_walk(const DTree<T> &t, int bucket, bool 
         if (scan_cnt > 0) scan_range(scan_first, scan_cnt);
         while (next < 0 && sp > 0) {
             const int cn = uni(stk[--sp]);
-            const D dc = node_d2(t, px, py, pz, cn);
+            const D dc = node_d2_k<T, MK>(t, px, py, pz, cn);
             if (__any(active && dc <= (D)h.worst_d())) next = cn;
         }
         if (next < 0) break;

@@ -45,3 +45,18 @@ def test_cxx_dropin_header_builds_and_maps_errors(tmp_path, defs):
     out = subprocess.run([exe], capture_output=True, text=True, timeout=60)
     assert out.returncode == 0, out.stdout + out.stderr
     assert ("Instantiate self_knn_query with larger _MAX_K_." in out.stdout)
+
+
+@pytest.mark.skipif(not os.path.exists(HIPCC), reason="hipcc not installed")
+def test_float_prefilter_never_rejects_a_leaf_within_the_bound(tmp_path):
+    """The packet walk's float32 pre-filter for integer periodic metrics
+    (wrap_filter_near) passes every leaf whose exact d2 is at most the bound,
+    over bounds equal to, just below and just above d2, and B up to 31."""
+    src = os.path.join(ROOT, "tests", "host", "wrap_filter_check.cpp")
+    exe = str(tmp_path / "wrap_filter_check")
+    subprocess.check_call([HIPCC, "-O2", "-std=c++17", "-ffp-contract=off", "--cuda-host-only",
+                           "-x", "hip", src, "-o", exe], stderr=subprocess.DEVNULL)
+    out = subprocess.run([exe], capture_output=True, text=True, timeout=300)
+    print(out.stdout)
+    assert out.returncode == 0, out.stdout
+    assert "false rejections: 0," in out.stdout
