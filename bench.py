@@ -44,6 +44,8 @@ def main():
     ap.add_argument("--sweep-buckets", action="store_true")
     ap.add_argument("--topk", type=int, default=0,
                     help="packet top-k container: 0 auto, 1 registers, 2 LDS heap")
+    ap.add_argument("--window", type=int, default=0,
+                    help="packet window pre-scan (leaves on each side; 0 auto, -1 none)")
     ap.add_argument("--model", default=os.path.join(ROOT, "fasttree_amd", "models",
                                                     "nfw10m_int64_ph21_k32.npz"),
                     help="trained learned-radius model (npz); the learned step is skipped "
@@ -53,6 +55,8 @@ def main():
     ap.add_argument("--pmc-probe", action="store_true",
                     help="one build, one self-kNN and one 1 GiB copy kernel, then exit (for "
                          "hardware counter runs); prints no JSON")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="run only the batched multi-tree workload (development)")
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the batched multi-tree workload (64 trees, 4M queries)")
     args = ap.parse_args()
@@ -64,6 +68,11 @@ def main():
 
     import fasttree_amd as ft
     from fasttree_amd import synth
+
+    if args.batched_only:
+        print(json.dumps({"batched": batched_bench(ft, args.steps, args.warmup, args.seed)}),
+              flush=True)
+        return
 
     n, k = args.n, args.k
     t0 = time.time()
@@ -85,7 +94,7 @@ def main():
         ev[1].record(stream)
         ft.self_knn_query(tree, k, results=res, dists=dst, exclude_self=True,
                           stream=stream.handle, algorithm=algorithm, leaf_bucket=args.bucket,
-                          topk=args.topk)
+                          topk=args.topk, window=args.window)
         ev[2].record(stream)
         stream.synchronize()
         return ev[0].elapsed_ms(ev[1]), ev[1].elapsed_ms(ev[2])
@@ -150,7 +159,7 @@ def main():
             ev[1].record(stream)
             ft.self_knn_query(tree, k, results=res2, dists=dst2, stream=stream.handle,
                               seed_bound=seeds, stats=lst, leaf_bucket=args.bucket,
-                              topk=args.topk)
+                              topk=args.topk, window=args.window)
             ev[2].record(stream)
             stream.synchronize()
             if rep:
@@ -180,6 +189,13 @@ def main():
                 ev[1].record(stream)
                 stream.synchronize()
                 log(f"topk {tk} bucket {bk}: {ev[0].elapsed_ms(ev[1]):.2f} ms")
+        for wn in (-1, 32, 64, 128, 256):
+            ev[0].record(stream)
+            ft.self_knn_query(tree, k, results=res, dists=dst, stream=stream.handle,
+                              algorithm=ft.ALGO_PACKET, leaf_bucket=args.bucket or 64, window=wn)
+            ev[1].record(stream)
+            stream.synchronize()
+            log(f"window {wn}: {ev[0].elapsed_ms(ev[1]):.2f} ms")
     if args.seed_probe:
         dk = dst.to_host().reshape(n, k)[:, k - 1].astype(np.float64)
         for f in (1.0, 1.2, 1.5, 2.0, 0.9):
@@ -187,7 +203,8 @@ def main():
             st2 = ft.DeviceArray(8, np.uint64).zero()
             ev[0].record(stream)
             ft.self_knn_query(tree, k, results=res, dists=dst, stream=stream.handle,
-                              seed_bound=sd, stats=st2, leaf_bucket=args.bucket)
+                              seed_bound=sd, stats=st2, leaf_bucket=args.bucket,
+                              window=args.window)
             ev[1].record(stream)
             stream.synchronize()
             s2 = st2.to_host()
@@ -332,20 +349,24 @@ def batched_bench(ft, steps, warmup, seed):
     dst2 = ft.DeviceArray(total_r, np.uint64)
     scratch = ft.BvhScratch()
 
-    def loop():
+    def loop(verbose=False):
+        t0 = time.time()
         for t, tr in enumerate(trees):
             ft.build_bvh(ft.particles(*parts[t]), tr, scratch=scratch, stream=stream.handle)
-        for (t, kk, qf, m, rf) in reqs:
+        for i, (t, kk, qf, m, rf) in enumerate(reqs):
             if m == 0:
                 continue
-            o = ft.QueryOpts(256, 0, 0, 0, 0, 0, None, None)
+            o = ft.QueryOpts(256, 0, 0, 0, 0, 0, None, None, 0, 0)
             ft._check(L.ft_knn_query(stream.handle, C.byref(trees[t].c), dq[0].ptr + 4 * qf,
                                      dq[1].ptr + 4 * qf, dq[2].ptr + 4 * qf, int(kk), int(m),
                                      res2.ptr + 8 * rf, dst2.ptr + 8 * rf, C.byref(o)))
+            if verbose and i % 32 == 31:
+                stream.synchronize()
+                log(f"per-tree loop: {i + 1} requests, {time.time() - t0:.1f}s")
 
-    loop()
+    loop(verbose=True)
     stream.synchronize()
-    loop_ms = min(timed(loop) for _ in range(2))
+    loop_ms = timed(loop)
     same = True
     rng = np.random.default_rng(seed)
     for i in rng.choice(len(reqs), 24, replace=False):

@@ -76,7 +76,8 @@ class _Tree(C.Structure):
 class QueryOpts(C.Structure):
     _fields_ = [("max_k", C.c_int32), ("mode", C.c_int32), ("algorithm", C.c_int32),
                 ("group_width", C.c_int32), ("leaf_bucket", C.c_int32), ("topk", C.c_int32),
-                ("seed_bound", C.c_void_p), ("stats", C.c_void_p)]
+                ("seed_bound", C.c_void_p), ("stats", C.c_void_p), ("window", C.c_int32),
+                ("reserved2", C.c_int32)]
 
 
 EXPORTS = [
@@ -459,9 +460,9 @@ def build_bvh(p: particles, tree: TreeSoA, scratch: BvhScratch | None = None, bb
 # Queries
 # ---------------------------------------------------------------------------
 def _opts(max_k=0, mode=MODE_CANONICAL, algorithm=ALGO_AUTO, group_width=0, leaf_bucket=0,
-          seed_bound=None, stats=None, topk=0):
+          seed_bound=None, stats=None, topk=0, window=0):
     return QueryOpts(int(max_k), int(mode), int(algorithm), int(group_width), int(leaf_bucket),
-                     int(topk), _ptr(seed_bound), _ptr(stats))
+                     int(topk), _ptr(seed_bound), _ptr(stats), int(window), 0)
 
 
 def _outputs(tree, nq, k, results, dists):

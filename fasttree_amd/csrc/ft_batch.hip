@@ -37,7 +37,7 @@ int build_bvh_t(hipStream_t s, ft_tree *t, Scratch &S, const T *x, const T *y, c
                 const uint32_t *id, const int8_t *ghost, size_t n, const T *bbox6,
                 const uint64_t *keys);
 int radix_sort_u64(hipStream_t s, Scratch &S, const uint64_t *keys_in, size_t n,
-                   uint64_t *sorted_out);
+                   uint64_t *sorted_out, int key_bits);
 int scratch_ensure_public(Scratch &S, size_t n);
 
 // A request as the kernels see it (device copy).
@@ -184,7 +184,7 @@ __global__ __launch_bounds__(64) void batch_packet_kernel(BatchView<T> b,
 
 // Per-lane tier: `lanes` queries per item (item j covers entries
 // [j*lanes, (j+1)*lanes) of its request), LDS heap rows of capacity kcap.
-template <typename T>
+template <typename T, int MK>
 __global__ __launch_bounds__(64) void batch_lane_kernel(BatchView<T> b,
                                                         const uint2 *__restrict__ items,
                                                         int n_items, uint32_t *__restrict__ ctr,
@@ -215,8 +215,9 @@ __global__ __launch_bounds__(64) void batch_lane_kernel(BatchView<T> b,
             pz = b.qz[qi];
         }
         LdsHeap<D, false> h{hd, hi, lanes, lane, 0};
-        lane_heap_walk<T, false, true>(t, q.k, active, px, py, pz, -1, h, stk, lanes, lane,
-                                       visits, drops, inserts);
+        lane_heap_walk<T, false, true, LANE_STACK_CANON, MK>(t, q.k, active, px, py, pz, -1, h,
+                                                             stk, lanes, lane, visits, drops,
+                                                             inserts);
         if (active) {
             h.sort();
             uint64_t *res = b.res + q.result_first + (size_t)local * q.k;
@@ -269,8 +270,9 @@ __global__ __launch_bounds__(64) void batch_exact_kernel(BatchView<T> b, int lan
             pz = b.qz[qi];
         }
         LdsHeap<D, false> h{hd, hi, lanes, lane, 0};
-        lane_heap_walk<T, false, false>(t, active ? q.k : 0, active, px, py, pz, -1, h, stk, lanes,
-                                        lane, visits, drops, inserts);
+        lane_heap_walk<T, false, false, LANE_STACK_CANON, MK_ANY>(t, active ? q.k : 0, active, px,
+                                                                  py, pz, -1, h, stk, lanes, lane,
+                                                                  visits, drops, inserts);
         if (active) {
             h.sort();
             uint64_t *res = b.res + q.result_first + (size_t)local * q.k;
@@ -526,17 +528,37 @@ template <typename T> static BatchView<T> view(ft_batch *b) {
     return v;
 }
 
+// Waves of a persistent kernel: as many as can be resident on the device
+// (occupancy x compute units), at most one per item.
+template <typename K> static unsigned persistent_waves(K kernel, size_t dyn_lds, int n_items) {
+    static int n_cu = 0;
+    if (!n_cu) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            n_cu < 1)
+            n_cu = 256;
+    }
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 64, dyn_lds) != hipSuccess ||
+        per_cu < 1)
+        per_cu = 8;
+    return (unsigned)std::max(1L, std::min<long>((long)n_items, (long)per_cu * n_cu));
+}
+
 // Launches one tier kernel over `n_items` items starting at `first`.
 template <typename T, int MK>
 static int launch_variant_mk(ft_batch *b, hipStream_t s, int v, int first, int n_items,
                              uint32_t *ctr) {
     const BatchView<T> bv = view<T>(b);
     const uint2 *it = b->ditems + first;
-    const unsigned waves = std::min<unsigned>((unsigned)n_items, 2048u);
     constexpr bool FP = CoordTraits<T>::fp;
-#define FT_BPK(M_, HEAP_)                                                                     \
-    hipLaunchKernelGGL((batch_packet_kernel<T, M_, HEAP_, MK>), dim3(waves), dim3(64), 0, s, bv, \
-                       it, n_items, ctr)
+#define FT_BPK(M_, HEAP_)                                                                       \
+    do {                                                                                        \
+        auto kf = batch_packet_kernel<T, M_, HEAP_, MK>;                                        \
+        hipLaunchKernelGGL(kf, dim3(persistent_waves(kf, 0, n_items)), dim3(64), 0, s, bv, it,  \
+                           n_items, ctr);                                                       \
+    } while (0)
     switch (v) {
     case V_PK1: FT_BPK(1, false); break;
     case V_PK4: FT_BPK(4, false); break;
@@ -558,8 +580,9 @@ static int launch_variant_mk(ft_batch *b, hipStream_t s, int v, int first, int n
         size_t lds = 0;
         const int lanes = lane_tier_lanes<T>(b->kmax, lds);
         if (!lanes) return fail(FT_ELDS, "k too large for local memory: reduce k");
-        hipLaunchKernelGGL((batch_lane_kernel<T>), dim3(waves), dim3(64), lds, s, bv, it, n_items,
-                           ctr, lanes, b->kmax);
+        auto kf = batch_lane_kernel<T, MK>;
+        hipLaunchKernelGGL(kf, dim3(persistent_waves(kf, lds, n_items)), dim3(64), lds, s, bv, it,
+                           n_items, ctr, lanes, b->kmax);
         break;
     }
     default: return fail(FT_EINVAL, "unknown batch tier");
@@ -666,7 +689,7 @@ template <typename T> static int enqueue_sort(ft_batch *b, hipStream_t s) {
     hipLaunchKernelGGL((batch_key_kernel<T>), dim3(cdiv(b->n_entries, 256)), dim3(256), 0, s,
                        view<T>(b), b->n_entries, rbits, qb, b->dkeys);
     FT_CHECK_LAUNCH();
-    return radix_sort_u64(s, b->sort_scratch, b->dkeys, (size_t)b->n_entries, b->dsorted);
+    return radix_sort_u64(s, b->sort_scratch, b->dkeys, (size_t)b->n_entries, b->dsorted, 64);
 }
 
 // Tier kernels on parallel branches (or in order on s), then the exact pass.

@@ -701,6 +701,7 @@ static inline unsigned grid_for(size_t n, int block, unsigned cap = 0) {
 static int scratch_ensure(Scratch &S, size_t n) {
     if (n <= S.cap_n && S.block) return FT_OK;
     if (S.block) {
+        FT_HIP(hipDeviceSynchronize());  // earlier launches may still use the old block
         FT_HIP(hipFree(S.block));
         S.block = nullptr;
     }
@@ -829,12 +830,13 @@ static int radix_sort(hipStream_t s, Scratch &S, const uint64_t *keys_in, size_t
     return FT_OK;
 }
 
-// Stable sort of n 64-bit keys (all 64 bits) for other modules: sorted keys
-// to sorted_out, the permutation (sorted position -> input index) to S.perm.
+// Stable sort of n keys of key_bits significant bits for other modules:
+// sorted keys to sorted_out, the permutation (sorted position -> input index)
+// to S.perm.
 int radix_sort_u64(hipStream_t s, Scratch &S, const uint64_t *keys_in, size_t n,
-                   uint64_t *sorted_out) {
+                   uint64_t *sorted_out, int key_bits) {
     FT_TRY(scratch_ensure(S, n));
-    return radix_sort(s, S, keys_in, n, 64, sorted_out, false);
+    return radix_sort(s, S, keys_in, n, key_bits, sorted_out, false);
 }
 
 template <typename T> static int launch_topology(hipStream_t s, ft_tree *t, Scratch &S) {

@@ -281,43 +281,104 @@ __device__ inline void mlp_tile_forward(const MlpSmem &S, MlpWave &W, const floa
     }
 }
 
-// Inference: seeds (dist_t) = exp(ymean + ystd * out + margin) - eps, or the
-// raw prediction (log d2) when `pred` is given.
+// ===========================================================================
+// Inference (predictions and seeds).  Activations are kept transposed:
+// Z^T = W^T H^T, with the weight tile as the A operand and the activation tile
+// as B, so a layer's MFMA output (D: unit 4*(l/16)+i of the tile, sample
+// l%16) is already the B operand of the next layer and never goes through
+// LDS.  Each wave holds every weight tile in registers (A operands), the
+// biases as accumulator initial values, and streams 16-sample tiles.
+// Same roundings as the training forward: bf16 weights, bf16 normalised
+// inputs and hidden activations, fp32 accumulation and fp32 output layer.
+// Seeds (dist_t) = exp(ymean + ystd * out + margin) - eps; `pred` receives the
+// raw prediction (log d2) when given.
+// ===========================================================================
 template <typename D>
-__global__ __launch_bounds__(256) void mlp_forward_kernel(const float *__restrict__ prm,
-                                                          const float *__restrict__ norm,
-                                                          const float *__restrict__ feat, int n,
-                                                          float *__restrict__ pred, D *__restrict__ seed,
-                                                          double margin, double eps) {
-    __shared__ MlpSmem S;
-    __shared__ MlpWave Wv[4];
-    stage_weights(S, prm, norm);
-    __syncthreads();
-    MlpWave &W = Wv[threadIdx.x >> 6];
-    const int l = threadIdx.x & 63, lg = l >> 4, lr = l & 15;
+__global__ __launch_bounds__(256) void mlp_infer_kernel(const float *__restrict__ prm,
+                                                        const float *__restrict__ norm,
+                                                        const float *__restrict__ feat, int n,
+                                                        float *__restrict__ pred,
+                                                        D *__restrict__ seed, double margin,
+                                                        double eps) {
+    const int l = threadIdx.x & 63, lr = l & 15, lg = l >> 4;
+    auto wt = [&](int base, int kc, int nt) -> bf16x4 {  // W^T tile: W[kc*16+4lg+i][nt*16+lr]
+        bf16x4 v;
+        for (int i = 0; i < 4; ++i)
+            v[i] = (short)to_bf16(prm[base + (kc * 16 + 4 * lg + i) * NH + nt * 16 + lr]);
+        return v;
+    };
+    auto col = [&](int base, int nt) -> f32x4 {  // per D row: unit nt*16+4lg+i
+        f32x4 v;
+        for (int i = 0; i < 4; ++i) v[i] = prm[base + nt * 16 + 4 * lg + i];
+        return v;
+    };
+    bf16x4 a1[4], a2[4][4], a3[4][4];
+    f32x4 b1[4], b2[4], b3[4], w4[4];
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) {
+        a1[nt] = wt(P_W1, 0, nt);
+#pragma unroll
+        for (int kc = 0; kc < 4; ++kc) {
+            a2[nt][kc] = wt(P_W2, kc, nt);
+            a3[nt][kc] = wt(P_W3, kc, nt);
+        }
+        b1[nt] = col(P_B1, nt);
+        b2[nt] = col(P_B2, nt);
+        b3[nt] = col(P_B3, nt);
+        w4[nt] = col(P_W4, nt);
+    }
+    const float b4 = prm[P_B4];
+    float fm[4], fi[4];
+    for (int i = 0; i < 4; ++i) {
+        fm[i] = norm[4 * lg + i];
+        fi[i] = norm[NF + 4 * lg + i];
+    }
+    const float ymean = norm[2 * NF], ystd = norm[2 * NF + 1];
     const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
     const int nwaves = gridDim.x * 4;
     for (int s0 = wave * 16; s0 < n; s0 += nwaves * 16) {
-        Fwd F;
-        mlp_tile_forward(S, W, feat, s0, n, F);
-        if (lr == 0) {
-            for (int i = 0; i < 4; ++i) {
-                const int s = s0 + 4 * lg + i;
-                if (s >= n) continue;
-                const float y = S.ymean + S.ystd * F.out[i];
-                if (pred) pred[s] = y;
-                if (seed) {
-                    double v = exp((double)y + margin) - eps;
-                    if (v < 0) v = 0;
-                    if constexpr (std::is_floating_point<D>::value) {
-                        seed[s] = v;
-                    } else {
-                        seed[s] = v >= 9.0e18 ? (D)9000000000000000000ull : (D)v;
-                    }
+        const int s = s0 + lr;
+        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (s < n) v = ((const float4 *)(feat + (size_t)s * NF))[lg];
+        const float vi[4] = {v.x, v.y, v.z, v.w};
+        bf16x4 x;
+        for (int i = 0; i < 4; ++i) x[i] = (short)to_bf16((vi[i] - fm[i]) * fi[i]);
+        bf16x4 h1[4], h2[4];
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt) {
+            const f32x4 z = mfma16(a1[nt], x, b1[nt]);
+            for (int i = 0; i < 4; ++i) h1[nt][i] = (short)to_bf16(z[i] > 0.f ? z[i] : 0.f);
+        }
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt) {
+            f32x4 z = b2[nt];
+#pragma unroll
+            for (int kc = 0; kc < 4; ++kc) z = mfma16(a2[nt][kc], h1[kc], z);
+            for (int i = 0; i < 4; ++i) h2[nt][i] = (short)to_bf16(z[i] > 0.f ? z[i] : 0.f);
+        }
+        float part = 0.f;
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt) {
+            f32x4 z = b3[nt];
+#pragma unroll
+            for (int kc = 0; kc < 4; ++kc) z = mfma16(a3[nt][kc], h2[kc], z);
+            for (int i = 0; i < 4; ++i) part += (z[i] > 0.f ? z[i] : 0.f) * w4[nt][i];
+        }
+        part += __shfl_xor(part, 16, 64);
+        part += __shfl_xor(part, 32, 64);
+        if (lg == 0 && s < n) {
+            const float y = ymean + ystd * (part + b4);
+            if (pred) pred[s] = y;
+            if (seed) {
+                double sv = exp((double)y + margin) - eps;
+                if (sv < 0) sv = 0;
+                if constexpr (std::is_floating_point<D>::value) {
+                    seed[s] = sv;
+                } else {
+                    seed[s] = sv >= 9.0e18 ? (D)9000000000000000000ull : (D)sv;
                 }
             }
         }
-        __builtin_amdgcn_wave_barrier();
     }
 }
 
@@ -564,8 +625,8 @@ int ft_knn_features(ft_stream s, const ft_tree *tree, const int32_t *leaf_ids, i
 int ft_mlp_forward(ft_stream s, ft_mlp *mdl, const float *features, int n, float *pred) {
     if (!mdl) return fail(FT_EINVAL, "null model");
     if (n <= 0) return FT_OK;
-    const int blocks = std::min((n + 63) / 64, 4096);
-    hipLaunchKernelGGL(mlp_forward_kernel<double>, dim3(blocks), dim3(256), 0, as_stream(s),
+    const int blocks = std::min((n + 63) / 64, 2048);
+    hipLaunchKernelGGL(mlp_infer_kernel<double>, dim3(blocks), dim3(256), 0, as_stream(s),
                        (const float *)mdl->prm, (const float *)mdl->norm, features, n, pred,
                        (double *)nullptr, 0.0, 0.0);
     FT_CHECK_LAUNCH();
@@ -621,15 +682,15 @@ int ft_mlp_seed(ft_stream s, ft_mlp *mdl, const ft_tree *tree, const int32_t *le
     if (nq <= 0 || tree->num_leaves == 0) return FT_OK;
     FT_TRY(ft_knn_features(s, tree, leaf_ids, nq, k, scratch_features));
     const hipStream_t hs = as_stream(s);
-    const int blocks = std::min((nq + 63) / 64, 4096);
+    const int blocks = std::min((nq + 63) / 64, 2048);
     if (tree->coord_type == FT_COORD_F32 || tree->coord_type == FT_COORD_F64) {
         double eps = 0.0;  // the features' eps is relative; seeds need no offset for FP
-        hipLaunchKernelGGL(mlp_forward_kernel<double>, dim3(blocks), dim3(256), 0, hs,
+        hipLaunchKernelGGL(mlp_infer_kernel<double>, dim3(blocks), dim3(256), 0, hs,
                            (const float *)mdl->prm, (const float *)mdl->norm,
                            (const float *)scratch_features, nq, (float *)nullptr, (double *)seed,
                            mdl->margin, eps);
     } else {
-        hipLaunchKernelGGL(mlp_forward_kernel<uint64_t>, dim3(blocks), dim3(256), 0, hs,
+        hipLaunchKernelGGL(mlp_infer_kernel<uint64_t>, dim3(blocks), dim3(256), 0, hs,
                            (const float *)mdl->prm, (const float *)mdl->norm,
                            (const float *)scratch_features, nq, (float *)nullptr,
                            (uint64_t *)seed, mdl->margin, 1.0);

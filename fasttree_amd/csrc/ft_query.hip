@@ -22,111 +22,112 @@
 namespace ft {
 
 int scratch_ensure_public(Scratch &S, size_t n);
+int radix_sort_u64(hipStream_t s, Scratch &S, const uint64_t *keys_in, size_t n,
+                   uint64_t *sorted_out, int key_bits);
 
 
 // ===========================================================================
-// Per-lane DFS with the canonical register top-k (k <= M <= 32).
-// GROUPED: the wave skips a node unless some active lane still needs it
-// (any_of_group, src/hlbvh.hpp:1563-1566).  Results do not depend on it.
+// Query positions.  A "slot" is a position in the launch's query order; the
+// query it holds is order[slot] (external queries sorted along the curve) or
+// the slot itself.  Results are written at the query's own row.  Re-run lists
+// hold slots in ascending order (compacted from per-slot flags), so a listed
+// pass walks its queries in curve order too.
 // ===========================================================================
-template <typename T, int M, bool GROUPED>
-__global__ __launch_bounds__(LANE_BLOCK) void lane_knn_reg_kernel(DTree<T> t, QSrc<T> qs, int nq,
-                                                                  int k, int excl,
-                                                                  uint64_t *__restrict__ res,
-                                                                  typename CoordTraits<T>::D *__restrict__ dists,
-                                                                  uint64_t *__restrict__ stats) {
+struct QueryMap {
+    const uint32_t *order;      // slot -> query (null: identity)
+    const int32_t *list;        // LISTED: slots to run
+    const uint32_t *list_count; // LISTED: number of list entries (device)
+    uint8_t *redo;              // direct passes: per-slot flag, set when the slot
+                                // must be re-run exactly
+};
+
+__device__ inline int slot_query(const QueryMap &qm, int slot) {
+    return qm.order ? (int)qm.order[slot] : slot;
+}
+
+// ===========================================================================
+// Per-lane DFS with the canonical register top-k (k <= M <= 32): the
+// reference's grouped self-kNN / knn_query_small_k walk on wave64 lanes.
+// Direct passes use the reference's 64-entry stack and flag a query whose
+// walk would need more; LISTED passes (the exact re-runs) use 128 entries.
+// ===========================================================================
+template <typename T, int M, bool GROUPED, bool LISTED, int MK>
+__global__ __launch_bounds__(LANE_BLOCK) void lane_knn_reg_kernel(
+    DTree<T> t, QSrc<T> qs, int nq, int k, int excl, QueryMap qm, uint64_t *__restrict__ res,
+    typename CoordTraits<T>::D *__restrict__ dists, uint64_t *__restrict__ stats) {
     using D = typename CoordTraits<T>::D;
-    __shared__ int32_t stk[LANE_STACK_CANON][LANE_BLOCK];
+    constexpr int DEPTH = LISTED ? LANE_STACK_CANON : STACK_DEPTH;
+    __shared__ int32_t stk[DEPTH * LANE_BLOCK];
     const int lane = threadIdx.x;
-    const int q = blockIdx.x * LANE_BLOCK + lane;
-    const bool active = q < nq;
-    const int n = t.n;
-    T px = T(0), py = T(0), pz = T(0);
-    int self = -1;
-    if (active) load_query(t, qs, q, px, py, pz, self);
-    TopK<D, M> h;
-    h.init(k, dist_max<D>());
-    uint64_t visits = 0, drops = 0, inserts = 0;
-    int sp = 0;
-    if (active) stk[sp++][lane] = 0;
-    while (sp > 0) {
-        const int node = stk[--sp][lane];
-        const D d2 = node_d2(t, px, py, pz, node);
-        ++visits;
-        const bool pruned = d2 > h.worst();
-        if (GROUPED) {
-            if (!__any(!pruned)) continue;
+    const int nlist = LISTED ? (int)*qm.list_count : nq;
+    uint64_t visits = 0, inserts = 0;
+    for (int base = blockIdx.x * LANE_BLOCK; base < nlist;
+         base += (LISTED ? gridDim.x * LANE_BLOCK : nlist)) {
+        const int pos = base + lane;
+        const bool active = pos < nlist;
+        const int slot = active ? (LISTED ? qm.list[pos] : pos) : 0;
+        const int q = active ? slot_query(qm, slot) : 0;
+        T px = T(0), py = T(0), pz = T(0);
+        int self = -1;
+        if (active) load_query(t, qs, q, px, py, pz, self);
+        if (!excl) self = -1;
+        TopK<D, M> h;
+        h.init(k, dist_max<D>());
+        const bool ovf = lane_reg_walk<T, M, GROUPED, DEPTH, MK>(t, active, px, py, pz, self, h,
+                                                                 stk, LANE_BLOCK, lane, visits,
+                                                                 inserts);
+        if (active) {
+            if (!LISTED && ovf) qm.redo[slot] = 1;
+            else h.store(k, res, dists, (size_t)q);
         }
-        if (pruned) continue;
-        if (node >= n - 1 || n == 1) {
-            const int leaf = (n == 1) ? 0 : node;
-            const int rank = leaf - (n - 1);
-            if (excl && rank == self) continue;
-            const int32_t ri = ret_index(t, leaf);
-            const bool take = h.beats_worst(d2, ri);
-            h.insert(take, d2, ri);
-            inserts += take ? 1 : 0;
-        } else {
-            const int l = t.left[node], r = t.right[node];
-            const D ld = node_d2(t, px, py, pz, l);
-            const D rd = node_d2(t, px, py, pz, r);
-            if (sp < LANE_STACK_CANON - 2) {
-                if (ld <= rd) {
-                    stk[sp++][lane] = r;
-                    stk[sp++][lane] = l;
-                } else {
-                    stk[sp++][lane] = l;
-                    stk[sp++][lane] = r;
-                }
-            } else {
-                ++drops;
-            }
-        }
+        if (!LISTED) break;
     }
-    if (active) h.store(k, res, dists, (size_t)q);
     wave_add_stats(stats, 0, visits);
-    wave_add_stats(stats, 2, drops);
     wave_add_stats(stats, 4, inserts);
 }
 
 // Per-lane DFS with an LDS heap row per lane (LocalMaxHeap, src/maxheap.hpp:340-478):
-// REF = the reference's mechanics (ties as the reference picks them), else
-// canonical.  LISTED: the block's lanes take entries of a device list of query
-// positions (grid-stride; the list length is read on the device).
-template <typename T, bool REF, bool GROUPED, bool LISTED>
+// REF = the reference's mechanics (ties as the reference picks them, 64-entry
+// stack with its silent drop), else canonical (direct passes flag overflowing
+// walks; LISTED re-runs use 128 entries).
+template <typename T, bool REF, bool GROUPED, bool LISTED, int MK>
 __global__ __launch_bounds__(LANE_BLOCK) void lane_knn_lds_kernel(
-    DTree<T> t, QSrc<T> qs, int nq, int k, int excl, int lanes, const int32_t *__restrict__ list,
-    const uint32_t *__restrict__ list_count, uint64_t *__restrict__ res,
-    typename CoordTraits<T>::D *__restrict__ dists, uint64_t *__restrict__ stats) {
+    DTree<T> t, QSrc<T> qs, int nq, int k, int excl, int lanes, QueryMap qm,
+    uint64_t *__restrict__ res, typename CoordTraits<T>::D *__restrict__ dists,
+    uint64_t *__restrict__ stats) {
     using D = typename CoordTraits<T>::D;
-    constexpr int DEPTH = REF ? STACK_DEPTH : LANE_STACK_CANON;
+    constexpr int DEPTH = (LISTED && !REF) ? LANE_STACK_CANON : STACK_DEPTH;
     extern __shared__ __align__(16) unsigned char lds[];
     D *hd = (D *)lds;                                   // [k][lanes]
     int32_t *hi = (int32_t *)(hd + (size_t)k * lanes);  // [k][lanes]
     int32_t *stk = hi + (size_t)k * lanes;              // [DEPTH][lanes]
-    (void)DEPTH;
     const int lane = threadIdx.x;
     const bool lane_ok = lane < lanes;
-    const int nlist = LISTED ? (int)*list_count : nq;
+    const int nlist = LISTED ? (int)*qm.list_count : nq;
     uint64_t visits = 0, drops = 0, inserts = 0;
     for (int base = blockIdx.x * lanes; base < nlist; base += (LISTED ? gridDim.x * lanes : nlist)) {
-        const int slot = base + lane;
-        const bool active = lane_ok && slot < nlist;
-        const int q = active ? (LISTED ? list[slot] : slot) : -1;
+        const int pos = base + lane;
+        const bool active = lane_ok && pos < nlist;
+        const int slot = active ? (LISTED ? qm.list[pos] : pos) : 0;
+        const int q = active ? slot_query(qm, slot) : 0;
         T px = T(0), py = T(0), pz = T(0);
         int self = -1;
         if (active) load_query(t, qs, q, px, py, pz, self);
         if (!excl) self = -1;
         LdsHeap<D, REF> h{hd, hi, lanes, lane, 0};
-        lane_heap_walk<T, REF, GROUPED>(t, k, active, px, py, pz, self, h, stk, lanes, lane,
-                                        visits, drops, inserts);
+        const bool ovf = lane_heap_walk<T, REF, GROUPED, DEPTH, MK>(
+            t, k, active, px, py, pz, self, h, stk, lanes, lane, visits, drops, inserts);
         if (active) {
-            h.sort();
-            const size_t o = (size_t)q * k;
-            for (int s = 0; s < k; ++s) {
-                const bool ok = s < h.count && h.ii(s) >= 0;
-                res[o + s] = ok ? (uint64_t)(uint32_t)h.ii(s) : ~0ull;
-                dists[o + s] = ok ? h.dd(s) : missing_dist<D>();
+            if (!REF && !LISTED && ovf) {
+                qm.redo[slot] = 1;
+            } else {
+                h.sort();
+                const size_t o = (size_t)q * k;
+                for (int s = 0; s < k; ++s) {
+                    const bool ok = s < h.count && h.ii(s) >= 0;
+                    res[o + s] = ok ? (uint64_t)(uint32_t)h.ii(s) : ~0ull;
+                    dists[o + s] = ok ? h.dd(s) : missing_dist<D>();
+                }
             }
         }
         if (!LISTED) break;
@@ -264,34 +265,25 @@ __global__ __launch_bounds__(64) void merge_knn_kernel(DTree<T> t, QSrc<T> qs, i
 
 
 // ===========================================================================
-// Packet traversal: a wavefront of 64 queries shares one DFS.  A node is
-// expanded when some lane's box distance does not exceed that lane's current
-// k-th distance; leaf children are tested at once; a subtree with at most
-// `bucket` leaves is scanned as a contiguous run of leaf slots.  Node data is
-// wave-uniform, the stack lives in LDS, results are canonical.
-//
-// Query mapping: direct (lane -> query blockIdx*64+lane) or LISTED (each wave
-// takes 64 entries of a device list of query positions; the list length is
-// read on the device, so no host round trip is needed).
+// Packet traversal: a wavefront of 64 queries (consecutive slots) shares one
+// DFS (packet_walk).  Results are canonical.
 // SEEDED: each lane's k-th distance starts at its seed bound; a lane that ends
-// with fewer than k neighbours inside the seed is appended to the re-run list.
-// PACKED: 64-bit (d2, index) keys; queries the packed form cannot order
-// exactly are appended to the re-run list too.  The re-run is LISTED,
-// unseeded and lexicographic, so its results are exact.
+// with fewer than k neighbours inside the seed flags its slot for an exact
+// re-run.  PACKED: 64-bit (d2, index) keys; queries the packed form cannot
+// order exactly are flagged too.  The re-run is the per-lane walk over the
+// compacted list of flagged slots, unseeded and lexicographic, so exact.
 // ===========================================================================
 struct PacketArgs {
-    int nq, k, excl, bucket, ibits, pad;
-    const int32_t *list;        // LISTED: query positions
-    const uint32_t *list_count; // LISTED: number of entries
-    int32_t *rerun;             // re-run list (SEEDED or PACKED)
-    uint32_t *rerun_count;
+    int nq, k, excl, bucket, ibits;
+    int window;  // self queries over all leaves: leaves pre-scanned on each side
     uint64_t *stats;
 };
 
-template <typename T, int M, bool HEAP, bool SEEDED, bool PACKED, bool LISTED, int MK>
+template <typename T, int M, bool HEAP, bool SEEDED, bool PACKED, int MK>
 __global__ __launch_bounds__(64) void packet_knn_kernel(
-    DTree<T> t, QSrc<T> qs, PacketArgs a, const typename CoordTraits<T>::D *__restrict__ seed,
-    uint64_t *__restrict__ res, typename CoordTraits<T>::D *__restrict__ dists) {
+    DTree<T> t, QSrc<T> qs, PacketArgs a, QueryMap qm,
+    const typename CoordTraits<T>::D *__restrict__ seed, uint64_t *__restrict__ res,
+    typename CoordTraits<T>::D *__restrict__ dists) {
     using D = typename CoordTraits<T>::D;
     using H = typename std::conditional<HEAP, TopKHeapLds<M>,
                                         typename TopKSel<D, M, PACKED>::type>::type;
@@ -305,34 +297,33 @@ __global__ __launch_bounds__(64) void packet_knn_kernel(
     }
     const int lane = threadIdx.x;
     const int k = a.k;
-    const int nlist = LISTED ? (int)*a.list_count : a.nq;
     PacketCounters c;
-
-    for (int base = blockIdx.x * 64; base < nlist; base += (LISTED ? gridDim.x * 64 : nlist)) {
-        const int slot = base + lane;
-        const bool active = slot < nlist;
-        const int q = active ? (LISTED ? a.list[slot] : slot) : -1;
-        T px = T(0), py = T(0), pz = T(0);
-        int self = -1;
-        if (active) load_query(t, qs, q, px, py, pz, self);
-        if (!a.excl) self = -1;
-        H h;
-        if constexpr (HEAP) h.key = heap_mem;
-        if constexpr (PACKED) h.init(k, (SEEDED && active) ? (uint64_t)seed[q] : ~0ull, a.ibits);
-        else h.init(k, (SEEDED && active) ? seed[q] : dist_max<D>());
-        packet_walk<T, H, MK>(t, a.bucket, active, px, py, pz, self, h, stk, c);
-        if (active) {
-            bool redo = false;
-            if constexpr (PACKED) redo = h.needs_exact(k);
-            else if constexpr (SEEDED) redo = h.missing(k) > 0;
-            if (redo) {
-                const uint32_t s2 = atomicAdd(a.rerun_count, 1u);
-                a.rerun[s2] = q;
-            } else {
-                h.store(k, res, dists, (size_t)q);
-            }
-        }
-        if (!LISTED) break;
+    const int slot = blockIdx.x * 64 + lane;
+    const bool active = slot < a.nq;
+    const int q = active ? slot_query(qm, slot) : 0;
+    T px = T(0), py = T(0), pz = T(0);
+    int self = -1;
+    if (active) load_query(t, qs, q, px, py, pz, self);
+    if (!a.excl) self = -1;
+    H h;
+    if constexpr (HEAP) h.key = heap_mem;
+    if constexpr (PACKED) h.init(k, (SEEDED && active) ? (uint64_t)seed[q] : ~0ull, a.ibits);
+    else h.init(k, (SEEDED && active) ? seed[q] : dist_max<D>());
+    // Self queries over all leaves: the packet is leaf ranks [r0, r0 + 64), so
+    // its neighbours along the curve are the leaves on either side.
+    int wlo = 0, whi = 0;
+    if (a.window > 0) {
+        const int r0 = blockIdx.x * 64;
+        wlo = r0 - a.window > 0 ? r0 - a.window : 0;
+        whi = r0 + 64 + a.window < t.n ? r0 + 64 + a.window : t.n;
+    }
+    packet_walk<T, H, MK>(t, a.bucket, active, px, py, pz, self, h, stk, c, wlo, whi);
+    if (active) {
+        bool redo = false;
+        if constexpr (PACKED) redo = h.needs_exact(k);
+        else if constexpr (SEEDED) redo = h.missing(k) > 0;
+        if (redo) qm.redo[slot] = 1;
+        else h.store(k, res, dists, (size_t)q);
     }
     if (lane == 0 && a.stats) {
         atomicAdd((unsigned long long *)&a.stats[0], (unsigned long long)c.nodes);
@@ -340,6 +331,83 @@ __global__ __launch_bounds__(64) void packet_knn_kernel(
         if (c.overflow) atomicAdd((unsigned long long *)&a.stats[2], (unsigned long long)c.overflow);
     }
     wave_add_stats(a.stats, 4, c.inserts);
+}
+
+// ===========================================================================
+// Order-preserving compaction of per-slot flags into a slot list (ascending),
+// with the count left on the device: per-tile counts, one scan block, scatter.
+// ===========================================================================
+constexpr int CP_TILE = 4096;  // flags per block (256 threads x 16)
+
+__global__ __launch_bounds__(256) void flag_count_kernel(const uint8_t *__restrict__ flags, int n,
+                                                         uint32_t *__restrict__ tile_counts) {
+    __shared__ uint32_t s_w[4];
+    const int base = blockIdx.x * CP_TILE;
+    uint32_t c = 0;
+    for (int i = threadIdx.x; i < CP_TILE; i += 256) {
+        const int j = base + i;
+        c += (j < n && flags[j]) ? 1u : 0u;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+    if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) tile_counts[blockIdx.x] = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+}
+
+// Exclusive scan of the tile counts in place; total to *count.
+__global__ __launch_bounds__(1024) void flag_scan_kernel(uint32_t *__restrict__ tile_counts,
+                                                         int nt, uint32_t *__restrict__ count) {
+    __shared__ uint32_t s_w[16];
+    __shared__ uint32_t s_carry;
+    if (threadIdx.x == 0) s_carry = 0;
+    __syncthreads();
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int base = 0; base < nt; base += 1024) {
+        const int i = base + threadIdx.x;
+        const uint32_t v = i < nt ? tile_counts[i] : 0u;
+        uint32_t x = v;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(x, o, 64);
+            if (lane >= o) x += y;
+        }
+        if (lane == 63) s_w[w] = x;
+        __syncthreads();
+        uint32_t add = s_carry;
+        for (int q = 0; q < w; ++q) add += s_w[q];
+        if (i < nt) tile_counts[i] = add + x - v;
+        __syncthreads();
+        if (threadIdx.x == 1023) s_carry = add + x;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *count = s_carry;
+}
+
+__global__ __launch_bounds__(256) void flag_scatter_kernel(const uint8_t *__restrict__ flags,
+                                                           int n,
+                                                           const uint32_t *__restrict__ tile_base,
+                                                           int32_t *__restrict__ list) {
+    __shared__ uint32_t s_w[4];
+    __shared__ uint32_t s_run;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (threadIdx.x == 0) s_run = tile_base[blockIdx.x];
+    __syncthreads();
+    const int base = blockIdx.x * CP_TILE;
+    for (int r = 0; r < CP_TILE; r += 256) {
+        const int j = base + r + threadIdx.x;
+        const bool f = j < n && flags[j];
+        const uint64_t m = __ballot(f);
+        const uint32_t below = (uint32_t)__popcll(m & (lane ? (~0ull >> (64 - lane)) : 0ull));
+        if (lane == 0) s_w[w] = (uint32_t)__popcll(m);
+        __syncthreads();
+        uint32_t off = s_run;
+        for (int q = 0; q < w; ++q) off += s_w[q];
+        if (f) list[off + below] = j;
+        __syncthreads();
+        if (threadIdx.x == 0) s_run += s_w[0] + s_w[1] + s_w[2] + s_w[3];
+        __syncthreads();
+    }
 }
 
 // ===========================================================================
@@ -456,6 +524,40 @@ __global__ __launch_bounds__(256) void scan_add_kernel(int64_t *out, int nq,
 }
 
 // ===========================================================================
+// Query order for packet launches: external points by their Morton code (21
+// bits per axis) inside the tree's root box, subset leaf ranks by rank, so a
+// wavefront's 64 queries are neighbours whatever order the caller used.
+// ===========================================================================
+template <typename T>
+__global__ __launch_bounds__(256) void query_key_kernel(DTree<T> t, QSrc<T> qs, int nq,
+                                                        uint64_t *__restrict__ keys) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nq) return;
+    if (!qs.qx) {
+        keys[i] = (uint64_t)(uint32_t)qs.leaf_ids[i];
+        return;
+    }
+    constexpr int QB = 21;
+    const double side = (double)((1u << QB) - 1);
+    auto quant = [&](T v, T lo, T hi) -> uint64_t {
+        const double ext = (double)hi - (double)lo;
+        double u = ext > 0.0 ? ((double)v - (double)lo) / ext : 0.0;
+        u = u < 0.0 ? 0.0 : (u > 1.0 ? 1.0 : u);
+        return (uint64_t)(u * side);
+    };
+    const uint64_t ix = quant(qs.qx[i], t.mnx[0], t.mxx[0]);
+    const uint64_t iy = quant(qs.qy[i], t.mny[0], t.mxy[0]);
+    const uint64_t iz = quant(qs.qz[i], t.mnz[0], t.mxz[0]);
+    uint64_t m = 0;
+    for (int b = 0; b < QB; ++b) {
+        m |= ((ix >> b) & 1ull) << (3 * b + 2);
+        m |= ((iy >> b) & 1ull) << (3 * b + 1);
+        m |= ((iz >> b) & 1ull) << (3 * b);
+    }
+    keys[i] = m;
+}
+
+// ===========================================================================
 // Host dispatch
 // ===========================================================================
 static inline unsigned blocks_for(size_t n, int per) {
@@ -463,27 +565,11 @@ static inline unsigned blocks_for(size_t n, int per) {
     return (unsigned)(g ? g : 1);
 }
 
-template <typename T, bool GROUPED>
-static int launch_lane_reg(hipStream_t s, const DTree<T> &dt, const QSrc<T> &qs, int nq, int k,
-                           int excl, uint64_t *res, typename CoordTraits<T>::D *d,
-                           uint64_t *stats) {
-    const dim3 g(blocks_for(nq, LANE_BLOCK)), b(LANE_BLOCK);
-#define FT_LANE_REG(MK) hipLaunchKernelGGL((lane_knn_reg_kernel<T, MK, GROUPED>), g, b, 0, s, dt, qs, nq, k, excl, res, d, stats)
-    if (k <= 1) FT_LANE_REG(1);
-    else if (k <= 4) FT_LANE_REG(4);
-    else if (k <= 8) FT_LANE_REG(8);
-    else if (k <= 16) FT_LANE_REG(16);
-    else FT_LANE_REG(32);
-#undef FT_LANE_REG
-    FT_CHECK_LAUNCH();
-    return FT_OK;
-}
-
-// Lanes per block for an LDS heap of k entries: as many as fit 64 KiB.
-template <typename T, bool REF> static int lds_lanes(int k, size_t &lds) {
+// Lanes per block for an LDS heap of k entries with a DEPTH-entry stack: as
+// many as fit 64 KiB.
+template <typename T> static int lds_lanes(int k, int depth, size_t &lds) {
     using D = typename CoordTraits<T>::D;
-    const size_t per_lane =
-        (size_t)k * (sizeof(D) + 4) + (size_t)(REF ? STACK_DEPTH : LANE_STACK_CANON) * 4;
+    const size_t per_lane = (size_t)k * (sizeof(D) + 4) + (size_t)depth * 4;
     int lanes = 64;
     const size_t budget = 64 * 1024;
     while (lanes > 1 && (size_t)lanes * per_lane > budget) lanes >>= 1;
@@ -491,32 +577,15 @@ template <typename T, bool REF> static int lds_lanes(int k, size_t &lds) {
     return lds <= budget ? lanes : 0;
 }
 
-template <typename T, bool REF, bool GROUPED>
-static int launch_lane_lds(hipStream_t s, const DTree<T> &dt, const QSrc<T> &qs, int nq, int k,
-                           int excl, uint64_t *res, typename CoordTraits<T>::D *d,
-                           uint64_t *stats, const int32_t *list = nullptr,
-                           const uint32_t *list_count = nullptr) {
-    size_t lds = 0;
-    const int lanes = lds_lanes<T, REF>(k, lds);
-    if (!lanes) return fail(FT_ELDS, "k too large for local memory: reduce k");
-    if (list) {
-        const unsigned g = std::min<unsigned>(blocks_for(nq, lanes), 4096u);
-        hipLaunchKernelGGL((lane_knn_lds_kernel<T, REF, GROUPED, true>), dim3(g), dim3(LANE_BLOCK),
-                           lds, s, dt, qs, nq, k, excl, lanes, list, list_count, res, d, stats);
-    } else {
-        hipLaunchKernelGGL((lane_knn_lds_kernel<T, REF, GROUPED, false>),
-                           dim3(blocks_for(nq, lanes)), dim3(LANE_BLOCK), lds, s, dt, qs, nq, k,
-                           excl, lanes, (const int32_t *)nullptr, (const uint32_t *)nullptr, res,
-                           d, stats);
-    }
-    FT_CHECK_LAUNCH();
-    return FT_OK;
-}
-
-// Device buffers for the two re-run lists, grown on demand (one per process).
+// Re-run buffers: per-slot flags, the compacted slot list, per-tile counts
+// and the list length.  Grown on demand, one set per process (calls on one
+// device are ordered on their streams; concurrent streams need separate
+// processes or the batch API).
 struct RerunBuf {
-    int32_t *list[2] = {nullptr, nullptr};
-    uint32_t *count = nullptr;  // [0], [1]: list lengths
+    uint8_t *flags = nullptr;
+    int32_t *list = nullptr;
+    uint32_t *tiles = nullptr;
+    uint32_t *count = nullptr;
     size_t cap = 0;
     void *block = nullptr;
 };
@@ -524,23 +593,32 @@ static RerunBuf g_rerun;
 
 static int rerun_ensure(size_t n) {
     if (n <= g_rerun.cap && g_rerun.block) return FT_OK;
-    if (g_rerun.block) (void)hipFree(g_rerun.block);
+    // growing: earlier launches on any stream may still read the old buffers
+    if (g_rerun.block) {
+        FT_HIP(hipDeviceSynchronize());
+        (void)hipFree(g_rerun.block);
+    }
     g_rerun.block = nullptr;
     const size_t cap = n + n / 2 + 1024;
+    const size_t tiles = cap / CP_TILE + 2;
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t bytes = al(cap) + al(cap * 4) + al(tiles * 4) + 256;
     void *p = nullptr;
-    if (hipMalloc(&p, 2 * cap * 4 + 256) != hipSuccess)
-        return fail(FT_ENOMEM, "re-run list allocation failed");
+    if (hipMalloc(&p, bytes) != hipSuccess) return fail(FT_ENOMEM, "re-run buffer allocation failed");
+    char *c = (char *)p;
     g_rerun.block = p;
-    g_rerun.list[0] = (int32_t *)p;
-    g_rerun.list[1] = (int32_t *)p + cap;
-    g_rerun.count = (uint32_t *)((char *)p + 2 * cap * 4);
+    g_rerun.flags = (uint8_t *)c;
+    g_rerun.list = (int32_t *)(c + al(cap));
+    g_rerun.tiles = (uint32_t *)(c + al(cap) + al(cap * 4));
+    g_rerun.count = (uint32_t *)(c + al(cap) + al(cap * 4) + al(tiles * 4));
     g_rerun.cap = cap;
     return FT_OK;
 }
 
-__global__ void zero_counts_kernel(uint32_t *p) {
-    p[0] = 0;
-    p[1] = 0;
+__global__ void fill_u8_kernel(uint8_t *p, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        p[i] = 0;
 }
 __global__ void add_count_kernel(uint64_t *stats, const uint32_t *count) { stats[3] += *count; }
 
@@ -553,115 +631,166 @@ static inline int bit_length(uint64_t v) {
     return b;
 }
 
-template <typename T, int M, bool HEAP, bool SEEDED, bool PACKED, bool LISTED>
-static void launch_pk(hipStream_t s, dim3 g, const DTree<T> &dt, const QSrc<T> &qs,
-                      const PacketArgs &a, const typename CoordTraits<T>::D *seed, uint64_t *res,
-                      typename CoordTraits<T>::D *d) {
-    switch (metric_kind<T>(dt.m)) {
-    case MK_WRAP:
-        hipLaunchKernelGGL((packet_knn_kernel<T, M, HEAP, SEEDED, PACKED, LISTED, MK_WRAP>), g,
-                           dim3(64), 0, s, dt, qs, a, seed, res, d);
-        break;
-    case MK_OPEN:
-        hipLaunchKernelGGL((packet_knn_kernel<T, M, HEAP, SEEDED, PACKED, LISTED, MK_OPEN>), g,
-                           dim3(64), 0, s, dt, qs, a, seed, res, d);
-        break;
-    default:
-        hipLaunchKernelGGL((packet_knn_kernel<T, M, HEAP, SEEDED, PACKED, LISTED, MK_ANY>), g,
-                           dim3(64), 0, s, dt, qs, a, seed, res, d);
-        break;
-    }
+// Flags -> ascending slot list, count on the device.
+static int compact_flags(hipStream_t s, int n) {
+    const unsigned nt = blocks_for((size_t)n, CP_TILE);
+    hipLaunchKernelGGL(flag_count_kernel, dim3(nt), dim3(256), 0, s,
+                       (const uint8_t *)g_rerun.flags, n, g_rerun.tiles);
+    hipLaunchKernelGGL(flag_scan_kernel, dim3(1), dim3(1024), 0, s, g_rerun.tiles, (int)nt,
+                       g_rerun.count);
+    hipLaunchKernelGGL(flag_scatter_kernel, dim3(nt), dim3(256), 0, s,
+                       (const uint8_t *)g_rerun.flags, n, (const uint32_t *)g_rerun.tiles,
+                       g_rerun.list);
+    FT_CHECK_LAUNCH();
+    return FT_OK;
 }
 
-// Runs the packet traversal and its exact re-runs:
-//   pass 1  direct, seeded or not, packed keys for integer metrics;
-//   pass 2  (packed only) unseeded packed run of the queries pass 1 listed;
-//   pass 3  unseeded lexicographic run of what is still listed -- exact.
-// The LISTED passes read list lengths on the device: no host round trip.
-template <typename T, int M, bool HEAP>
-static int packet_run(hipStream_t s, const DTree<T> &dt, const QSrc<T> &qs, int nq, int k,
-                      int excl, int bucket, const typename CoordTraits<T>::D *seed, uint64_t *res,
-                      typename CoordTraits<T>::D *d, uint64_t *stats) {
-    constexpr bool INTD = !CoordTraits<T>::fp;
-    PacketArgs a{};
-    a.nq = nq;
-    a.k = k;
-    a.excl = excl;
-    a.bucket = bucket;
-    a.ibits = bit_length((uint64_t)dt.n);
-    a.stats = stats;
-    const bool need_list = seed != nullptr || INTD;
-    if (need_list) {
-        FT_TRY(rerun_ensure((size_t)nq));
-        hipLaunchKernelGGL(zero_counts_kernel, dim3(1), dim3(1), 0, s, g_rerun.count);
-        a.rerun = g_rerun.list[0];
-        a.rerun_count = g_rerun.count;
-    }
-    const dim3 g(blocks_for(nq, 64));
-    if (seed) {
-        if constexpr (INTD) launch_pk<T, M, HEAP, true, true, false>(s, g, dt, qs, a, seed, res, d);
-        else launch_pk<T, M, false, true, false, false>(s, g, dt, qs, a, seed, res, d);
+// Runs the statement with M = the register tier width for k (1, 4, 8, 16, 32).
+#define FT_BY_M(K, ...)                                        \
+    do {                                                       \
+        if ((K) <= 1) { constexpr int M = 1; __VA_ARGS__; }         \
+        else if ((K) <= 4) { constexpr int M = 4; __VA_ARGS__; }    \
+        else if ((K) <= 8) { constexpr int M = 8; __VA_ARGS__; }    \
+        else if ((K) <= 16) { constexpr int M = 16; __VA_ARGS__; }  \
+        else { constexpr int M = 32; __VA_ARGS__; }                 \
+    } while (0)
+
+// Exact re-run of the listed slots: per-lane walk with a 128-entry stack,
+// unseeded, lexicographic top-k (registers for k <= 32, LDS heap above).
+template <typename T, int MK>
+static int launch_exact_listed(hipStream_t s, const DTree<T> &dt, const QSrc<T> &qs, int nq,
+                               int k, int excl, const uint32_t *order, uint64_t *res,
+                               typename CoordTraits<T>::D *d) {
+    const QueryMap qm{order, g_rerun.list, g_rerun.count, nullptr};
+    if (k <= 32) {
+        const unsigned g = std::min<unsigned>(blocks_for(nq, LANE_BLOCK), 2048u);
+        FT_BY_M(k, hipLaunchKernelGGL((lane_knn_reg_kernel<T, M, true, true, MK>), dim3(g),
+                                      dim3(LANE_BLOCK), 0, s, dt, qs, nq, k, excl, qm, res, d,
+                                      (uint64_t *)nullptr));
     } else {
-        if constexpr (INTD) launch_pk<T, M, HEAP, false, true, false>(s, g, dt, qs, a, seed, res, d);
-        else launch_pk<T, M, false, false, false, false>(s, g, dt, qs, a, seed, res, d);
-    }
-    FT_CHECK_LAUNCH();
-    if (!need_list) return FT_OK;
-    if (stats) {
-        hipLaunchKernelGGL(add_count_kernel, dim3(1), dim3(1), 0, s, stats,
-                           (const uint32_t *)g_rerun.count);
-        FT_CHECK_LAUNCH();
-    }
-    const unsigned waves = std::min<unsigned>(blocks_for(nq, 64), 2048u);
-    PacketArgs b = a;
-    b.stats = nullptr;
-    int lex_list = 0;
-    if constexpr (INTD) {
-        if (seed) {
-            b.list = g_rerun.list[0];
-            b.list_count = g_rerun.count;
-            b.rerun = g_rerun.list[1];
-            b.rerun_count = g_rerun.count + 1;
-            launch_pk<T, M, HEAP, false, true, true>(s, dim3(waves), dt, qs, b, nullptr, res, d);
-            FT_CHECK_LAUNCH();
-            lex_list = 1;
-        }
-    }
-    b.list = g_rerun.list[lex_list];
-    b.list_count = g_rerun.count + lex_list;
-    b.rerun = nullptr;
-    b.rerun_count = nullptr;
-    if constexpr (HEAP) {
-        // exact: per-lane lexicographic LDS heap over the listed queries
-        return launch_lane_lds<T, false, true>(s, dt, qs, nq, k, excl, res, d, nullptr, b.list,
-                                               b.list_count);
-    } else {
-        launch_pk<T, M, false, false, false, true>(s, dim3(waves), dt, qs, b, nullptr, res, d);
+        size_t lds = 0;
+        const int lanes = lds_lanes<T>(k, LANE_STACK_CANON, lds);
+        if (!lanes) return fail(FT_ELDS, "k too large for local memory: reduce k");
+        const unsigned g = std::min<unsigned>(blocks_for(nq, lanes), 4096u);
+        hipLaunchKernelGGL((lane_knn_lds_kernel<T, false, true, true, MK>), dim3(g),
+                           dim3(LANE_BLOCK), lds, s, dt, qs, nq, k, excl, lanes, qm, res, d,
+                           (uint64_t *)nullptr);
     }
     FT_CHECK_LAUNCH();
     return FT_OK;
 }
 
-template <typename T>
-static int launch_packet(hipStream_t s, const DTree<T> &dt, const QSrc<T> &qs, int nq, int k,
-                         int excl, int bucket, const typename CoordTraits<T>::D *seed,
-                         uint64_t *res, typename CoordTraits<T>::D *d, uint64_t *stats,
-                         int topk) {
+template <typename T, int M, bool HEAP, bool SEEDED, bool PACKED, int MK>
+static void launch_pk(hipStream_t s, const DTree<T> &dt, const QSrc<T> &qs, const PacketArgs &a,
+                      const QueryMap &qm, const typename CoordTraits<T>::D *seed, uint64_t *res,
+                      typename CoordTraits<T>::D *d) {
+    hipLaunchKernelGGL((packet_knn_kernel<T, M, HEAP, SEEDED, PACKED, MK>),
+                       dim3(blocks_for(a.nq, 64)), dim3(64), 0, s, dt, qs, a, qm, seed, res, d);
+}
+
+template <typename T, int M, bool HEAP, int MK>
+static void launch_packet_m(hipStream_t s, const DTree<T> &dt, const QSrc<T> &qs,
+                            const PacketArgs &a, const QueryMap &qm,
+                            const typename CoordTraits<T>::D *seed, uint64_t *res,
+                            typename CoordTraits<T>::D *d) {
     constexpr bool INTD = !CoordTraits<T>::fp;
-    const bool heap = INTD && (k > 32 || topk == FT_TOPK_LDS_HEAP);
-    if (!heap) {
-        if (k <= 1) return packet_run<T, 1, false>(s, dt, qs, nq, k, excl, bucket, seed, res, d, stats);
-        if (k <= 4) return packet_run<T, 4, false>(s, dt, qs, nq, k, excl, bucket, seed, res, d, stats);
-        if (k <= 8) return packet_run<T, 8, false>(s, dt, qs, nq, k, excl, bucket, seed, res, d, stats);
-        if (k <= 16) return packet_run<T, 16, false>(s, dt, qs, nq, k, excl, bucket, seed, res, d, stats);
-        if (k <= 32) return packet_run<T, 32, false>(s, dt, qs, nq, k, excl, bucket, seed, res, d, stats);
+    if (seed) launch_pk<T, M, HEAP, true, INTD, MK>(s, dt, qs, a, qm, seed, res, d);
+    else launch_pk<T, M, HEAP, false, INTD, MK>(s, dt, qs, a, qm, seed, res, d);
+}
+
+// One canonical query launch: a direct pass (packet or per-lane walk) that
+// flags the slots it cannot answer exactly (packed keys that saturate, seeds
+// that were too small, per-lane stacks that would overflow), compaction of
+// the flags, and the exact listed re-run.  No host round trip.
+template <typename T, int MK>
+static int run_canonical(hipStream_t s, const DTree<T> &dt, const QSrc<T> &qs, int nq, int k,
+                         int excl, int algo, int bucket, int topk, int window,
+                         const uint32_t *order,
+                         const typename CoordTraits<T>::D *seed, uint64_t *res,
+                         typename CoordTraits<T>::D *d, uint64_t *stats) {
+    constexpr bool INTD = !CoordTraits<T>::fp;
+    FT_TRY(rerun_ensure((size_t)nq));
+    hipLaunchKernelGGL(fill_u8_kernel, dim3(std::min<unsigned>(blocks_for(nq, 256), 4096u)),
+                       dim3(256), 0, s, g_rerun.flags, (size_t)nq);
+    const QueryMap qm{order, nullptr, nullptr, g_rerun.flags};
+    if (algo == FT_ALGO_PACKET) {
+        PacketArgs a{};
+        a.nq = nq;
+        a.k = k;
+        a.excl = excl;
+        a.bucket = bucket;
+        a.ibits = bit_length((uint64_t)dt.n);
+        a.window = (!qs.qx && !qs.leaf_ids && !order) ? window : 0;
+        a.stats = stats;
+        const bool heap = INTD && (k > 32 || topk == FT_TOPK_LDS_HEAP);
+        if (!heap) {
+            FT_BY_M(k, (launch_packet_m<T, M, false, MK>(s, dt, qs, a, qm, seed, res, d)));
+        } else if constexpr (INTD) {
+            if (k <= 32) launch_packet_m<T, 32, true, MK>(s, dt, qs, a, qm, seed, res, d);
+            else if (k <= 64) launch_packet_m<T, 64, true, MK>(s, dt, qs, a, qm, seed, res, d);
+            else if (k <= 128) launch_packet_m<T, 128, true, MK>(s, dt, qs, a, qm, seed, res, d);
+            else return fail(FT_EINVAL, "packet traversal: k above the heap tiers");
+        }
+    } else if (k <= 32) {
+        const dim3 g(blocks_for(nq, LANE_BLOCK)), b(LANE_BLOCK);
+        if (algo == FT_ALGO_PLAIN)
+            FT_BY_M(k, hipLaunchKernelGGL((lane_knn_reg_kernel<T, M, false, false, MK>), g, b, 0, s,
+                                          dt, qs, nq, k, excl, qm, res, d, stats));
+        else
+            FT_BY_M(k, hipLaunchKernelGGL((lane_knn_reg_kernel<T, M, true, false, MK>), g, b, 0, s,
+                                          dt, qs, nq, k, excl, qm, res, d, stats));
+    } else {
+        size_t lds = 0;
+        const int lanes = lds_lanes<T>(k, STACK_DEPTH, lds);
+        if (!lanes) return fail(FT_ELDS, "k too large for local memory: reduce k");
+        const dim3 g(blocks_for(nq, lanes)), b(LANE_BLOCK);
+        if (algo == FT_ALGO_PLAIN)
+            hipLaunchKernelGGL((lane_knn_lds_kernel<T, false, false, false, MK>), g, b, lds, s, dt,
+                               qs, nq, k, excl, lanes, qm, res, d, stats);
+        else
+            hipLaunchKernelGGL((lane_knn_lds_kernel<T, false, true, false, MK>), g, b, lds, s, dt,
+                               qs, nq, k, excl, lanes, qm, res, d, stats);
     }
-    if constexpr (INTD) {
-        if (k <= 32) return packet_run<T, 32, true>(s, dt, qs, nq, k, excl, bucket, seed, res, d, stats);
-        if (k <= 64) return packet_run<T, 64, true>(s, dt, qs, nq, k, excl, bucket, seed, res, d, stats);
-        if (k <= 128) return packet_run<T, 128, true>(s, dt, qs, nq, k, excl, bucket, seed, res, d, stats);
+    FT_CHECK_LAUNCH();
+    FT_TRY(compact_flags(s, nq));
+    if (stats) {
+        hipLaunchKernelGGL(add_count_kernel, dim3(1), dim3(1), 0, s, stats,
+                           (const uint32_t *)g_rerun.count);
+        FT_CHECK_LAUNCH();
     }
-    return fail(FT_EINVAL, "packet traversal: k above the heap tiers");
+    return launch_exact_listed<T, MK>(s, dt, qs, nq, k, excl, order, res, d);
+}
+
+// Query-order sort buffers (one set per process, like the re-run buffers).
+struct QueryOrderBuf {
+    Scratch S;
+    uint64_t *keys = nullptr, *sorted = nullptr;
+    size_t cap = 0;
+};
+static QueryOrderBuf g_qorder;
+
+template <typename T>
+static int query_order(hipStream_t s, const DTree<T> &dt, const QSrc<T> &qs, int nq,
+                       const uint32_t *&order) {
+    if (g_qorder.cap < (size_t)nq) {
+        if (g_qorder.keys) {
+            FT_HIP(hipDeviceSynchronize());  // earlier launches may still read them
+            (void)hipFree(g_qorder.keys);
+        }
+        g_qorder.keys = nullptr;
+        const size_t cap = (size_t)nq + (size_t)nq / 2 + 1024;
+        if (hipMalloc(&g_qorder.keys, 2 * cap * 8) != hipSuccess)
+            return fail(FT_ENOMEM, "query order allocation failed");
+        g_qorder.sorted = g_qorder.keys + cap;
+        g_qorder.cap = cap;
+    }
+    hipLaunchKernelGGL((query_key_kernel<T>), dim3(blocks_for(nq, 256)), dim3(256), 0, s, dt, qs,
+                       nq, g_qorder.keys);
+    FT_CHECK_LAUNCH();
+    FT_TRY(radix_sort_u64(s, g_qorder.S, g_qorder.keys, (size_t)nq, g_qorder.sorted,
+                          qs.qx ? 63 : 32));
+    order = g_qorder.S.perm;
+    return FT_OK;
 }
 
 template <typename T>
@@ -690,30 +819,43 @@ static int knn_dispatch(hipStream_t s, const ft_tree *tree, const QSrc<T> &qs, i
             FT_CHECK_LAUNCH();
             return FT_OK;
         }
+        size_t lds = 0;
+        const int lanes = lds_lanes<T>(k, STACK_DEPTH, lds);
+        if (!lanes) return fail(FT_ELDS, "k too large for local memory: reduce k");
+        const QueryMap qm{nullptr, nullptr, nullptr, nullptr};
+        const dim3 g(blocks_for(nq, lanes)), b(LANE_BLOCK);
         if (algo == FT_ALGO_PLAIN)
-            return launch_lane_lds<T, true, false>(s, dt, qs, nq, k, excl, res, dists, stats);
-        return launch_lane_lds<T, true, true>(s, dt, qs, nq, k, excl, res, dists, stats);
+            hipLaunchKernelGGL((lane_knn_lds_kernel<T, true, false, false, MK_ANY>), g, b, lds, s,
+                               dt, qs, nq, k, excl, lanes, qm, res, dists, stats);
+        else
+            hipLaunchKernelGGL((lane_knn_lds_kernel<T, true, true, false, MK_ANY>), g, b, lds, s,
+                               dt, qs, nq, k, excl, lanes, qm, res, dists, stats);
+        FT_CHECK_LAUNCH();
+        return FT_OK;
     }
-    // Canonical mode.
-    // Packet tiers: registers for k <= 32; an LDS heap per lane for integer
-    // metrics up to k = 128.  Larger k (or FP metrics above 32) take the
-    // per-lane LDS heap walk.
+    // Canonical mode.  Packet tiers: registers for k <= 32; an LDS heap per
+    // lane for integer metrics up to k = 128.  Larger k (or FP metrics above
+    // 32) take the per-lane heap walk.
     const bool packet_ok = k <= 32 || (!CoordTraits<T>::fp && k <= 128);
     if (algo == FT_ALGO_AUTO) algo = packet_ok ? FT_ALGO_PACKET : FT_ALGO_GROUPED;
     if (o.seed_bound && packet_ok) algo = FT_ALGO_PACKET;
-    if (algo == FT_ALGO_PACKET && packet_ok) {
-        const int bucket = o.leaf_bucket > 0 ? std::min(o.leaf_bucket, 64) : 32;
-        return launch_packet<T>(s, dt, qs, nq, k, excl, bucket, (const D *)o.seed_bound, res,
-                                dists, stats, o.topk);
+    if (algo == FT_ALGO_PACKET && !packet_ok) algo = FT_ALGO_GROUPED;
+    const D *seed = algo == FT_ALGO_PACKET ? (const D *)o.seed_bound : nullptr;
+    const int bucket = o.leaf_bucket > 0 ? std::min(o.leaf_bucket, 64) : 32;
+    const uint32_t *order = nullptr;
+    if (algo == FT_ALGO_PACKET && (qs.qx || qs.leaf_ids) && nq > 64)
+        FT_TRY(query_order<T>(s, dt, qs, nq, order));
+    const int window = o.window < 0 ? 0 : (o.window == 0 ? 64 : std::min(o.window, 1 << 20));
+    switch (metric_kind<T>(dt.m)) {
+    case MK_WRAP:
+        return run_canonical<T, MK_WRAP>(s, dt, qs, nq, k, excl, algo, bucket, o.topk, window,
+                                         order, seed, res, dists, stats);
+    case MK_OPEN:
+        return run_canonical<T, MK_OPEN>(s, dt, qs, nq, k, excl, algo, bucket, o.topk, window,
+                                         order, seed, res, dists, stats);
+    default:
+        return fail(FT_EINVAL, "unsupported metric: periodic integer coordinates need B <= 31");
     }
-    if (k <= 32) {
-        if (algo == FT_ALGO_PLAIN)
-            return launch_lane_reg<T, false>(s, dt, qs, nq, k, excl, res, dists, stats);
-        return launch_lane_reg<T, true>(s, dt, qs, nq, k, excl, res, dists, stats);
-    }
-    if (algo == FT_ALGO_PLAIN)
-        return launch_lane_lds<T, false, false>(s, dt, qs, nq, k, excl, res, dists, stats);
-    return launch_lane_lds<T, false, true>(s, dt, qs, nq, k, excl, res, dists, stats);
 }
 
 // Entry points used by the C ABI layer (device pointers only).

@@ -361,17 +361,22 @@ template <int M> struct TopKPacked {
     }
     __device__ inline uint64_t worst_d() const { return key[M - 1] >> ib; }
     __device__ inline bool beats_worst(uint64_t d, int32_t i) const { return make(d, i) < key[M - 1]; }
-    // A lane that takes nothing carries the all-ones key, which no slot is
-    // above, so the chain needs no predicate.
+    // Sorted insertion as an in-place shift from the top slot down: slot s
+    // takes slot s-1 when the new key is below both, the new key when it is
+    // below s only.  A lane that takes nothing uses the all-ones key, which is
+    // below no slot, so no predicate is needed; every compare reads an
+    // unshifted slot, so the compares are independent.
     __device__ inline void insert(bool take, uint64_t d, int32_t i) {
-        uint64_t c = take ? make(d, i) : ~0ull;
+        const uint64_t c = take ? make(d, i) : ~0ull;
+        bool lt_hi = c < key[M - 1];
 #pragma unroll
-        for (int s = 0; s < M; ++s) {
-            const uint64_t t = key[s];
-            const bool lt = c < t;
-            key[s] = lt ? c : t;
-            c = lt ? t : c;
+        for (int s = M - 1; s >= 1; --s) {
+            const bool lt_lo = c < key[s - 1];
+            const uint64_t from = lt_lo ? key[s - 1] : c;
+            key[s] = lt_hi ? from : key[s];
+            lt_hi = lt_lo;
         }
+        key[0] = lt_hi ? c : key[0];
     }
     // Sentinels left in live slots, or a saturated k-th distance.
     __device__ inline bool needs_exact(int k) const {
@@ -426,7 +431,7 @@ struct PacketCounters {
 template <typename T, typename H, int MK>
 __device__ __forceinline__ void packet_walk(const DTree<T> &t, int bucket, bool active, T px, T py,
                                             T pz, int self, H &h, int32_t *stk,
-                                            PacketCounters &c) {
+                                            PacketCounters &c, int win_lo = 0, int win_hi = 0) {
     using D = typename CoordTraits<T>::D;
     const int lane = threadIdx.x & 63;
     const int n = t.n;
@@ -441,7 +446,7 @@ __device__ __forceinline__ void packet_walk(const DTree<T> &t, int bucket, bool 
     // phase 2 inserts the marked leaves in rounds (each lane fetches its own
     // next candidate from the holding lane), so the insertion chain runs
     // once per round instead of once per leaf.
-    auto scan_range = [&](int first, int cnt) {
+    auto scan_range = [&](int first, int cnt, bool skip_window) {
         T bx = T(0), by = T(0), bz = T(0);
         int32_t bo = 0;
         if (lane < cnt) {
@@ -476,19 +481,50 @@ __device__ __forceinline__ void packet_walk(const DTree<T> &t, int bucket, bool 
         if (!active) cand = 0;
         const int sr = self - first;
         if (sr >= 0 && sr < cnt) cand &= ~(1ull << sr);
+        if (skip_window) {  // leaves of the pre-scanned window are already in h
+            const int a0 = win_lo - first > 0 ? win_lo - first : 0;
+            const int a1 = win_hi - first < cnt ? win_hi - first : cnt;
+            if (a1 > a0) {
+                const uint64_t hi_m = a1 >= 64 ? ~0ull : ((1ull << a1) - 1);
+                cand &= ~(hi_m & ~((1ull << a0) - 1));
+            }
+        }
         c.leaves += (uint64_t)cnt;
-        while (__any(cand != 0)) {
+        // One insertion round: each lane takes its lowest remaining candidate.
+        // Integer periodic metrics need only the low 32 bits of coordinates.
+        auto round = [&]() {
             const bool has = cand != 0;
             const int j = has ? (__ffsll((long long)cand) - 1) : 0;
             cand &= cand - 1;
-            const T x = shfl_any(bx, j), y = shfl_any(by, j), z = shfl_any(bz, j);
             const int32_t ri = shfl_any(bo, j);
-            const D d = leaf_d2_k<T, MK>(t.m, px, py, pz, x, y, z);
+            D d;
+            if constexpr (!CoordTraits<T>::fp && MK == MK_WRAP) {
+                const int sh = 32 - t.m.bits;
+                const uint64_t gx = wrap_abs32((uint32_t)px, shfl32((uint32_t)bx, j), sh);
+                const uint64_t gy = wrap_abs32((uint32_t)py, shfl32((uint32_t)by, j), sh);
+                const uint64_t gz = wrap_abs32((uint32_t)pz, shfl32((uint32_t)bz, j), sh);
+                d = gx * gx + gy * gy + gz * gz;
+            } else {
+                const T x = shfl_any(bx, j), y = shfl_any(by, j), z = shfl_any(bz, j);
+                d = leaf_d2_k<T, MK>(t.m, px, py, pz, x, y, z);
+            }
             const bool take = has && h.beats_worst(d, ri);
             h.insert(take, d, ri);
             c.inserts += take ? 1 : 0;
+        };
+        // Two rounds per trip: the loop-carried top-k is copied once per trip.
+        while (__any(cand != 0)) {
+            round();
+            if (!__any(cand != 0)) break;
+            round();
         }
     };
+
+    // Window pre-scan: leaf ranks [win_lo, win_hi) (the packet's own run of
+    // leaves and its neighbours along the curve) go into h first, which gives
+    // most lanes a tight bound before the walk; the walk then skips them.
+    const bool window = win_hi > win_lo;
+    for (int f = win_lo; f < win_hi; f += 64) scan_range(f, win_hi - f < 64 ? win_hi - f : 64, false);
 
     int sp = 0;
     int node = 0;  // an internal node some lane needs (or the single leaf when n == 1)
@@ -500,7 +536,9 @@ __device__ __forceinline__ void packet_walk(const DTree<T> &t, int bucket, bool 
         } else {
             const int first = uni(t.rfirst[node]);
             const int last = uni(t.rlast[node]);
-            if (last - first + 1 <= bucket) {
+            if (window && first >= win_lo && last < win_hi) {
+                // every leaf below is already in h
+            } else if (last - first + 1 <= bucket) {
                 scan_first = first;
                 scan_cnt = last - first + 1;
             } else {
@@ -535,7 +573,7 @@ __device__ __forceinline__ void packet_walk(const DTree<T> &t, int bucket, bool 
                 }
             }
         }
-        if (scan_cnt > 0) scan_range(scan_first, scan_cnt);
+        if (scan_cnt > 0) scan_range(scan_first, scan_cnt, window);
         while (next < 0 && sp > 0) {
             const int cn = uni(stk[--sp]);
             const D dc = node_d2_k<T, MK>(t, px, py, pz, cn);
@@ -630,42 +668,48 @@ template <int M> struct TopKSel<uint64_t, M, true> {
 };
 
 // ---------------------------------------------------------------------------
-// Per-lane DFS with an LDS heap row per lane (LdsHeap).  REF: the reference's
-// walk exactly -- 64-entry stack, children pushed while sp < 62 (deeper ones
-// dropped and counted), `>=` prune.  Otherwise canonical: strict prune, a
-// 128-entry stack that cannot overflow.  GROUPED: the wave skips a node unless
-// some lane still needs it (results do not depend on it).  stk: [depth][lanes].
+// Per-lane DFS walks (one query per lane): pop, node d2, prune, leaf insert or
+// both children pushed, the farther first (src/hlbvh.hpp:1545-1628).  The
+// stack is [DEPTH][lanes] in LDS.
+//  * REF (LDS heap only): the reference's walk exactly -- `>=` prune, pushes
+//    only while sp < DEPTH - 2 = 62, deeper children dropped and counted.
+//  * canonical: strict prune; a push that would pass DEPTH - 2 sets the
+//    returned overflow flag instead.  The caller re-runs such a query with a
+//    128-entry stack (LANE_STACK_CANON), which no Karras tree over (64-bit
+//    key, 32-bit index) can fill, so results never depend on the stack size.
+// GROUPED: the wave skips a node unless some active lane still needs it
+// (any_of_group, src/hlbvh.hpp:1563-1566); results do not depend on it.
 // ---------------------------------------------------------------------------
-template <typename T, bool REF, bool GROUPED>
-__device__ __forceinline__ void lane_heap_walk(const DTree<T> &t, int k, bool active, T px, T py,
-                                               T pz, int self,
-                                               LdsHeap<typename CoordTraits<T>::D, REF> &h,
-                                               int32_t *stk, int lanes, int lane,
-                                               uint64_t &visits, uint64_t &drops,
-                                               uint64_t &inserts) {
+template <typename T, int M, bool GROUPED, int DEPTH, int MK>
+__device__ __forceinline__ bool lane_reg_walk(const DTree<T> &t, bool active, T px, T py, T pz,
+                                              int self, TopK<typename CoordTraits<T>::D, M> &h,
+                                              int32_t *stk, int lanes, int lane,
+                                              uint64_t &visits, uint64_t &inserts) {
     using D = typename CoordTraits<T>::D;
-    constexpr int DEPTH = REF ? STACK_DEPTH : LANE_STACK_CANON;
     const int n = t.n;
+    bool overflow = false;
     int sp = 0;
     if (active) stk[(sp++) * lanes + lane] = 0;
     while (sp > 0) {
         const int node = stk[(--sp) * lanes + lane];
-        const D d2 = node_d2(t, px, py, pz, node);
+        const D d2 = node_d2_k<T, MK>(t, px, py, pz, node);
         ++visits;
-        const bool pruned = h.prune(d2, k);
+        const bool pruned = d2 > h.worst();
         if (GROUPED) {
             if (!__any(!pruned)) continue;
         }
         if (pruned) continue;
         if (node >= n - 1 || n == 1) {
             const int leaf = (n == 1) ? 0 : node;
-            const int rank = leaf - (n - 1);
-            if (rank == self) continue;
-            if (h.push(d2, ret_index(t, leaf), k)) ++inserts;
+            if (leaf - (n - 1) == self) continue;
+            const int32_t ri = ret_index(t, leaf);
+            const bool take = h.beats_worst(d2, ri);
+            h.insert(take, d2, ri);
+            inserts += take ? 1 : 0;
         } else {
             const int l = t.left[node], r = t.right[node];
-            const D ld = node_d2(t, px, py, pz, l);
-            const D rd = node_d2(t, px, py, pz, r);
+            const D ld = node_d2_k<T, MK>(t, px, py, pz, l);
+            const D rd = node_d2_k<T, MK>(t, px, py, pz, r);
             if (sp < DEPTH - 2) {
                 if (ld <= rd) {
                     stk[(sp++) * lanes + lane] = r;
@@ -675,10 +719,58 @@ __device__ __forceinline__ void lane_heap_walk(const DTree<T> &t, int k, bool ac
                     stk[(sp++) * lanes + lane] = r;
                 }
             } else {
-                ++drops;
+                overflow = true;
             }
         }
     }
+    return overflow;
+}
+
+template <typename T, bool REF, bool GROUPED, int DEPTH, int MK>
+__device__ __forceinline__ bool lane_heap_walk(const DTree<T> &t, int k, bool active, T px, T py,
+                                               T pz, int self,
+                                               LdsHeap<typename CoordTraits<T>::D, REF> &h,
+                                               int32_t *stk, int lanes, int lane,
+                                               uint64_t &visits, uint64_t &drops,
+                                               uint64_t &inserts) {
+    using D = typename CoordTraits<T>::D;
+    const int n = t.n;
+    bool overflow = false;
+    int sp = 0;
+    if (active) stk[(sp++) * lanes + lane] = 0;
+    while (sp > 0) {
+        const int node = stk[(--sp) * lanes + lane];
+        const D d2 = node_d2_k<T, MK>(t, px, py, pz, node);
+        ++visits;
+        const bool pruned = h.prune(d2, k);
+        if (GROUPED) {
+            if (!__any(!pruned)) continue;
+        }
+        if (pruned) continue;
+        if (node >= n - 1 || n == 1) {
+            const int leaf = (n == 1) ? 0 : node;
+            if (leaf - (n - 1) == self) continue;
+            if (h.push(d2, ret_index(t, leaf), k)) ++inserts;
+        } else {
+            const int l = t.left[node], r = t.right[node];
+            const D ld = node_d2_k<T, MK>(t, px, py, pz, l);
+            const D rd = node_d2_k<T, MK>(t, px, py, pz, r);
+            if (sp < DEPTH - 2) {
+                if (ld <= rd) {
+                    stk[(sp++) * lanes + lane] = r;
+                    stk[(sp++) * lanes + lane] = l;
+                } else {
+                    stk[(sp++) * lanes + lane] = l;
+                    stk[(sp++) * lanes + lane] = r;
+                }
+            } else if (REF) {
+                ++drops;
+            } else {
+                overflow = true;
+            }
+        }
+    }
+    return overflow;
 }
 
 }  // namespace ft

@@ -124,7 +124,11 @@ typedef struct ft_query_opts {
                             /* Canonical mode only.  Results are unchanged by it.       */
     uint64_t *stats;     /* optional device uint64_t[8] counters, accumulated:           */
                          /* [0] nodes visited [1] leaves tested [2] stack overflows     */
-                         /* [3] seeded queries re-run unbounded [4] heap insertions     */
+                         /* [3] queries re-run exactly [4] heap insertions              */
+    int32_t window;      /* FT_ALGO_PACKET self queries over all leaves: leaves scanned  */
+                         /* on each side of a packet's own run before the walk (0 =     */
+                         /* auto, negative = none)                                       */
+    int32_t reserved2;
 } ft_query_opts;
 
 /* ---- library / device utilities --------------------------------------- */

@@ -188,3 +188,31 @@ def test_seeded_bound_never_changes_results(ft, orc):
         assert np.array_equal(d, db) and np.array_equal(r, rb), k
         st = stats.to_host()
         assert st[3] == int(np.sum(seed < db[:, -1]))  # exactly the too-small seeds re-ran
+
+
+def comb_points(ct):
+    """63 points whose Morton keys are the 63 powers of two below 2^63, plus the
+    origin: the radix tree is a comb 63 levels deep, deeper than the
+    reference's 64-entry per-lane stack can hold."""
+    pts = [(0, 0, 0)]
+    for i in range(21):
+        pts += [(1 << i, 0, 0), (0, 1 << i, 0), (0, 0, 1 << i)]
+    a = np.array(pts, dtype=O.COORD_DTYPES[ct])
+    return a[:, 0].copy(), a[:, 1].copy(), a[:, 2].copy()
+
+
+@pytest.mark.parametrize("ct", [O.CT_U64, O.CT_U32])
+def test_deep_comb_tree(ft, orc, ct):
+    """Canonical walks stay exact on a tree deeper than the 64-entry stack (the
+    overflowing queries are re-run with a deeper stack); reference mode drops
+    the deep children exactly as the oracle's restatement of the reference."""
+    pts = comb_points(ct)
+    tree, t = make(ft, ct, O.SFC_MORTON, 21, 0, pts)
+    for k in (1, 4, 33):
+        rb, db = O.knn_bruteforce(t, k, True)
+        for algo in ("plain", "grouped", "packet"):
+            r, d = ft.self_knn_query(tree, k, max_k=256, algorithm=ALGOS[algo])
+            assert np.array_equal(d, db) and np.array_equal(r, rb), (algo, k)
+        rr, dr = O.knn_reference(t, k, True, max_k=256)
+        r, d = ft.self_knn_query(tree, k, max_k=256, mode=ft.MODE_REFERENCE)
+        assert np.array_equal(d, dr) and np.array_equal(r, rr), ("reference", k)
