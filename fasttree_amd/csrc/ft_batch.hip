@@ -116,9 +116,27 @@ __global__ void batch_reset_kernel(uint32_t *ctr, int n) {
     if ((int)threadIdx.x < n) ctr[threadIdx.x] = 0u;
 }
 
-__device__ inline int grab_item(uint32_t *ctr) {
-    int it = 0;
-    if ((threadIdx.x & 63) == 0) it = (int)atomicAdd(ctr, 1u);
+// Work queue of a tier kernel: its item list is split into FT_XCDS contiguous
+// chunks [bnd[c], bnd[c+1]) of about equal cost, one per XCD (the hardware
+// hands workgroup b to XCD b % 8), so each XCD's L2 serves a contiguous run of
+// requests.  A wave takes the next item of its own XCD's chunk and, once that
+// is drained, of the following chunks; -1 when every chunk is drained, which
+// every wave reaches.
+__device__ inline int grab_item(uint32_t *ctr, const uint32_t *bnd) {
+    int it = -1;
+    if ((threadIdx.x & 63) == 0) {
+        const int x = (int)(blockIdx.x % FT_XCDS);
+        for (int t = 0; t < FT_XCDS; ++t) {
+            const int cidx = (x + t) % FT_XCDS;
+            const uint32_t lo = bnd[cidx], hi = bnd[cidx + 1];
+            if (lo >= hi) continue;
+            const uint32_t i = lo + atomicAdd(&ctr[cidx], 1u);
+            if (i < hi) {
+                it = (int)i;
+                break;
+            }
+        }
+    }
     return uni(__shfl(it, 0, 64));
 }
 
@@ -126,7 +144,8 @@ __device__ inline int grab_item(uint32_t *ctr) {
 template <typename T, int M, bool HEAP, int MK>
 __global__ __launch_bounds__(64) void batch_packet_kernel(BatchView<T> b,
                                                           const uint2 *__restrict__ items,
-                                                          int n_items, uint32_t *__restrict__ ctr) {
+                                                          const uint32_t *__restrict__ bnd,
+                                                          uint32_t *__restrict__ ctr) {
     using D = typename CoordTraits<T>::D;
     constexpr bool PACKED = !CoordTraits<T>::fp;
     static_assert(!HEAP || PACKED, "LDS heap tier needs packed integer keys");
@@ -141,8 +160,8 @@ __global__ __launch_bounds__(64) void batch_packet_kernel(BatchView<T> b,
     const int lane = threadIdx.x;
     PacketCounters c;
     while (true) {
-        const int it = grab_item(ctr);
-        if (it >= n_items) break;
+        const int it = grab_item(ctr, bnd);
+        if (it < 0) break;
         const uint2 w = items[it];
         const BatchReq q = b.reqs[uni((int)w.x)];
         const int j = uni((int)w.y);
@@ -187,8 +206,9 @@ __global__ __launch_bounds__(64) void batch_packet_kernel(BatchView<T> b,
 template <typename T, int MK>
 __global__ __launch_bounds__(64) void batch_lane_kernel(BatchView<T> b,
                                                         const uint2 *__restrict__ items,
-                                                        int n_items, uint32_t *__restrict__ ctr,
-                                                        int lanes, int kcap) {
+                                                        const uint32_t *__restrict__ bnd,
+                                                        uint32_t *__restrict__ ctr, int lanes,
+                                                        int kcap) {
     using D = typename CoordTraits<T>::D;
     extern __shared__ __align__(16) unsigned char lds[];
     D *hd = (D *)lds;
@@ -197,8 +217,8 @@ __global__ __launch_bounds__(64) void batch_lane_kernel(BatchView<T> b,
     const int lane = threadIdx.x;
     uint64_t visits = 0, drops = 0, inserts = 0;
     while (true) {
-        const int it = grab_item(ctr);
-        if (it >= n_items) break;
+        const int it = grab_item(ctr, bnd);
+        if (it < 0) break;
         const uint2 w = items[it];
         const BatchReq q = b.reqs[uni((int)w.x)];
         const int j = uni((int)w.y);
@@ -342,7 +362,8 @@ struct ft_batch {
     void *dtrees = nullptr;
     ft::BatchReq *dreqs = nullptr;
     uint2 *ditems = nullptr;
-    uint32_t *dctr = nullptr;  // [V_COUNT] + rerun count
+    uint32_t *dctr = nullptr;  // [V_COUNT][FT_XCDS] queue counters, then the re-run count
+    uint32_t *dbnd = nullptr;  // [V_COUNT][FT_XCDS + 1] chunk bounds, then 9 for the tuner
     int32_t *drerun = nullptr;
     uint64_t *dkeys = nullptr, *dsorted = nullptr;
     int item_first[ft::V_COUNT] = {}, item_count[ft::V_COUNT] = {};
@@ -464,6 +485,7 @@ template <typename T> static int upload_tables(ft_batch *b) {
     }
     b->n_entries = entry;
     std::vector<uint2> flat;
+    std::vector<uint32_t> bnd((size_t)V_COUNT * (FT_XCDS + 1) + FT_XCDS + 1, 0);
     for (int v = 0; v < V_COUNT; ++v) {
         auto &L = items[v];
         std::stable_sort(L.begin(), L.end(),
@@ -473,6 +495,17 @@ template <typename T> static int upload_tables(ft_batch *b) {
         b->item_first[v] = (int)flat.size();
         b->item_count[v] = (int)L.size();
         for (auto &p : L) flat.push_back(p.second);
+        // FT_XCDS contiguous chunks of about equal measured cost
+        double total = 0;
+        for (auto &p : L) total += p.first;
+        uint32_t *bv = &bnd[(size_t)v * (FT_XCDS + 1)];
+        double run = 0;
+        int c = 1;
+        for (size_t i = 0; i < L.size() && c < FT_XCDS; ++i) {
+            run += L[i].first;
+            while (c < FT_XCDS && run >= total * c / FT_XCDS) bv[c++] = (uint32_t)(i + 1);
+        }
+        for (; c <= FT_XCDS; ++c) bv[c] = (uint32_t)L.size();
     }
     // one device block: trees | reqs | items | counters | rerun | keys | sorted keys
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
@@ -480,8 +513,9 @@ template <typename T> static int upload_tables(ft_batch *b) {
     const size_t o_t = 0, s_t = al(sizeof(DTree<T>) * std::max(nt, 1));
     const size_t o_r = o_t + s_t, s_r = al(sizeof(BatchReq) * std::max(nr, 1));
     const size_t o_i = o_r + s_r, s_i = al(sizeof(uint2) * std::max<size_t>(flat.size(), 1));
-    const size_t o_c = o_i + s_i, s_c = al(4 * (V_COUNT + 1));
-    const size_t o_l = o_c + s_c, s_l = al(4 * ne);
+    const size_t o_c = o_i + s_i, s_c = al(4 * (V_COUNT * FT_XCDS + 1));
+    const size_t o_b = o_c + s_c, s_b = al(4 * bnd.size());
+    const size_t o_l = o_b + s_b, s_l = al(4 * ne);
     const size_t o_k = o_l + s_l, s_k = al(8 * ne);
     const size_t o_s = o_k + s_k, s_s = al(8 * ne);
     const size_t total = o_s + s_s;
@@ -497,6 +531,7 @@ template <typename T> static int upload_tables(ft_batch *b) {
     b->dreqs = (BatchReq *)(base + o_r);
     b->ditems = (uint2 *)(base + o_i);
     b->dctr = (uint32_t *)(base + o_c);
+    b->dbnd = (uint32_t *)(base + o_b);
     b->drerun = (int32_t *)(base + o_l);
     b->dkeys = (uint64_t *)(base + o_k);
     b->dsorted = (uint64_t *)(base + o_s);
@@ -505,7 +540,8 @@ template <typename T> static int upload_tables(ft_batch *b) {
     if (!flat.empty())
         FT_HIP(hipMemcpy(b->ditems, flat.data(), sizeof(uint2) * flat.size(),
                          hipMemcpyHostToDevice));
-    FT_HIP(hipMemset(b->dctr, 0, 4 * (V_COUNT + 1)));
+    FT_HIP(hipMemset(b->dctr, 0, 4 * (V_COUNT * FT_XCDS + 1)));
+    FT_HIP(hipMemcpy(b->dbnd, bnd.data(), 4 * bnd.size(), hipMemcpyHostToDevice));
     FT_TRY(scratch_ensure_public(b->sort_scratch, ne));
     b->dirty = false;
     return FT_OK;
@@ -523,7 +559,7 @@ template <typename T> static BatchView<T> view(ft_batch *b) {
     v.res = b->results;
     v.dists = (typename CoordTraits<T>::D *)b->dists;
     v.rerun = b->drerun;
-    v.rerun_count = b->dctr + V_COUNT;
+    v.rerun_count = b->dctr + V_COUNT * FT_XCDS;
     v.stats = b->stats;
     return v;
 }
@@ -546,18 +582,18 @@ template <typename K> static unsigned persistent_waves(K kernel, size_t dyn_lds,
     return (unsigned)std::max(1L, std::min<long>((long)n_items, (long)per_cu * n_cu));
 }
 
-// Launches one tier kernel over `n_items` items starting at `first`.
+// Launches one tier kernel over `n_items` items (its work queue: chunk bounds
+// `bnd`, counters `ctr`).
 template <typename T, int MK>
-static int launch_variant_mk(ft_batch *b, hipStream_t s, int v, int first, int n_items,
-                             uint32_t *ctr) {
+static int launch_variant_mk(ft_batch *b, hipStream_t s, int v, const uint2 *it, int n_items,
+                             const uint32_t *bnd, uint32_t *ctr) {
     const BatchView<T> bv = view<T>(b);
-    const uint2 *it = b->ditems + first;
     constexpr bool FP = CoordTraits<T>::fp;
 #define FT_BPK(M_, HEAP_)                                                                       \
     do {                                                                                        \
         auto kf = batch_packet_kernel<T, M_, HEAP_, MK>;                                        \
         hipLaunchKernelGGL(kf, dim3(persistent_waves(kf, 0, n_items)), dim3(64), 0, s, bv, it,  \
-                           n_items, ctr);                                                       \
+                           bnd, ctr);                                                           \
     } while (0)
     switch (v) {
     case V_PK1: FT_BPK(1, false); break;
@@ -582,7 +618,7 @@ static int launch_variant_mk(ft_batch *b, hipStream_t s, int v, int first, int n
         if (!lanes) return fail(FT_ELDS, "k too large for local memory: reduce k");
         auto kf = batch_lane_kernel<T, MK>;
         hipLaunchKernelGGL(kf, dim3(persistent_waves(kf, lds, n_items)), dim3(64), lds, s, bv, it,
-                           n_items, ctr, lanes, b->kmax);
+                           bnd, ctr, lanes, b->kmax);
         break;
     }
     default: return fail(FT_EINVAL, "unknown batch tier");
@@ -593,12 +629,12 @@ static int launch_variant_mk(ft_batch *b, hipStream_t s, int v, int first, int n
 }
 
 template <typename T>
-static int launch_variant(ft_batch *b, hipStream_t s, int v, int first, int n_items,
-                          uint32_t *ctr) {
+static int launch_variant(ft_batch *b, hipStream_t s, int v, const uint2 *it, int n_items,
+                          const uint32_t *bnd, uint32_t *ctr) {
     if (n_items <= 0) return FT_OK;
-    if (b->mk == MK_WRAP) return launch_variant_mk<T, MK_WRAP>(b, s, v, first, n_items, ctr);
-    if (b->mk == MK_OPEN) return launch_variant_mk<T, MK_OPEN>(b, s, v, first, n_items, ctr);
-    return launch_variant_mk<T, MK_ANY>(b, s, v, first, n_items, ctr);
+    if (b->mk == MK_WRAP) return launch_variant_mk<T, MK_WRAP>(b, s, v, it, n_items, bnd, ctr);
+    if (b->mk == MK_OPEN) return launch_variant_mk<T, MK_OPEN>(b, s, v, it, n_items, bnd, ctr);
+    return launch_variant_mk<T, MK_ANY>(b, s, v, it, n_items, bnd, ctr);
 }
 
 template <typename T> static int launch_exact(ft_batch *b, hipStream_t s) {
@@ -610,7 +646,7 @@ template <typename T> static int launch_exact(ft_batch *b, hipStream_t s) {
     FT_CHECK_LAUNCH();
     if (b->stats) {
         hipLaunchKernelGGL(batch_count_kernel, dim3(1), dim3(1), 0, s, b->stats,
-                           (const uint32_t *)(b->dctr + V_COUNT));
+                           (const uint32_t *)(b->dctr + V_COUNT * FT_XCDS));
         FT_CHECK_LAUNCH();
     }
     return FT_OK;
@@ -695,7 +731,8 @@ template <typename T> static int enqueue_sort(ft_batch *b, hipStream_t s) {
 // Tier kernels on parallel branches (or in order on s), then the exact pass.
 template <typename T> static int enqueue_queries(ft_batch *b, hipStream_t s) {
     if (b->n_entries == 0) return FT_OK;
-    hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(64), 0, s, b->dctr, V_COUNT + 1);
+    hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(128), 0, s, b->dctr,
+                       V_COUNT * FT_XCDS + 1);
     FT_CHECK_LAUNCH();
     std::vector<int> used;
     for (int v = 0; v < V_COUNT; ++v)
@@ -708,8 +745,9 @@ template <typename T> static int enqueue_queries(ft_batch *b, hipStream_t s) {
     }
     for (int i = 0; i < nb; ++i) {
         const int v = used[i];
-        FT_TRY(launch_variant<T>(b, nb > 1 ? b->side[i] : s, v, b->item_first[v],
-                                 b->item_count[v], b->dctr + v));
+        FT_TRY(launch_variant<T>(b, nb > 1 ? b->side[i] : s, v, b->ditems + b->item_first[v],
+                                 b->item_count[v], b->dbnd + v * (FT_XCDS + 1),
+                                 b->dctr + v * FT_XCDS));
     }
     if (nb > 1) {
         for (int i = 0; i < nb; ++i) {
@@ -777,7 +815,11 @@ template <typename T> static int tune_t(ft_batch *b, hipStream_t s) {
                     break;
                 }
                 (void)hipMemcpy(tmp, its.data(), its.size() * sizeof(uint2), hipMemcpyHostToDevice);
-                uint2 *saved = b->ditems;
+                // one chunk holding every item: all XCDs drain it
+                std::vector<uint32_t> tb(FT_XCDS + 1, (uint32_t)its.size());
+                tb[0] = 0;
+                uint32_t *tbnd = b->dbnd + V_COUNT * (FT_XCDS + 1);
+                (void)hipMemcpy(tbnd, tb.data(), 4 * tb.size(), hipMemcpyHostToDevice);
                 const std::vector<int> buckets =
                     is_packet(v) ? std::vector<int>{16, 32, 64} : std::vector<int>{32};
                 for (int bk : buckets) {
@@ -789,18 +831,16 @@ template <typename T> static int tune_t(ft_batch *b, hipStream_t s) {
                         q.variant = v;
                         (void)hipMemcpy(b->dreqs + i, &q, sizeof q, hipMemcpyHostToDevice);
                     }
-                    b->ditems = tmp;
                     float t_best = 1e30f;
                     for (int rep = 0; rep < 2 && rc == FT_OK; ++rep) {
-                        hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(64), 0, s, b->dctr,
-                                           V_COUNT + 1);
+                        hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(128), 0, s, b->dctr,
+                                           V_COUNT * FT_XCDS + 1);
                         (void)hipEventRecord(e0, s);
-                        rc = launch_variant<T>(b, s, v, 0, (int)its.size(), b->dctr);
+                        rc = launch_variant<T>(b, s, v, tmp, (int)its.size(), tbnd, b->dctr);
                         (void)hipEventRecord(e1, s);
                         (void)hipEventSynchronize(e1);
                         t_best = std::min(t_best, elapsed(e0, e1));
                     }
-                    b->ditems = saved;
                     if (rc != FT_OK) break;
                     if (t_best < best) {
                         best = t_best;

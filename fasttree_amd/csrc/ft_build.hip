@@ -419,7 +419,9 @@ __global__ __launch_bounds__(256) void gather_kernel(
     const uint32_t *__restrict__ id, const int8_t *__restrict__ ghost,
     const uint32_t *__restrict__ perm, int n, T *mnx, T *mxx, T *mny, T *mxy, T *mnz, T *mxz,
     uint32_t *tid_out, int8_t *tghost, int32_t *torig) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    // XCD-ordered blocks: neighbouring sorted particles (mostly neighbouring
+    // input rows of one cluster) are fetched through one L2.
+    const int i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t o = perm[i];
     const int leaf = i + n - 1;
@@ -472,7 +474,7 @@ __global__ __launch_bounds__(256) void karras_kernel(const uint64_t *__restrict_
                                                      int32_t *__restrict__ rfirst,
                                                      int32_t *__restrict__ rlast,
                                                      uint32_t *__restrict__ counters) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = xcd_block() * blockDim.x + threadIdx.x;  // searches read nearby keys
     if (i >= n - 1) return;
     const uint64_t ki = k[i];
     const int dd = kdelta(k, n, i, ki, i + 1) - kdelta(k, n, i, ki, i - 1);
@@ -846,7 +848,7 @@ template <typename T> static int launch_topology(hipStream_t s, ft_tree *t, Scra
         FT_CHECK_LAUNCH();
         return FT_OK;
     }
-    hipLaunchKernelGGL(karras_kernel, dim3(grid_for(n - 1, 256)), dim3(256), 0, s,
+    hipLaunchKernelGGL(karras_kernel, dim3(xcd_grid(grid_for(n - 1, 256))), dim3(256), 0, s,
                        (const uint64_t *)t->sorted_keys, n, t->left_child, t->right_child,
                        t->parent, t->range_first, t->range_last, S.counters);
     FT_CHECK_LAUNCH();
@@ -891,7 +893,7 @@ int build_bvh_t(hipStream_t s, ft_tree *t, Scratch &S, const T *x, const T *y, c
     prof_mark(S, s, 2);
     FT_TRY(radix_sort(s, S, kin, n, key_bits, t->sorted_keys, keys == nullptr));
     prof_mark(S, s, 3);
-    hipLaunchKernelGGL(gather_kernel<T>, dim3(grid_for(n, 256)), dim3(256), 0, s, x, y, z, id,
+    hipLaunchKernelGGL(gather_kernel<T>, dim3(xcd_grid(grid_for(n, 256))), dim3(256), 0, s, x, y, z, id,
                        ghost, (const uint32_t *)S.perm, (int)n, (T *)t->min_x, (T *)t->max_x,
                        (T *)t->min_y, (T *)t->max_y, (T *)t->min_z, (T *)t->max_z, t->id,
                        t->is_ghost, t->orig_idx);

@@ -208,6 +208,24 @@ template <typename D> __host__ __device__ inline D missing_dist() {
     else return 0;
 }
 
+// ---------------------------------------------------------------------------
+// XCD-aware block order.  MI355X hands workgroup b of a launch to XCD b % 8,
+// and each XCD has its own L2.  With the grid padded to a multiple of 8,
+// xcd_block() maps the workgroups of XCD x to the contiguous run of logical
+// blocks [x * G/8, (x+1) * G/8), so neighbouring data (neighbouring packets
+// of queries, neighbouring sorted particles) is read through one L2 instead
+// of being fetched into all eight.
+// ---------------------------------------------------------------------------
+constexpr int FT_XCDS = 8;
+__device__ inline int xcd_block() {
+    const int per = (int)gridDim.x / FT_XCDS;
+    return ((int)blockIdx.x % FT_XCDS) * per + (int)blockIdx.x / FT_XCDS;
+}
+inline unsigned xcd_grid(size_t blocks) {
+    const size_t g = (blocks + FT_XCDS - 1) / FT_XCDS * FT_XCDS;
+    return (unsigned)(g ? g : FT_XCDS);
+}
+
 // Device view of a built tree.
 template <typename T> struct DTree {
     const T *mnx, *mxx, *mny, *mxy, *mnz, *mxz;

@@ -61,7 +61,7 @@ __global__ __launch_bounds__(LANE_BLOCK) void lane_knn_reg_kernel(
     const int lane = threadIdx.x;
     const int nlist = LISTED ? (int)*qm.list_count : nq;
     uint64_t visits = 0, inserts = 0;
-    for (int base = blockIdx.x * LANE_BLOCK; base < nlist;
+    for (int base = (LISTED ? (int)blockIdx.x : xcd_block()) * LANE_BLOCK; base < nlist;
          base += (LISTED ? gridDim.x * LANE_BLOCK : nlist)) {
         const int pos = base + lane;
         const bool active = pos < nlist;
@@ -105,7 +105,8 @@ __global__ __launch_bounds__(LANE_BLOCK) void lane_knn_lds_kernel(
     const bool lane_ok = lane < lanes;
     const int nlist = LISTED ? (int)*qm.list_count : nq;
     uint64_t visits = 0, drops = 0, inserts = 0;
-    for (int base = blockIdx.x * lanes; base < nlist; base += (LISTED ? gridDim.x * lanes : nlist)) {
+    for (int base = (LISTED ? (int)blockIdx.x : xcd_block()) * lanes; base < nlist;
+         base += (LISTED ? gridDim.x * lanes : nlist)) {
         const int pos = base + lane;
         const bool active = lane_ok && pos < nlist;
         const int slot = active ? (LISTED ? qm.list[pos] : pos) : 0;
@@ -298,7 +299,9 @@ __global__ __launch_bounds__(64) void packet_knn_kernel(
     const int lane = threadIdx.x;
     const int k = a.k;
     PacketCounters c;
-    const int slot = blockIdx.x * 64 + lane;
+    const int blk = xcd_block();
+    if (blk * 64 >= a.nq) return;  // padding block of the XCD-ordered grid
+    const int slot = blk * 64 + lane;
     const bool active = slot < a.nq;
     const int q = active ? slot_query(qm, slot) : 0;
     T px = T(0), py = T(0), pz = T(0);
@@ -313,7 +316,7 @@ __global__ __launch_bounds__(64) void packet_knn_kernel(
     // its neighbours along the curve are the leaves on either side.
     int wlo = 0, whi = 0;
     if (a.window > 0) {
-        const int r0 = blockIdx.x * 64;
+        const int r0 = blk * 64;
         wlo = r0 - a.window > 0 ? r0 - a.window : 0;
         whi = r0 + 64 + a.window < t.n ? r0 + 64 + a.window : t.n;
     }
@@ -685,7 +688,8 @@ static void launch_pk(hipStream_t s, const DTree<T> &dt, const QSrc<T> &qs, cons
                       const QueryMap &qm, const typename CoordTraits<T>::D *seed, uint64_t *res,
                       typename CoordTraits<T>::D *d) {
     hipLaunchKernelGGL((packet_knn_kernel<T, M, HEAP, SEEDED, PACKED, MK>),
-                       dim3(blocks_for(a.nq, 64)), dim3(64), 0, s, dt, qs, a, qm, seed, res, d);
+                       dim3(xcd_grid(blocks_for(a.nq, 64))), dim3(64), 0, s, dt, qs, a, qm, seed,
+                       res, d);
 }
 
 template <typename T, int M, bool HEAP, int MK>
@@ -732,7 +736,7 @@ static int run_canonical(hipStream_t s, const DTree<T> &dt, const QSrc<T> &qs, i
             else return fail(FT_EINVAL, "packet traversal: k above the heap tiers");
         }
     } else if (k <= 32) {
-        const dim3 g(blocks_for(nq, LANE_BLOCK)), b(LANE_BLOCK);
+        const dim3 g(xcd_grid(blocks_for(nq, LANE_BLOCK))), b(LANE_BLOCK);
         if (algo == FT_ALGO_PLAIN)
             FT_BY_M(k, hipLaunchKernelGGL((lane_knn_reg_kernel<T, M, false, false, MK>), g, b, 0, s,
                                           dt, qs, nq, k, excl, qm, res, d, stats));
@@ -743,7 +747,7 @@ static int run_canonical(hipStream_t s, const DTree<T> &dt, const QSrc<T> &qs, i
         size_t lds = 0;
         const int lanes = lds_lanes<T>(k, STACK_DEPTH, lds);
         if (!lanes) return fail(FT_ELDS, "k too large for local memory: reduce k");
-        const dim3 g(blocks_for(nq, lanes)), b(LANE_BLOCK);
+        const dim3 g(xcd_grid(blocks_for(nq, lanes))), b(LANE_BLOCK);
         if (algo == FT_ALGO_PLAIN)
             hipLaunchKernelGGL((lane_knn_lds_kernel<T, false, false, false, MK>), g, b, lds, s, dt,
                                qs, nq, k, excl, lanes, qm, res, d, stats);
@@ -823,7 +827,7 @@ static int knn_dispatch(hipStream_t s, const ft_tree *tree, const QSrc<T> &qs, i
         const int lanes = lds_lanes<T>(k, STACK_DEPTH, lds);
         if (!lanes) return fail(FT_ELDS, "k too large for local memory: reduce k");
         const QueryMap qm{nullptr, nullptr, nullptr, nullptr};
-        const dim3 g(blocks_for(nq, lanes)), b(LANE_BLOCK);
+        const dim3 g(xcd_grid(blocks_for(nq, lanes))), b(LANE_BLOCK);
         if (algo == FT_ALGO_PLAIN)
             hipLaunchKernelGGL((lane_knn_lds_kernel<T, true, false, false, MK_ANY>), g, b, lds, s,
                                dt, qs, nq, k, excl, lanes, qm, res, dists, stats);
