@@ -207,8 +207,22 @@ __global__ __launch_bounds__(256) void encode_kernel(const T *__restrict__ x,
         if (sfc == FT_SFC_MORTON) key = spread3_21(q[0]) | (spread3_21(q[1]) << 1) | (spread3_21(q[2]) << 2);
         else key = ph_encode(q[0], q[1], q[2], B, tab);
         keys[i] = key;
-        if (hist)
-            for (int p = 0; p < passes; ++p) atomicAdd(&h[p][(key >> (8 * p)) & 255u], 1u);
+        if (hist) {
+            // When a wave's 64 keys share a digit (the high digits of spatially
+            // clustered input runs), one add of the lane count replaces 64
+            // atomics on the same LDS word.
+            const uint64_t act = __ballot(1);
+            const int lead = __ffsll((long long)act) - 1;
+            for (int p = 0; p < passes; ++p) {
+                const uint32_t dg = (uint32_t)(key >> (8 * p)) & 255u;
+                const uint32_t d0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)dg);
+                if (__ballot(dg == d0) == act) {
+                    if ((int)(threadIdx.x & 63) == lead) atomicAdd(&h[p][d0], (uint32_t)__popcll(act));
+                } else {
+                    atomicAdd(&h[p][dg], 1u);
+                }
+            }
+        }
     }
     if (hist) {
         __syncthreads();
@@ -228,8 +242,9 @@ __global__ __launch_bounds__(256) void encode_kernel(const T *__restrict__ x,
 // ===========================================================================
 constexpr int RS_BLOCK = 256;
 constexpr int RS_WAVES = RS_BLOCK / 64;
-constexpr int RS_ITEMS = 16;
-constexpr int RS_TILE = RS_BLOCK * RS_ITEMS;
+constexpr int RS_ITEMS = 12;                   // keys per thread
+constexpr int RS_TILE = RS_BLOCK * RS_ITEMS;   // 3072 keys per tile
+constexpr int RS_TILE_MIN = RS_BLOCK * 8;      // smallest tile: sizes the look-back words
 constexpr int RS_MAX_PASSES = 8;
 
 __global__ void fill_u32_kernel(uint32_t *p, uint32_t v, size_t n) {
@@ -302,8 +317,11 @@ __global__ __launch_bounds__(RS_BLOCK) void onesweep_kernel(
     uint64_t *__restrict__ keys_out, uint32_t *__restrict__ vals_out, int n, int shift,
     const uint32_t *__restrict__ digit_base, uint64_t *__restrict__ lookback,
     const uint32_t *__restrict__ epoch, int pass, uint32_t *__restrict__ tile_ctr) {
+    // One staging buffer: keys first, then the values (as 32-bit words), with
+    // each position's digit kept for the second phase.
     __shared__ uint64_t s_keys[RS_TILE];
-    __shared__ uint32_t s_vals[RS_TILE];
+    __shared__ uint8_t s_dig[RS_TILE];
+    uint32_t *s_vals = (uint32_t *)s_keys;
     __shared__ uint32_t s_wave[RS_WAVES][256];
     __shared__ uint32_t s_local[256];
     __shared__ uint32_t s_global[256];
@@ -387,15 +405,16 @@ __global__ __launch_bounds__(RS_BLOCK) void onesweep_kernel(
     s_global[dg] = digit_base[dg] + excl;
     __syncthreads();
 
-    // Scatter into the tile's digit-sorted order in LDS.
+    // Scatter into the tile's digit-sorted order in LDS: keys, then values.
+    uint32_t lpos[RS_ITEMS];
 #pragma unroll
     for (int j = 0; j < RS_ITEMS; ++j) {
         const int idx = base + j * 64 + lane;
+        lpos[j] = 0xFFFFFFFFu;
         if (idx < n) {
             const uint32_t d = (uint32_t)(k[j] >> shift) & 255u;
-            const uint32_t lp = s_local[d] + s_wave[wave][d] + rank[j];
-            s_keys[lp] = k[j];
-            s_vals[lp] = v[j];
+            lpos[j] = s_local[d] + s_wave[wave][d] + rank[j];
+            s_keys[lpos[j]] = k[j];
         }
     }
     __syncthreads();
@@ -403,9 +422,17 @@ __global__ __launch_bounds__(RS_BLOCK) void onesweep_kernel(
     for (int i = tid; i < tile_n; i += RS_BLOCK) {
         const uint64_t key = s_keys[i];
         const uint32_t d = (uint32_t)(key >> shift) & 255u;
-        const uint32_t pos = s_global[d] + ((uint32_t)i - s_local[d]);
-        keys_out[pos] = key;
-        vals_out[pos] = s_vals[i];
+        s_dig[i] = (uint8_t)d;
+        keys_out[s_global[d] + ((uint32_t)i - s_local[d])] = key;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < RS_ITEMS; ++j)
+        if (lpos[j] != 0xFFFFFFFFu) s_vals[lpos[j]] = v[j];
+    __syncthreads();
+    for (int i = tid; i < tile_n; i += RS_BLOCK) {
+        const uint32_t d = s_dig[i];
+        vals_out[s_global[d] + ((uint32_t)i - s_local[d])] = s_vals[i];
     }
 }
 
@@ -708,7 +735,7 @@ static int scratch_ensure(Scratch &S, size_t n) {
         S.block = nullptr;
     }
     const size_t cap = n + n / 2 + 1024;
-    const size_t tiles = (cap + RS_TILE - 1) / RS_TILE + 1;
+    const size_t tiles = (cap + RS_TILE_MIN - 1) / RS_TILE_MIN + 1;
     auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
     size_t off = 0;
     const size_t o_keys_in = off; off += al(cap * 8);
