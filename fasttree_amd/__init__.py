@@ -394,7 +394,7 @@ class BvhScratch:
         ms = (C.c_double * 8)()
         n = C.c_int()
         _check(lib().ft_scratch_stage_ms(self.handle, ms, 8, C.byref(n)))
-        names = ["bbox", "encode", "sort", "gather", "karras", "refit"]
+        names = ["bbox", "encode", "sort", "assemble", "assemble_top"]
         return {names[i]: ms[i] for i in range(n.value)}
 
     def free_all(self):

@@ -1,6 +1,7 @@
 // ft_build.hip -- gfx950 kernels of the HLBVH build (build_bvh, src/hlbvh.hpp:2165-2302):
-//   bbox -> SFC encode -> stable LSD radix sort (key u64 + index u32) -> leaf gather
-//   -> Karras topology -> bottom-up refit.
+//   bbox -> SFC encode -> stable LSD radix sort (key u64 + index u32) -> tree assembly
+//   (leaf gather, topology and boxes in one bottom-up pass per chunk of leaves, then
+//   the nodes that span chunks).
 // Every stage is stream-ordered; nothing synchronises with the host.
 #include "ft_internal.hpp"
 
@@ -437,115 +438,32 @@ __global__ __launch_bounds__(RS_BLOCK) void onesweep_kernel(
 }
 
 // ===========================================================================
-// K8: leaf gather (src/hlbvh.hpp:2266-2279): leaf slot i+n-1 <- particle perm[i]
-// into both min_* and max_*; id (iota when absent), ghost (0 when absent), orig.
-// ===========================================================================
-template <typename T>
-__global__ __launch_bounds__(256) void gather_kernel(
-    const T *__restrict__ x, const T *__restrict__ y, const T *__restrict__ z,
-    const uint32_t *__restrict__ id, const int8_t *__restrict__ ghost,
-    const uint32_t *__restrict__ perm, int n, T *mnx, T *mxx, T *mny, T *mxy, T *mnz, T *mxz,
-    uint32_t *tid_out, int8_t *tghost, int32_t *torig) {
-    // XCD-ordered blocks: neighbouring sorted particles (mostly neighbouring
-    // input rows of one cluster) are fetched through one L2.
-    const int i = xcd_block() * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t o = perm[i];
-    const int leaf = i + n - 1;
-    const T a = x[o], b = y[o], c = z[o];
-    mnx[leaf] = a; mxx[leaf] = a;
-    mny[leaf] = b; mxy[leaf] = b;
-    mnz[leaf] = c; mxz[leaf] = c;
-    tid_out[leaf] = id ? id[o] : o;
-    tghost[leaf] = ghost ? ghost[o] : (int8_t)0;
-    torig[leaf] = (int32_t)o;
-}
-
-// Copies caller-sorted leaves (build_tree's optional arrays, src/hlbvh.hpp:910-922).
-template <typename T>
-__global__ __launch_bounds__(256) void leaves_from_sorted_kernel(
-    const T *__restrict__ sx, const T *__restrict__ sy, const T *__restrict__ sz,
-    const uint32_t *__restrict__ sid, const int8_t *__restrict__ sghost,
-    const int32_t *__restrict__ sorig, int n, T *mnx, T *mxx, T *mny, T *mxy, T *mnz, T *mxz,
-    uint32_t *tid_out, int8_t *tghost, int32_t *torig) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int leaf = i + n - 1;
-    const T a = sx[i], b = sy[i], c = sz[i];
-    mnx[leaf] = a; mxx[leaf] = a;
-    mny[leaf] = b; mxy[leaf] = b;
-    mnz[leaf] = c; mxz[leaf] = c;
-    if (sid) tid_out[leaf] = sid[i];
-    if (sghost) tghost[leaf] = sghost[i];
-    if (sorig) torig[leaf] = sorig[i];
-}
-
-// ===========================================================================
-// K11: Karras internal nodes (src/hlbvh.hpp:773-903).  delta(i,j) = -1 outside
-// [0,n); clz64 of differing keys; 64 + clz32(i^j) for equal keys (an
-// order-preserving offset of the reference's prefix lengths, so every
-// comparison -- and therefore the tree -- is the same).  Also records each
-// node's leaf range and clears the refit counter.
-// ===========================================================================
-__device__ inline int kdelta(const uint64_t *__restrict__ k, int n, int i, uint64_t ki, int j) {
-    if (j < 0 || j >= n) return -1;
-    const uint64_t kj = k[j];
-    if (ki != kj) return __clzll((long long)(ki ^ kj));
-    return 64 + __clz((int)((uint32_t)i ^ (uint32_t)j));
-}
-
-__global__ __launch_bounds__(256) void karras_kernel(const uint64_t *__restrict__ k, int n,
-                                                     int32_t *__restrict__ left,
-                                                     int32_t *__restrict__ right,
-                                                     int32_t *__restrict__ parent,
-                                                     int32_t *__restrict__ rfirst,
-                                                     int32_t *__restrict__ rlast,
-                                                     uint32_t *__restrict__ counters) {
-    const int i = xcd_block() * blockDim.x + threadIdx.x;  // searches read nearby keys
-    if (i >= n - 1) return;
-    const uint64_t ki = k[i];
-    const int dd = kdelta(k, n, i, ki, i + 1) - kdelta(k, n, i, ki, i - 1);
-    const int d = (dd > 0) - (dd < 0);
-    const int dmin = kdelta(k, n, i, ki, i - d);
-    int lmax = 2;
-    while (kdelta(k, n, i, ki, i + lmax * d) > dmin) lmax *= 2;
-    int l = 0;
-    for (int t = lmax / 2; t >= 1; t /= 2)
-        if (kdelta(k, n, i, ki, i + (l + t) * d) > dmin) l += t;
-    const int j = i + l * d;
-    const int dnode = kdelta(k, n, i, ki, j);
-    int s = 0, ts = 1;
-    while (ts <= l) ts *= 2;
-    ts /= 2;
-    for (int t = ts; t >= 1; t /= 2)
-        if (s + t < l && kdelta(k, n, i, ki, i + (s + t) * d) > dnode) s += t;
-    const int split = i + s * d + min(d, 0);
-    const int lo = min(i, j), hi = max(i, j);
-    const int li = (lo == split) ? split + n - 1 : split;
-    const int ri = (hi == split + 1) ? split + 1 + n - 1 : split + 1;
-    left[i] = li;
-    right[i] = ri;
-    parent[li] = i;
-    parent[ri] = i;
-    rfirst[i] = lo;
-    rlast[i] = hi;
-    counters[i] = 0;
-    if (i == 0) parent[0] = -1;
-}
-
-// ===========================================================================
-// K13: refit (src/hlbvh.hpp:925-969).  Each leaf climbs; the first arrival at a
-// node stops, the second combines its own box with the sibling's (fmin/fmax for
-// FP, </> for integers: sfc_min/sfc_max, :580-594) and continues.
+// Tree assembly: leaf gather (src/hlbvh.hpp:2266-2279), Karras topology
+// (src/hlbvh.hpp:773-903) and the bottom-up refit (src/hlbvh.hpp:925-969) in
+// one pass over the sorted leaves, plus a second pass for the few nodes that
+// span chunks.
 //
-// Cross-XCD visibility without whole-cache maintenance: internal boxes are
-// written with agent-scope stores (write-through to the device coherence
-// point), each climber waits for its own stores to complete (s_waitcnt
-// vmcnt(0)) before its counter increment, and the second arrival reads the
-// sibling with agent-scope loads issued after its increment returned.  This is
-// the agent-scope release/acquire pair minus the L2 write-back / invalidate,
-// which only non-atomic accesses need.
+// The topology is built bottom-up.  split_delta(i) is the common-prefix length
+// of leaves i and i+1 over (key, index): clz64 of differing keys, 64 +
+// clz32(i ^ (i+1)) for equal keys, -1 outside [0, n-1).  These are the prefix
+// lengths the reference's top-down search compares, so the binary radix tree
+// -- unique for distinct (key, index) strings -- is the same.  The parent of
+// the node over leaves [l, r] is the node split at r (this node is its left
+// child) when split_delta(r) > split_delta(l-1), else the node split at l-1
+// (right child).  Karras labels follow from that: the root is 0, a left child
+// is labelled by its last leaf and a right child by its first leaf, so a
+// node's label is known once its parent's side is.  Each node is identified
+// by its split until then.  A climber reaching a split posts its far end; the
+// second climber to arrive forms the parent from both ends, merges the boxes
+// (fmin/fmax for FP, </> for integers: sfc_min/sfc_max, :580-594) and goes on.
 // ===========================================================================
+__device__ inline int split_delta(const uint64_t *__restrict__ k, int n, int i) {
+    if (i < 0 || i + 1 >= n) return -1;
+    const uint64_t a = k[i], b = k[i + 1];
+    if (a != b) return __clzll((long long)(a ^ b));
+    return 64 + __clz((int)((uint32_t)i ^ (uint32_t)(i + 1)));
+}
+
 template <typename T> __device__ inline T smin(T a, T b) {
     if constexpr (CoordTraits<T>::fp) return ffmin(a, b);
     else return (a < b) ? a : b;
@@ -561,161 +479,241 @@ template <typename T> __device__ inline void st_agent(T *p, T v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-template <typename T>
-__global__ __launch_bounds__(256) void refit_kernel(int n, const int32_t *__restrict__ left,
-                                                    const int32_t *__restrict__ right,
-                                                    const int32_t *__restrict__ parent,
-                                                    uint32_t *__restrict__ counters, T *mnx,
-                                                    T *mxx, T *mny, T *mxy, T *mnz, T *mxz) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    int cur = i + n - 1;
-    T b0 = mnx[cur], b1 = mxx[cur], b2 = mny[cur], b3 = mxy[cur], b4 = mnz[cur], b5 = mxz[cur];
-    while (cur != 0) {
-        const int p = parent[cur];
-        if (p < 0 || p >= n - 1) break;
-        // This climber's box stores (previous iteration) complete before the increment.
-        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-        const uint32_t prev =
-            __hip_atomic_fetch_add(&counters[p], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (prev == 0) return;
-        asm volatile("" ::: "memory");
-        const int l = left[p], r = right[p];
-        const int sib = (l == cur) ? r : l;
-        b0 = smin(b0, ld_agent(mnx + sib)); b1 = smax(b1, ld_agent(mxx + sib));
-        b2 = smin(b2, ld_agent(mny + sib)); b3 = smax(b3, ld_agent(mxy + sib));
-        b4 = smin(b4, ld_agent(mnz + sib)); b5 = smax(b5, ld_agent(mxz + sib));
-        st_agent(mnx + p, b0); st_agent(mxx + p, b1);
-        st_agent(mny + p, b2); st_agent(mxy + p, b3);
-        st_agent(mnz + p, b4); st_agent(mxz + p, b5);
-        cur = p;
+// Where the leaves come from: the input particles through the sort
+// permutation (build_bvh), caller-sorted arrays (build_tree with leaves), or
+// the tree's own leaf slots (build_tree without them).
+enum { LEAF_GATHER = 0, LEAF_SORTED = 1, LEAF_INPLACE = 2 };
+
+template <typename T> struct LeafSrc {
+    const T *x, *y, *z;
+    const uint32_t *id;
+    const int8_t *ghost;
+    const int32_t *orig;    // LEAF_SORTED only
+    const uint32_t *perm;   // LEAF_GATHER only
+};
+
+template <typename T> struct TreeOut {
+    T *mnx, *mxx, *mny, *mxy, *mnz, *mxz;
+    int32_t *left, *right, *parent, *rfirst, *rlast;
+    uint32_t *id;
+    int8_t *ghost;
+    int32_t *orig;
+};
+
+// Writes internal node `lab` over leaves [l, r] split at g, with its box.
+template <typename T, bool AGENT>
+__device__ inline void write_node(const TreeOut<T> &o, int n, int lab, int l, int r, int g,
+                                  T b0, T b1, T b2, T b3, T b4, T b5) {
+    const int off = n - 1;
+    const int li = (l == g) ? off + g : g;
+    const int ri = (r == g + 1) ? off + g + 1 : g + 1;
+    o.left[lab] = li;
+    o.right[lab] = ri;
+    o.rfirst[lab] = l;
+    o.rlast[lab] = r;
+    o.parent[li] = lab;
+    o.parent[ri] = lab;
+    if constexpr (AGENT) {
+        st_agent(o.mnx + lab, b0); st_agent(o.mxx + lab, b1);
+        st_agent(o.mny + lab, b2); st_agent(o.mxy + lab, b3);
+        st_agent(o.mnz + lab, b4); st_agent(o.mxz + lab, b5);
+    } else {
+        o.mnx[lab] = b0; o.mxx[lab] = b1;
+        o.mny[lab] = b2; o.mxy[lab] = b3;
+        o.mnz[lab] = b4; o.mxz[lab] = b5;
     }
 }
 
-// Chunked refit, pass 1: workgroup w owns leaves [a, a + REFIT_CHUNK).  Every
-// internal node whose leaf range lies inside the chunk has its Karras label in
-// the chunk too (a node's label is one end of its range), so the chunk's
-// topology (parent / children / ranges of labels [a, a+C)), its leaf points,
-// the arrival counters and the boxes all sit in LDS, loaded and stored with
-// coalesced accesses; the climb synchronises at workgroup scope only.  A
-// climber stops below the first node whose range leaves the chunk and records
-// the node it completed (a "chunk top") for pass 2.
-constexpr int REFIT_CHUNK = 1024;
+// Appends v to the chunk-top list: one counter add per wave (the lanes active
+// at the call).
+__device__ inline void push_top(int32_t *tops, uint32_t *count, int v) {
+    const uint64_t m = __ballot(1);
+    const int lane = (int)__lane_id();
+    const int leader = __ffsll((long long)m) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(count, (uint32_t)__popcll(m));
+    base = (uint32_t)__shfl((int)base, leader);
+    tops[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = v;
+}
 
-template <typename T>
-__global__ __launch_bounds__(REFIT_CHUNK) void refit_chunk_kernel(
-    int n, const int32_t *__restrict__ left, const int32_t *__restrict__ right,
-    const int32_t *__restrict__ parent, const int32_t *__restrict__ rfirst,
-    const int32_t *__restrict__ rlast, T *mnx, T *mxx, T *mny, T *mxy, T *mnz, T *mxz,
-    int32_t *__restrict__ tops, uint32_t *__restrict__ top_count) {
-    constexpr int C = REFIT_CHUNK;
-    __shared__ T sbox[6][C];
+// Pass 1: workgroup w owns leaves [a, a + C).  Every node whose leaves all lie
+// in the chunk is split inside it, and its label lies in it too, so the
+// arrival words (one per split), the boxes (by label) and the leaf points sit
+// in LDS and the climbs synchronise at workgroup scope.  A climber whose
+// parent is split at a chunk edge records its node as a chunk top and stops.
+// Afterwards, a split inside the chunk reached only once has its other child
+// outside: the child that did arrive is a chunk top too.  Arrival word of a
+// split: arrivals << 16 | (far end - a + 1) summed.
+template <typename T, int LEAF, int C>
+__global__ __launch_bounds__(C) void assemble_kernel(int n, const uint64_t *__restrict__ k,
+                                                     LeafSrc<T> src, TreeOut<T> o,
+                                                     int32_t *__restrict__ tops,
+                                                     uint32_t *__restrict__ top_count) {
+    static_assert(C <= 1024, "arrival words hold chunk offsets in 16 bits");
     __shared__ T spt[3][C];
-    __shared__ int32_t s_pin[C], s_l[C], s_r[C], s_rf[C], s_rl[C], s_pleaf[C];
-    __shared__ uint32_t scnt[C];
-    const int a = blockIdx.x * C;
-    const int hi = min(a + C, n);  // leaves [a, hi)
+    __shared__ T sbox[6][C];
+    __shared__ int8_t sdel[C + 1];  // sdel[j] = split_delta(a - 1 + j), in [-1, 96]
+    __shared__ uint32_t sarr[C];
+    // XCD-ordered chunks: the gather's reads of one cluster go through one L2.
+    const int a = xcd_block() * C;
     const int t = threadIdx.x;
-    const int lab = a + t;
+    const int i = a + t;
     const int off = n - 1;
-    if (lab < n - 1) {
-        s_pin[t] = parent[lab];
-        s_l[t] = left[lab];
-        s_r[t] = right[lab];
-        s_rf[t] = rfirst[lab];
-        s_rl[t] = rlast[lab];
-    }
     T px = T(0), py = T(0), pz = T(0);
-    if (lab < n) {
-        s_pleaf[t] = parent[off + lab];
-        px = mnx[off + lab];
-        py = mny[off + lab];
-        pz = mnz[off + lab];
-        spt[0][t] = px;
-        spt[1][t] = py;
-        spt[2][t] = pz;
+    if (i < n) {
+        const int leaf = off + i;
+        if constexpr (LEAF == LEAF_GATHER) {
+            const uint32_t q = src.perm[i];
+            px = src.x[q]; py = src.y[q]; pz = src.z[q];
+            o.id[leaf] = src.id ? src.id[q] : q;
+            o.ghost[leaf] = src.ghost ? src.ghost[q] : (int8_t)0;
+            o.orig[leaf] = (int32_t)q;
+        } else if constexpr (LEAF == LEAF_SORTED) {
+            px = src.x[i]; py = src.y[i]; pz = src.z[i];
+            if (src.id) o.id[leaf] = src.id[i];
+            if (src.ghost) o.ghost[leaf] = src.ghost[i];
+            if (src.orig) o.orig[leaf] = src.orig[i];
+        } else {
+            px = o.mnx[leaf]; py = o.mny[leaf]; pz = o.mnz[leaf];
+        }
+        if constexpr (LEAF != LEAF_INPLACE) {
+            o.mnx[leaf] = px; o.mxx[leaf] = px;
+            o.mny[leaf] = py; o.mxy[leaf] = py;
+            o.mnz[leaf] = pz; o.mxz[leaf] = pz;
+        }
+        spt[0][t] = px; spt[1][t] = py; spt[2][t] = pz;
     }
-    scnt[t] = 0;
+    sdel[t + 1] = (int8_t)split_delta(k, n, i);
+    if (t == 0) sdel[0] = (int8_t)split_delta(k, n, a - 1);
+    sarr[t] = 0u;
     __syncthreads();
-    if (lab < n) {
-        int cur = off + lab;
+
+    if (i < n) {
+        int l = i, r = i, g = -1;
         T b0 = px, b1 = px, b2 = py, b3 = py, b4 = pz, b5 = pz;
         while (true) {
-            const int p = (cur >= off) ? s_pleaf[cur - off - a] : s_pin[cur - a];
-            if (p < 0) break;  // cur is the root
-            if (p < a || p >= hi || s_rf[p - a] < a || s_rl[p - a] >= hi) {
-                const uint32_t slot = atomicAdd(top_count, 1u);
-                tops[slot] = cur;
+            const bool root = (l == 0 && r == off);
+            const bool up_right = !root && sdel[r - a + 1] > sdel[l - a];
+            const int lab = (l == r) ? off + l : (root ? 0 : (up_right ? r : l));
+            if (l < r) {
+                write_node<T, false>(o, n, lab, l, r, g, b0, b1, b2, b3, b4, b5);
+                sbox[0][lab - a] = b0; sbox[1][lab - a] = b1; sbox[2][lab - a] = b2;
+                sbox[3][lab - a] = b3; sbox[4][lab - a] = b4; sbox[5][lab - a] = b5;
+            }
+            if (root) {
+                o.parent[0] = -1;
                 break;
             }
-            const int pl = p - a;
-            const uint32_t prev =
-                __hip_atomic_fetch_add(&scnt[pl], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
-            if (prev == 0) break;
-            const int sib = (s_l[pl] == cur) ? s_r[pl] : s_l[pl];
-            if (sib >= off) {  // a leaf of this chunk: its box is the particle
-                const int sl = sib - off - a;
-                const T x = spt[0][sl], y = spt[1][sl], z = spt[2][sl];
+            const int p = up_right ? r : l - 1;
+            if (p < a || p > a + C - 2) {
+                push_top(tops, top_count, lab);
+                break;
+            }
+            const uint32_t far = (uint32_t)((up_right ? l : r) - a + 1);
+            const uint32_t prev = __hip_atomic_fetch_add(&sarr[p - a], (1u << 16) | far,
+                                                         __ATOMIC_ACQ_REL,
+                                                         __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (prev == 0u) break;
+            const int sf = (int)(prev & 0xFFFFu) - 1 + a;
+            const bool sleaf = up_right ? (sf == p + 1) : (sf == p);
+            const int sj = (up_right ? p + 1 : p) - a;
+            if (sleaf) {
+                const T x = spt[0][sj], y = spt[1][sj], z = spt[2][sj];
                 b0 = smin(b0, x); b1 = smax(b1, x);
                 b2 = smin(b2, y); b3 = smax(b3, y);
                 b4 = smin(b4, z); b5 = smax(b5, z);
             } else {
-                const int sl = sib - a;
-                b0 = smin(b0, sbox[0][sl]); b1 = smax(b1, sbox[1][sl]);
-                b2 = smin(b2, sbox[2][sl]); b3 = smax(b3, sbox[3][sl]);
-                b4 = smin(b4, sbox[4][sl]); b5 = smax(b5, sbox[5][sl]);
+                b0 = smin(b0, sbox[0][sj]); b1 = smax(b1, sbox[1][sj]);
+                b2 = smin(b2, sbox[2][sj]); b3 = smax(b3, sbox[3][sj]);
+                b4 = smin(b4, sbox[4][sj]); b5 = smax(b5, sbox[5][sj]);
             }
-            sbox[0][pl] = b0; sbox[1][pl] = b1; sbox[2][pl] = b2;
-            sbox[3][pl] = b3; sbox[4][pl] = b4; sbox[5][pl] = b5;
-            cur = p;
+            if (up_right) r = sf;
+            else l = sf;
+            g = p;
         }
     }
     __syncthreads();
-    // Coalesced write-back of the nodes completed here (both children arrived).
-    if (lab < n - 1 && scnt[t] == 2u) {
-        mnx[lab] = sbox[0][t]; mxx[lab] = sbox[1][t];
-        mny[lab] = sbox[2][t]; mxy[lab] = sbox[3][t];
-        mnz[lab] = sbox[4][t]; mxz[lab] = sbox[5][t];
-    }
-}
-
-// Chunked refit, pass 2: the chunk tops climb through the nodes that span
-// chunks, with the agent-scope scheme of refit_kernel (write-through box
-// stores, store completion before the counter increment, agent-scope loads of
-// the sibling after it).  Boxes written by pass 1 are visible after the kernel
-// boundary.  A fixed grid walks the device-side list.
-template <typename T>
-__global__ __launch_bounds__(256) void refit_top_kernel(
-    int n, const int32_t *__restrict__ left, const int32_t *__restrict__ right,
-    const int32_t *__restrict__ parent, uint32_t *__restrict__ counters, T *mnx, T *mxx, T *mny,
-    T *mxy, T *mnz, T *mxz, const int32_t *__restrict__ tops, const uint32_t *__restrict__ top_count) {
-    const int count = (int)*top_count;
-    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < count; t += gridDim.x * blockDim.x) {
-        int cur = tops[t];
-        T b0 = ld_agent(mnx + cur), b1 = ld_agent(mxx + cur), b2 = ld_agent(mny + cur);
-        T b3 = ld_agent(mxy + cur), b4 = ld_agent(mnz + cur), b5 = ld_agent(mxz + cur);
-        while (cur != 0) {
-            const int p = parent[cur];
-            if (p < 0 || p >= n - 1) break;
-            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-            const uint32_t prev =
-                __hip_atomic_fetch_add(&counters[p], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (prev == 0) break;
-            asm volatile("" ::: "memory");
-            const int l = left[p], r = right[p];
-            const int sib = (l == cur) ? r : l;
-            b0 = smin(b0, ld_agent(mnx + sib)); b1 = smax(b1, ld_agent(mxx + sib));
-            b2 = smin(b2, ld_agent(mny + sib)); b3 = smax(b3, ld_agent(mxy + sib));
-            b4 = smin(b4, ld_agent(mnz + sib)); b5 = smax(b5, ld_agent(mxz + sib));
-            st_agent(mnx + p, b0); st_agent(mxx + p, b1);
-            st_agent(mny + p, b2); st_agent(mxy + p, b3);
-            st_agent(mnz + p, b4); st_agent(mxz + p, b5);
-            cur = p;
+    const int p = a + t;
+    if (t < C - 1 && p < off) {
+        const uint32_t v = sarr[t];
+        if ((v >> 16) == 1u) {
+            const int f = (int)(v & 0xFFFFu) - 1 + a;
+            const int lab = (f <= p) ? ((f == p) ? off + p : p) : ((f == p + 1) ? off + p + 1 : p + 1);
+            push_top(tops, top_count, lab);
         }
     }
 }
 
-__global__ void single_leaf_kernel(int32_t *parent) { parent[0] = -1; }
+// Pass 2: the chunk tops climb through the nodes that span chunks.  Arrival
+// words are global, one per split, tagged with a device-side build counter so
+// that no clearing pass is needed: word = epoch << 32 | far end, swapped in
+// atomically; a swapped-out word of an older epoch means "first".  Boxes are
+// written with agent-scope stores (write-through to the device coherence
+// point), each climber waits for its own stores (s_waitcnt vmcnt(0)) before
+// its swap, and the second arrival reads the sibling with agent-scope loads
+// after the swap returned -- the agent-scope release/acquire pair without the
+// L2 write-back / invalidate that only non-atomic accesses need.  Pass 1's
+// writes are visible after the kernel boundary.
+template <typename T>
+__global__ __launch_bounds__(256) void assemble_top_kernel(int n, const uint64_t *__restrict__ k,
+                                                           TreeOut<T> o,
+                                                           const int32_t *__restrict__ tops,
+                                                           const uint32_t *__restrict__ top_count,
+                                                           uint64_t *__restrict__ words,
+                                                           const uint32_t *__restrict__ epoch) {
+    const int count = (int)*top_count;
+    const uint64_t ep = (uint64_t)*epoch << 32;
+    const int off = n - 1;
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < count; q += gridDim.x * blockDim.x) {
+        const int lab0 = tops[q];
+        int l, r;
+        if (lab0 >= off) {
+            l = r = lab0 - off;
+        } else {
+            l = o.rfirst[lab0];
+            r = o.rlast[lab0];
+        }
+        T b0 = o.mnx[lab0], b1 = o.mxx[lab0], b2 = o.mny[lab0];
+        T b3 = o.mxy[lab0], b4 = o.mnz[lab0], b5 = o.mxz[lab0];
+        int dl = split_delta(k, n, l - 1), dr = split_delta(k, n, r);
+        while (!(l == 0 && r == off)) {
+            const bool up_right = dr > dl;
+            const int p = up_right ? r : l - 1;
+            const uint64_t mine = ep | (uint32_t)(up_right ? l : r);
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+            const uint64_t prev =
+                __hip_atomic_exchange(&words[p], mine, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if ((prev & 0xFFFFFFFF00000000ull) != ep) break;
+            asm volatile("" ::: "memory");
+            const int sf = (int)(uint32_t)prev;
+            const int sib = up_right ? ((sf == p + 1) ? off + p + 1 : p + 1)
+                                     : ((sf == p) ? off + p : p);
+            b0 = smin(b0, ld_agent(o.mnx + sib)); b1 = smax(b1, ld_agent(o.mxx + sib));
+            b2 = smin(b2, ld_agent(o.mny + sib)); b3 = smax(b3, ld_agent(o.mxy + sib));
+            b4 = smin(b4, ld_agent(o.mnz + sib)); b5 = smax(b5, ld_agent(o.mxz + sib));
+            if (up_right) r = sf;
+            else l = sf;
+            const bool root = (l == 0 && r == off);
+            int lab = 0;
+            if (!root) {
+                dl = split_delta(k, n, l - 1);
+                dr = split_delta(k, n, r);
+                lab = (dr > dl) ? r : l;
+            }
+            write_node<T, true>(o, n, lab, l, r, p, b0, b1, b2, b3, b4, b5);
+            if (root) o.parent[0] = -1;
+        }
+    }
+}
+
+// Starts a topology pass: clears the chunk-top count and advances the arrival
+// words' epoch.
+__global__ void topo_begin_kernel(uint32_t *top_count, uint32_t *epoch) {
+    if (threadIdx.x == 0) {
+        *top_count = 0u;
+        *epoch = *epoch + 1u;
+    }
+}
 
 // ===========================================================================
 // Host orchestration
@@ -744,13 +742,14 @@ static int scratch_ensure(Scratch &S, size_t n) {
     const size_t o_vals_a = off; off += al(cap * 4);
     const size_t o_vals_b = off; off += al(cap * 4);
     const size_t o_perm = off; off += al(cap * 4);
-    const size_t o_ctr = off; off += al(cap * 4);
+    const size_t o_words = off; off += al(cap * 8);
     const size_t o_tops = off; off += al(cap * 4);
     const size_t o_hist = off; off += al(RS_MAX_PASSES * 256 * 4 * 2);
     const size_t o_look = off; off += al(tiles * 256 * 8);
     const size_t o_tile = off; off += al(64);
     const size_t o_epoch = off; off += al(64);
     const size_t o_topc = off; off += al(64);
+    const size_t o_tepoch = off; off += al(64);
     const size_t o_bbox = off; off += al(16 * 8);
     const size_t o_part = off; off += al(1024 * 6 * 8);
     void *blk = nullptr;
@@ -767,9 +766,10 @@ static int scratch_ensure(Scratch &S, size_t n) {
     S.vals_a = (uint32_t *)(b + o_vals_a);
     S.vals_b = (uint32_t *)(b + o_vals_b);
     S.perm = (uint32_t *)(b + o_perm);
-    S.counters = (uint32_t *)(b + o_ctr);
+    S.split_words = (uint64_t *)(b + o_words);
     S.tops = (int32_t *)(b + o_tops);
     S.top_count = (uint32_t *)(b + o_topc);
+    S.topo_epoch = (uint32_t *)(b + o_tepoch);
     S.hist = (uint32_t *)(b + o_hist);
     S.lookback = (uint64_t *)(b + o_look);
     S.lookback_words = tiles * 256;
@@ -868,33 +868,32 @@ int radix_sort_u64(hipStream_t s, Scratch &S, const uint64_t *keys_in, size_t n,
     return radix_sort(s, S, keys_in, n, key_bits, sorted_out, false);
 }
 
-template <typename T> static int launch_topology(hipStream_t s, ft_tree *t, Scratch &S) {
+// Assembles the tree over t->sorted_keys: leaves from `src` (LEAF mode),
+// topology, ranges and boxes.  Pass 2 runs only when the tree spans chunks.
+constexpr int ASM_CHUNK = 1024;
+
+template <typename T, int LEAF>
+static int launch_assemble(hipStream_t s, ft_tree *t, Scratch &S, const LeafSrc<T> &src) {
     const int n = (int)t->num_leaves;
-    if (n == 1) {
-        hipLaunchKernelGGL(single_leaf_kernel, dim3(1), dim3(1), 0, s, t->parent);
+    if (n < 1) return FT_OK;
+    const TreeOut<T> o{(T *)t->min_x,      (T *)t->max_x,       (T *)t->min_y, (T *)t->max_y,
+                       (T *)t->min_z,      (T *)t->max_z,       t->left_child, t->right_child,
+                       t->parent,          t->range_first,      t->range_last, t->id,
+                       t->is_ghost,        t->orig_idx};
+    const uint64_t *keys = (const uint64_t *)t->sorted_keys;
+    hipLaunchKernelGGL(topo_begin_kernel, dim3(1), dim3(64), 0, s, S.top_count, S.topo_epoch);
+    const unsigned chunks = (unsigned)((n + ASM_CHUNK - 1) / ASM_CHUNK);
+    hipLaunchKernelGGL((assemble_kernel<T, LEAF, ASM_CHUNK>), dim3(xcd_grid(chunks)),
+                       dim3(ASM_CHUNK), 0, s, n, keys, src, o, S.tops, S.top_count);
+    FT_CHECK_LAUNCH();
+    prof_mark(S, s, 4);
+    if (chunks > 1) {
+        hipLaunchKernelGGL(assemble_top_kernel<T>, dim3(std::min(chunks * 4u, 2048u)), dim3(256),
+                           0, s, n, keys, o, (const int32_t *)S.tops,
+                           (const uint32_t *)S.top_count, S.split_words,
+                           (const uint32_t *)S.topo_epoch);
         FT_CHECK_LAUNCH();
-        return FT_OK;
     }
-    hipLaunchKernelGGL(karras_kernel, dim3(xcd_grid(grid_for(n - 1, 256))), dim3(256), 0, s,
-                       (const uint64_t *)t->sorted_keys, n, t->left_child, t->right_child,
-                       t->parent, t->range_first, t->range_last, S.counters);
-    FT_CHECK_LAUNCH();
-    prof_mark(S, s, 5);
-    const unsigned chunks = (unsigned)((n + REFIT_CHUNK - 1) / REFIT_CHUNK);
-    hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, S.top_count, 0u, (size_t)1);
-    hipLaunchKernelGGL(refit_chunk_kernel<T>, dim3(chunks), dim3(REFIT_CHUNK), 0, s, n,
-                       (const int32_t *)t->left_child, (const int32_t *)t->right_child,
-                       (const int32_t *)t->parent, (const int32_t *)t->range_first,
-                       (const int32_t *)t->range_last, (T *)t->min_x, (T *)t->max_x,
-                       (T *)t->min_y, (T *)t->max_y, (T *)t->min_z, (T *)t->max_z, S.tops,
-                       S.top_count);
-    FT_CHECK_LAUNCH();
-    hipLaunchKernelGGL(refit_top_kernel<T>, dim3(std::min(chunks * 4u, 2048u)), dim3(256), 0, s,
-                       n, (const int32_t *)t->left_child, (const int32_t *)t->right_child,
-                       (const int32_t *)t->parent, S.counters, (T *)t->min_x, (T *)t->max_x,
-                       (T *)t->min_y, (T *)t->max_y, (T *)t->min_z, (T *)t->max_z,
-                       (const int32_t *)S.tops, (const uint32_t *)S.top_count);
-    FT_CHECK_LAUNCH();
     return FT_OK;
 }
 
@@ -920,15 +919,10 @@ int build_bvh_t(hipStream_t s, ft_tree *t, Scratch &S, const T *x, const T *y, c
     prof_mark(S, s, 2);
     FT_TRY(radix_sort(s, S, kin, n, key_bits, t->sorted_keys, keys == nullptr));
     prof_mark(S, s, 3);
-    hipLaunchKernelGGL(gather_kernel<T>, dim3(xcd_grid(grid_for(n, 256))), dim3(256), 0, s, x, y, z, id,
-                       ghost, (const uint32_t *)S.perm, (int)n, (T *)t->min_x, (T *)t->max_x,
-                       (T *)t->min_y, (T *)t->max_y, (T *)t->min_z, (T *)t->max_z, t->id,
-                       t->is_ghost, t->orig_idx);
-    FT_CHECK_LAUNCH();
-    prof_mark(S, s, 4);
-    FT_TRY(launch_topology<T>(s, t, S));
-    prof_mark(S, s, 6);
-    if (S.profiling) S.n_stages = 6;
+    LeafSrc<T> src{x, y, z, id, ghost, nullptr, (const uint32_t *)S.perm};
+    FT_TRY((launch_assemble<T, LEAF_GATHER>(s, t, S, src)));
+    prof_mark(S, s, 5);
+    if (S.profiling) S.n_stages = 5;
     return FT_OK;
 }
 
@@ -1014,13 +1008,11 @@ int build_tree_t(hipStream_t s, ft_tree *t, Scratch &S, const uint64_t *sorted_k
     if (sorted_keys != t->sorted_keys)
         FT_HIP(hipMemcpyAsync(t->sorted_keys, sorted_keys, n * 8, hipMemcpyDeviceToDevice, s));
     if (sx && sy && sz) {
-        hipLaunchKernelGGL(leaves_from_sorted_kernel<T>, dim3(grid_for(n, 256)), dim3(256), 0, s,
-                           sx, sy, sz, sid, sghost, sorig, (int)n, (T *)t->min_x, (T *)t->max_x,
-                           (T *)t->min_y, (T *)t->max_y, (T *)t->min_z, (T *)t->max_z, t->id,
-                           t->is_ghost, t->orig_idx);
-        FT_CHECK_LAUNCH();
+        LeafSrc<T> src{sx, sy, sz, sid, sghost, sorig, nullptr};
+        return launch_assemble<T, LEAF_SORTED>(s, t, S, src);
     }
-    return launch_topology<T>(s, t, S);
+    LeafSrc<T> src{};
+    return launch_assemble<T, LEAF_INPLACE>(s, t, S, src);
 }
 template int build_tree_t<float>(hipStream_t, ft_tree *, Scratch &, const uint64_t *,
                                  const float *, const float *, const float *, const uint32_t *,

@@ -165,7 +165,8 @@ int ft_scratch_create(ft_scratch **scratch);
 int ft_scratch_destroy(ft_scratch *scratch);
 /* Stage timing of the next builds: on = record HIP events between build stages. */
 int ft_scratch_set_profiling(ft_scratch *scratch, int on);
-/* ms of the last profiled build per stage: bbox, encode, sort, gather, karras, refit. */
+/* ms of the last profiled build per stage: bbox, encode, sort, assemble (leaf gather,  */
+/* topology and in-chunk boxes), assemble_top (nodes spanning chunks).                  */
 int ft_scratch_stage_ms(ft_scratch *scratch, double *ms, int max_stages, int *n_stages);
 
 /* ---- build ------------------------------------------------------------- */
