@@ -538,12 +538,15 @@ __device__ inline void push_top(int32_t *tops, uint32_t *count, int v) {
 
 // Pass 1: workgroup w owns leaves [a, a + C).  Every node whose leaves all lie
 // in the chunk is split inside it, and its label lies in it too, so the
-// arrival words (one per split), the boxes (by label) and the leaf points sit
-// in LDS and the climbs synchronise at workgroup scope.  A climber whose
-// parent is split at a chunk edge records its node as a chunk top and stops.
-// Afterwards, a split inside the chunk reached only once has its other child
-// outside: the child that did arrive is a chunk top too.  Arrival word of a
-// split: arrivals << 16 | (far end - a + 1) summed.
+// arrival words (one per split), the boxes and node fields (by label), the
+// parents (by label and by leaf) and the leaf points sit in LDS; the climbs
+// synchronise at workgroup scope and the results leave in coalesced stores.
+// A climber whose parent is split at a chunk edge records its node as a chunk
+// top and stops.  Afterwards, a split inside the chunk reached only once has
+// its other child outside: the child that did arrive is a chunk top too.
+// Arrival word of a split: arrivals << 16 | (far end - a + 1) summed.
+constexpr int32_t NO_PARENT = INT32_MIN;  // parent not known inside the chunk
+
 template <typename T, int LEAF, int C>
 __global__ __launch_bounds__(C) void assemble_kernel(int n, const uint64_t *__restrict__ k,
                                                      LeafSrc<T> src, TreeOut<T> o,
@@ -552,8 +555,10 @@ __global__ __launch_bounds__(C) void assemble_kernel(int n, const uint64_t *__re
     static_assert(C <= 1024, "arrival words hold chunk offsets in 16 bits");
     __shared__ T spt[3][C];
     __shared__ T sbox[6][C];
-    __shared__ int8_t sdel[C + 1];  // sdel[j] = split_delta(a - 1 + j), in [-1, 96]
+    __shared__ int32_t sl[C], sr[C], sfirst[C], slast[C];  // by label - a; sfirst < 0: not here
+    __shared__ int32_t spar_in[C], spar_leaf[C];           // by label - a, by leaf - a
     __shared__ uint32_t sarr[C];
+    __shared__ int8_t sdel[C + 1];  // sdel[j] = split_delta(a - 1 + j), in [-1, 96]
     // XCD-ordered chunks: the gather's reads of one cluster go through one L2.
     const int a = xcd_block() * C;
     const int t = threadIdx.x;
@@ -586,6 +591,9 @@ __global__ __launch_bounds__(C) void assemble_kernel(int n, const uint64_t *__re
     sdel[t + 1] = (int8_t)split_delta(k, n, i);
     if (t == 0) sdel[0] = (int8_t)split_delta(k, n, a - 1);
     sarr[t] = 0u;
+    sfirst[t] = -1;
+    spar_in[t] = NO_PARENT;
+    spar_leaf[t] = NO_PARENT;
     __syncthreads();
 
     if (i < n) {
@@ -595,13 +603,22 @@ __global__ __launch_bounds__(C) void assemble_kernel(int n, const uint64_t *__re
             const bool root = (l == 0 && r == off);
             const bool up_right = !root && sdel[r - a + 1] > sdel[l - a];
             const int lab = (l == r) ? off + l : (root ? 0 : (up_right ? r : l));
-            if (l < r) {
-                write_node<T, false>(o, n, lab, l, r, g, b0, b1, b2, b3, b4, b5);
-                sbox[0][lab - a] = b0; sbox[1][lab - a] = b1; sbox[2][lab - a] = b2;
-                sbox[3][lab - a] = b3; sbox[4][lab - a] = b4; sbox[5][lab - a] = b5;
+            if (l < r) {  // this climber completed internal node `lab`, split at g
+                const int j = lab - a;
+                sl[j] = (l == g) ? off + g : g;
+                sr[j] = (r == g + 1) ? off + g + 1 : g + 1;
+                sfirst[j] = l;
+                slast[j] = r;
+                if (l == g) spar_leaf[g - a] = lab;
+                else spar_in[g - a] = lab;
+                if (r == g + 1) spar_leaf[g + 1 - a] = lab;
+                else spar_in[g + 1 - a] = lab;
+                sbox[0][j] = b0; sbox[1][j] = b1; sbox[2][j] = b2;
+                sbox[3][j] = b3; sbox[4][j] = b4; sbox[5][j] = b5;
             }
             if (root) {
-                o.parent[0] = -1;
+                if (l < r) spar_in[0] = -1;
+                else spar_leaf[0] = -1;  // a single leaf is the root
                 break;
             }
             const int p = up_right ? r : l - 1;
@@ -633,15 +650,31 @@ __global__ __launch_bounds__(C) void assemble_kernel(int n, const uint64_t *__re
         }
     }
     __syncthreads();
-    const int p = a + t;
-    if (t < C - 1 && p < off) {
-        const uint32_t v = sarr[t];
-        if ((v >> 16) == 1u) {
-            const int f = (int)(v & 0xFFFFu) - 1 + a;
-            const int lab = (f <= p) ? ((f == p) ? off + p : p) : ((f == p + 1) ? off + p + 1 : p + 1);
-            push_top(tops, top_count, lab);
+    // Chunk tops left by splits reached once.
+    {
+        const int p = a + t;
+        if (t < C - 1 && p < off) {
+            const uint32_t v = sarr[t];
+            if ((v >> 16) == 1u) {
+                const int f = (int)(v & 0xFFFFu) - 1 + a;
+                const int lab = (f <= p) ? ((f == p) ? off + p : p)
+                                         : ((f == p + 1) ? off + p + 1 : p + 1);
+                push_top(tops, top_count, lab);
+            }
         }
     }
+    // Coalesced write-back of what was completed here.
+    if (i < off && sfirst[t] >= 0) {
+        o.left[i] = sl[t];
+        o.right[i] = sr[t];
+        o.rfirst[i] = sfirst[t];
+        o.rlast[i] = slast[t];
+        o.mnx[i] = sbox[0][t]; o.mxx[i] = sbox[1][t];
+        o.mny[i] = sbox[2][t]; o.mxy[i] = sbox[3][t];
+        o.mnz[i] = sbox[4][t]; o.mxz[i] = sbox[5][t];
+    }
+    if (i < off && spar_in[t] != NO_PARENT) o.parent[i] = spar_in[t];
+    if (i < n && spar_leaf[t] != NO_PARENT) o.parent[off + i] = spar_leaf[t];
 }
 
 // Pass 2: the chunk tops climb through the nodes that span chunks.  Arrival
@@ -870,7 +903,7 @@ int radix_sort_u64(hipStream_t s, Scratch &S, const uint64_t *keys_in, size_t n,
 
 // Assembles the tree over t->sorted_keys: leaves from `src` (LEAF mode),
 // topology, ranges and boxes.  Pass 2 runs only when the tree spans chunks.
-constexpr int ASM_CHUNK = 1024;
+constexpr int ASM_CHUNK = 512;
 
 template <typename T, int LEAF>
 static int launch_assemble(hipStream_t s, ft_tree *t, Scratch &S, const LeafSrc<T> &src) {
