@@ -542,6 +542,9 @@ template <typename T> static int upload_tables(ft_batch *b) {
                          hipMemcpyHostToDevice));
     FT_HIP(hipMemset(b->dctr, 0, 4 * (V_COUNT * FT_XCDS + 1)));
     FT_HIP(hipMemcpy(b->dbnd, bnd.data(), 4 * bnd.size(), hipMemcpyHostToDevice));
+    // The clear and copies above go through the null stream; the step runs on
+    // non-blocking streams that do not wait for it.
+    FT_HIP(hipDeviceSynchronize());
     FT_TRY(scratch_ensure_public(b->sort_scratch, ne));
     b->dirty = false;
     return FT_OK;
@@ -831,6 +834,8 @@ template <typename T> static int tune_t(ft_batch *b, hipStream_t s) {
                         q.variant = v;
                         (void)hipMemcpy(b->dreqs + i, &q, sizeof q, hipMemcpyHostToDevice);
                     }
+                    // null-stream copies above, before the timed launches on s
+                    (void)hipDeviceSynchronize();
                     float t_best = 1e30f;
                     for (int rep = 0; rep < 2 && rc == FT_OK; ++rep) {
                         hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(128), 0, s, b->dctr,

@@ -699,6 +699,7 @@ __global__ __launch_bounds__(256) void assemble_top_kernel(int n, const uint64_t
     const int off = n - 1;
     for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < count; q += gridDim.x * blockDim.x) {
         const int lab0 = tops[q];
+        if (lab0 < 0 || lab0 > 2 * off) continue;  // never true for a list this build wrote
         int l, r;
         if (lab0 >= off) {
             l = r = lab0 - off;
@@ -706,6 +707,7 @@ __global__ __launch_bounds__(256) void assemble_top_kernel(int n, const uint64_t
             l = o.rfirst[lab0];
             r = o.rlast[lab0];
         }
+        if (l < 0 || r > off || l > r) continue;
         T b0 = o.mnx[lab0], b1 = o.mxx[lab0], b2 = o.mny[lab0];
         T b3 = o.mxy[lab0], b4 = o.mnz[lab0], b5 = o.mxz[lab0];
         int dl = split_delta(k, n, l - 1), dr = split_delta(k, n, r);
@@ -787,8 +789,12 @@ static int scratch_ensure(Scratch &S, size_t n) {
     const size_t o_part = off; off += al(1024 * 6 * 8);
     void *blk = nullptr;
     if (hipMalloc(&blk, off) != hipSuccess) return fail(FT_ENOMEM, "scratch allocation failed");
-    // One-time clear: look-back words carry an epoch, so they never need clearing again.
+    // One-time clear: look-back and arrival words carry an epoch, so they never
+    // need clearing again.  hipMemset runs on the null stream, which callers'
+    // non-blocking streams do not wait for: finish it before any of them can
+    // use the block.
     FT_HIP(hipMemset(blk, 0, off));
+    FT_HIP(hipDeviceSynchronize());
     char *b = (char *)blk;
     S.block = blk;
     S.block_bytes = off;

@@ -150,12 +150,16 @@ int ft_memcpy_d2h(void *dst, const void *src, size_t bytes, ft_stream s) {
     FT_HIP(hipStreamSynchronize(as_stream(s)));
     return FT_OK;
 }
+// Without a stream these run on the null stream, which non-blocking streams
+// do not wait for: they then complete before returning.
 int ft_memcpy_d2d(void *dst, const void *src, size_t bytes, ft_stream s) {
     FT_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, as_stream(s)));
+    if (!s) FT_HIP(hipStreamSynchronize(nullptr));
     return FT_OK;
 }
 int ft_memset(void *dst, int value, size_t bytes, ft_stream s) {
     FT_HIP(hipMemsetAsync(dst, value, bytes, as_stream(s)));
+    if (!s) FT_HIP(hipStreamSynchronize(nullptr));
     return FT_OK;
 }
 

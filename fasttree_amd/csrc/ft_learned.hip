@@ -567,6 +567,7 @@ int ft_mlp_create(ft_mlp **out) {
         return fail(FT_ENOMEM, "model allocation failed");
     }
     FT_HIP(hipMemset(mdl->block, 0, bytes));
+    FT_HIP(hipDeviceSynchronize());  // the null-stream clear, before any stream uses the block
     float *b = (float *)mdl->block;
     mdl->prm = b;
     mdl->grad = b + NPARAM;
@@ -595,6 +596,9 @@ int ft_mlp_set(ft_mlp *mdl, const float *params, const float *norm, double margi
         FT_HIP(hipMemset(mdl->m, 0, NPARAM * 4));
         FT_HIP(hipMemset(mdl->v, 0, NPARAM * 4));
         mdl->step = 0;
+    }
+    if (params || norm) {
+        FT_HIP(hipDeviceSynchronize());  // null-stream copies and clears, before stream work
     }
     return FT_OK;
 }

@@ -137,6 +137,8 @@ const char *ft_version(void);
 int ft_device_count(int *count);
 int ft_malloc(void **ptr, size_t bytes);
 int ft_free(void *ptr);
+/* h2d / d2h return once the copy is done.  d2d and memset are ordered on s; with */
+/* s = NULL they also complete before returning.                                 */
 int ft_memcpy_h2d(void *dst, const void *src, size_t bytes, ft_stream s);
 int ft_memcpy_d2h(void *dst, const void *src, size_t bytes, ft_stream s);
 int ft_memcpy_d2d(void *dst, const void *src, size_t bytes, ft_stream s);
