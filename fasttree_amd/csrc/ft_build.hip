@@ -479,6 +479,9 @@ template <typename T> __device__ inline void st_agent(T *p, T v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+constexpr int ASM_CHUNK = 1024;     // leaves per assembly workgroup
+constexpr int ASM_CHUNK_MIN = 256;  // sizes the per-chunk top counts in the scratch
+
 // Where the leaves come from: the input particles through the sort
 // permutation (build_bvh), caller-sorted arrays (build_tree with leaves), or
 // the tree's own leaf slots (build_tree without them).
@@ -524,43 +527,37 @@ __device__ inline void write_node(const TreeOut<T> &o, int n, int lab, int l, in
     }
 }
 
-// Appends v to the chunk-top list: one counter add per wave (the lanes active
-// at the call).
-__device__ inline void push_top(int32_t *tops, uint32_t *count, int v) {
-    const uint64_t m = __ballot(1);
-    const int lane = (int)__lane_id();
-    const int leader = __ffsll((long long)m) - 1;
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(count, (uint32_t)__popcll(m));
-    base = (uint32_t)__shfl((int)base, leader);
-    tops[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = v;
-}
-
 // Pass 1: workgroup w owns leaves [a, a + C).  Every node whose leaves all lie
 // in the chunk is split inside it, and its label lies in it too, so the
-// arrival words (one per split), the boxes and node fields (by label), the
-// parents (by label and by leaf) and the leaf points sit in LDS; the climbs
-// synchronise at workgroup scope and the results leave in coalesced stores.
+// arrival words (one per split), the boxes (by label) and the leaf points sit
+// in LDS and the climbs synchronise at workgroup scope.  STAGE: node fields
+// and parents are staged in LDS too and leave in coalesced stores; otherwise
+// each completed node is stored directly.
 // A climber whose parent is split at a chunk edge records its node as a chunk
 // top and stops.  Afterwards, a split inside the chunk reached only once has
-// its other child outside: the child that did arrive is a chunk top too.
+// its other child outside: the child that did arrive is a chunk top too.  The
+// chunk tops are disjoint subtrees of the chunk's leaves, so there are at most
+// as many as leaves: chunk c lists them in tops[c*C, c*C + count[c]).
 // Arrival word of a split: arrivals << 16 | (far end - a + 1) summed.
 constexpr int32_t NO_PARENT = INT32_MIN;  // parent not known inside the chunk
 
-template <typename T, int LEAF, int C>
+template <typename T, int LEAF, int C, bool STAGE>
 __global__ __launch_bounds__(C) void assemble_kernel(int n, const uint64_t *__restrict__ k,
                                                      LeafSrc<T> src, TreeOut<T> o,
                                                      int32_t *__restrict__ tops,
-                                                     uint32_t *__restrict__ top_count) {
+                                                     uint32_t *__restrict__ chunk_tops) {
     static_assert(C <= 1024, "arrival words hold chunk offsets in 16 bits");
+    constexpr int CS = STAGE ? C : 1;
     __shared__ T spt[3][C];
     __shared__ T sbox[6][C];
-    __shared__ int32_t sl[C], sr[C], sfirst[C], slast[C];  // by label - a; sfirst < 0: not here
-    __shared__ int32_t spar_in[C], spar_leaf[C];           // by label - a, by leaf - a
+    __shared__ int32_t sl[CS], sr[CS], sfirst[CS], slast[CS];  // by label - a; sfirst < 0: not here
+    __shared__ int32_t spar_in[CS], spar_leaf[CS];             // by label - a, by leaf - a
     __shared__ uint32_t sarr[C];
     __shared__ int8_t sdel[C + 1];  // sdel[j] = split_delta(a - 1 + j), in [-1, 96]
+    __shared__ uint32_t s_ntop;
     // XCD-ordered chunks: the gather's reads of one cluster go through one L2.
-    const int a = xcd_block() * C;
+    const int chunk = xcd_block();
+    const int a = chunk * C;
     const int t = threadIdx.x;
     const int i = a + t;
     const int off = n - 1;
@@ -589,11 +586,16 @@ __global__ __launch_bounds__(C) void assemble_kernel(int n, const uint64_t *__re
         spt[0][t] = px; spt[1][t] = py; spt[2][t] = pz;
     }
     sdel[t + 1] = (int8_t)split_delta(k, n, i);
-    if (t == 0) sdel[0] = (int8_t)split_delta(k, n, a - 1);
+    if (t == 0) {
+        sdel[0] = (int8_t)split_delta(k, n, a - 1);
+        s_ntop = 0u;
+    }
     sarr[t] = 0u;
-    sfirst[t] = -1;
-    spar_in[t] = NO_PARENT;
-    spar_leaf[t] = NO_PARENT;
+    if constexpr (STAGE) {
+        sfirst[t] = -1;
+        spar_in[t] = NO_PARENT;
+        spar_leaf[t] = NO_PARENT;
+    }
     __syncthreads();
 
     if (i < n) {
@@ -605,25 +607,33 @@ __global__ __launch_bounds__(C) void assemble_kernel(int n, const uint64_t *__re
             const int lab = (l == r) ? off + l : (root ? 0 : (up_right ? r : l));
             if (l < r) {  // this climber completed internal node `lab`, split at g
                 const int j = lab - a;
-                sl[j] = (l == g) ? off + g : g;
-                sr[j] = (r == g + 1) ? off + g + 1 : g + 1;
-                sfirst[j] = l;
-                slast[j] = r;
-                if (l == g) spar_leaf[g - a] = lab;
-                else spar_in[g - a] = lab;
-                if (r == g + 1) spar_leaf[g + 1 - a] = lab;
-                else spar_in[g + 1 - a] = lab;
+                if constexpr (STAGE) {
+                    sl[j] = (l == g) ? off + g : g;
+                    sr[j] = (r == g + 1) ? off + g + 1 : g + 1;
+                    sfirst[j] = l;
+                    slast[j] = r;
+                    if (l == g) spar_leaf[g - a] = lab;
+                    else spar_in[g - a] = lab;
+                    if (r == g + 1) spar_leaf[g + 1 - a] = lab;
+                    else spar_in[g + 1 - a] = lab;
+                } else {
+                    write_node<T, false>(o, n, lab, l, r, g, b0, b1, b2, b3, b4, b5);
+                }
                 sbox[0][j] = b0; sbox[1][j] = b1; sbox[2][j] = b2;
                 sbox[3][j] = b3; sbox[4][j] = b4; sbox[5][j] = b5;
             }
             if (root) {
-                if (l < r) spar_in[0] = -1;
-                else spar_leaf[0] = -1;  // a single leaf is the root
+                if constexpr (STAGE) {
+                    if (l < r) spar_in[0] = -1;
+                    else spar_leaf[0] = -1;  // a single leaf is the root
+                } else {
+                    o.parent[0] = -1;
+                }
                 break;
             }
             const int p = up_right ? r : l - 1;
             if (p < a || p > a + C - 2) {
-                push_top(tops, top_count, lab);
+                tops[a + (int)atomicAdd(&s_ntop, 1u)] = lab;
                 break;
             }
             const uint32_t far = (uint32_t)((up_right ? l : r) - a + 1);
@@ -659,22 +669,26 @@ __global__ __launch_bounds__(C) void assemble_kernel(int n, const uint64_t *__re
                 const int f = (int)(v & 0xFFFFu) - 1 + a;
                 const int lab = (f <= p) ? ((f == p) ? off + p : p)
                                          : ((f == p + 1) ? off + p + 1 : p + 1);
-                push_top(tops, top_count, lab);
+                tops[a + (int)atomicAdd(&s_ntop, 1u)] = lab;
             }
         }
     }
-    // Coalesced write-back of what was completed here.
-    if (i < off && sfirst[t] >= 0) {
-        o.left[i] = sl[t];
-        o.right[i] = sr[t];
-        o.rfirst[i] = sfirst[t];
-        o.rlast[i] = slast[t];
-        o.mnx[i] = sbox[0][t]; o.mxx[i] = sbox[1][t];
-        o.mny[i] = sbox[2][t]; o.mxy[i] = sbox[3][t];
-        o.mnz[i] = sbox[4][t]; o.mxz[i] = sbox[5][t];
+    if constexpr (STAGE) {
+        // Coalesced write-back of what was completed here.
+        if (i < off && sfirst[t] >= 0) {
+            o.left[i] = sl[t];
+            o.right[i] = sr[t];
+            o.rfirst[i] = sfirst[t];
+            o.rlast[i] = slast[t];
+            o.mnx[i] = sbox[0][t]; o.mxx[i] = sbox[1][t];
+            o.mny[i] = sbox[2][t]; o.mxy[i] = sbox[3][t];
+            o.mnz[i] = sbox[4][t]; o.mxz[i] = sbox[5][t];
+        }
+        if (i < off && spar_in[t] != NO_PARENT) o.parent[i] = spar_in[t];
+        if (i < n && spar_leaf[t] != NO_PARENT) o.parent[off + i] = spar_leaf[t];
     }
-    if (i < off && spar_in[t] != NO_PARENT) o.parent[i] = spar_in[t];
-    if (i < n && spar_leaf[t] != NO_PARENT) o.parent[off + i] = spar_leaf[t];
+    __syncthreads();
+    if (t == 0 && a < n) chunk_tops[chunk] = s_ntop;
 }
 
 // Pass 2: the chunk tops climb through the nodes that span chunks.  Arrival
@@ -691,14 +705,18 @@ template <typename T>
 __global__ __launch_bounds__(256) void assemble_top_kernel(int n, const uint64_t *__restrict__ k,
                                                            TreeOut<T> o,
                                                            const int32_t *__restrict__ tops,
-                                                           const uint32_t *__restrict__ top_count,
+                                                           const uint32_t *__restrict__ chunk_tops,
+                                                           int chunks, int chunk_size,
                                                            uint64_t *__restrict__ words,
                                                            const uint32_t *__restrict__ epoch) {
-    const int count = (int)*top_count;
     const uint64_t ep = (uint64_t)*epoch << 32;
     const int off = n - 1;
-    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < count; q += gridDim.x * blockDim.x) {
-        const int lab0 = tops[q];
+    const int lane = threadIdx.x & 63;
+    const int waves = gridDim.x * (blockDim.x >> 6);
+    // one wave per chunk, one lane per chunk top
+    for (int c = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); c < chunks; c += waves)
+    for (int q = lane; q < (int)chunk_tops[c]; q += 64) {
+        const int lab0 = tops[c * chunk_size + q];
         if (lab0 < 0 || lab0 > 2 * off) continue;  // never true for a list this build wrote
         int l, r;
         if (lab0 >= off) {
@@ -741,13 +759,9 @@ __global__ __launch_bounds__(256) void assemble_top_kernel(int n, const uint64_t
     }
 }
 
-// Starts a topology pass: clears the chunk-top count and advances the arrival
-// words' epoch.
-__global__ void topo_begin_kernel(uint32_t *top_count, uint32_t *epoch) {
-    if (threadIdx.x == 0) {
-        *top_count = 0u;
-        *epoch = *epoch + 1u;
-    }
+// Starts a topology pass: advances the arrival words' epoch.
+__global__ void topo_begin_kernel(uint32_t *epoch) {
+    if (threadIdx.x == 0) *epoch = *epoch + 1u;
 }
 
 // ===========================================================================
@@ -783,7 +797,7 @@ static int scratch_ensure(Scratch &S, size_t n) {
     const size_t o_look = off; off += al(tiles * 256 * 8);
     const size_t o_tile = off; off += al(64);
     const size_t o_epoch = off; off += al(64);
-    const size_t o_topc = off; off += al(64);
+    const size_t o_ctops = off; off += al(4 * (cap / ASM_CHUNK_MIN + 2));
     const size_t o_tepoch = off; off += al(64);
     const size_t o_bbox = off; off += al(16 * 8);
     const size_t o_part = off; off += al(1024 * 6 * 8);
@@ -807,7 +821,7 @@ static int scratch_ensure(Scratch &S, size_t n) {
     S.perm = (uint32_t *)(b + o_perm);
     S.split_words = (uint64_t *)(b + o_words);
     S.tops = (int32_t *)(b + o_tops);
-    S.top_count = (uint32_t *)(b + o_topc);
+    S.chunk_tops = (uint32_t *)(b + o_ctops);
     S.topo_epoch = (uint32_t *)(b + o_tepoch);
     S.hist = (uint32_t *)(b + o_hist);
     S.lookback = (uint64_t *)(b + o_look);
@@ -909,8 +923,8 @@ int radix_sort_u64(hipStream_t s, Scratch &S, const uint64_t *keys_in, size_t n,
 
 // Assembles the tree over t->sorted_keys: leaves from `src` (LEAF mode),
 // topology, ranges and boxes.  Pass 2 runs only when the tree spans chunks.
-constexpr int ASM_CHUNK = 512;
-
+// 1024-leaf chunks: LDS staging of the node fields when the whole chunk still
+// fits twice per CU (4-byte coordinates), direct stores otherwise.
 template <typename T, int LEAF>
 static int launch_assemble(hipStream_t s, ft_tree *t, Scratch &S, const LeafSrc<T> &src) {
     const int n = (int)t->num_leaves;
@@ -920,17 +934,18 @@ static int launch_assemble(hipStream_t s, ft_tree *t, Scratch &S, const LeafSrc<
                        t->parent,          t->range_first,      t->range_last, t->id,
                        t->is_ghost,        t->orig_idx};
     const uint64_t *keys = (const uint64_t *)t->sorted_keys;
-    hipLaunchKernelGGL(topo_begin_kernel, dim3(1), dim3(64), 0, s, S.top_count, S.topo_epoch);
-    const unsigned chunks = (unsigned)((n + ASM_CHUNK - 1) / ASM_CHUNK);
-    hipLaunchKernelGGL((assemble_kernel<T, LEAF, ASM_CHUNK>), dim3(xcd_grid(chunks)),
-                       dim3(ASM_CHUNK), 0, s, n, keys, src, o, S.tops, S.top_count);
+    constexpr int C = ASM_CHUNK;
+    constexpr bool STAGE = sizeof(T) <= 4;
+    hipLaunchKernelGGL(topo_begin_kernel, dim3(1), dim3(64), 0, s, S.topo_epoch);
+    const unsigned chunks = (unsigned)((n + C - 1) / C);
+    hipLaunchKernelGGL((assemble_kernel<T, LEAF, C, STAGE>), dim3(xcd_grid(chunks)), dim3(C), 0, s,
+                       n, keys, src, o, S.tops, S.chunk_tops);
     FT_CHECK_LAUNCH();
     prof_mark(S, s, 4);
     if (chunks > 1) {
-        hipLaunchKernelGGL(assemble_top_kernel<T>, dim3(std::min(chunks * 4u, 2048u)), dim3(256),
-                           0, s, n, keys, o, (const int32_t *)S.tops,
-                           (const uint32_t *)S.top_count, S.split_words,
-                           (const uint32_t *)S.topo_epoch);
+        hipLaunchKernelGGL(assemble_top_kernel<T>, dim3((chunks + 3) / 4), dim3(256), 0, s, n, keys,
+                           o, (const int32_t *)S.tops, (const uint32_t *)S.chunk_tops, (int)chunks,
+                           C, S.split_words, (const uint32_t *)S.topo_epoch);
         FT_CHECK_LAUNCH();
     }
     return FT_OK;

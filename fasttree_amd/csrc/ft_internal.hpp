@@ -282,8 +282,8 @@ struct Scratch {
     uint32_t *vals_a = nullptr, *vals_b = nullptr;
     uint32_t *perm = nullptr;     // final permutation (n)
     uint64_t *split_words = nullptr; // tree assembly: arrival word per split (n), epoch-tagged
-    int32_t *tops = nullptr;         // tree assembly: labels of the chunk tops (n)
-    uint32_t *top_count = nullptr;
+    int32_t *tops = nullptr;         // tree assembly: labels of the chunk tops, by chunk (n)
+    uint32_t *chunk_tops = nullptr;  // tree assembly: chunk tops per chunk
     uint32_t *topo_epoch = nullptr;  // tree assembly counter: tags the arrival words
     uint32_t *hist = nullptr;     // 8 passes x 256
     uint64_t *lookback = nullptr; // tiles x 256 status words
