@@ -243,8 +243,8 @@ __global__ __launch_bounds__(256) void encode_kernel(const T *__restrict__ x,
 // ===========================================================================
 constexpr int RS_BLOCK = 256;
 constexpr int RS_WAVES = RS_BLOCK / 64;
-constexpr int RS_ITEMS = 12;                   // keys per thread
-constexpr int RS_TILE = RS_BLOCK * RS_ITEMS;   // 3072 keys per tile
+constexpr int RS_ITEMS = 24;                   // keys per thread
+constexpr int RS_TILE = RS_BLOCK * RS_ITEMS;   // 6144 keys per tile
 constexpr int RS_TILE_MIN = RS_BLOCK * 8;      // smallest tile: sizes the look-back words
 constexpr int RS_MAX_PASSES = 8;
 
@@ -479,7 +479,7 @@ template <typename T> __device__ inline void st_agent(T *p, T v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-constexpr int ASM_CHUNK = 1024;     // leaves per assembly workgroup
+constexpr int ASM_CHUNK = 512;      // leaves per assembly workgroup
 constexpr int ASM_CHUNK_MIN = 256;  // sizes the per-chunk top counts in the scratch
 
 // Where the leaves come from: the input particles through the sort
@@ -503,8 +503,9 @@ template <typename T> struct TreeOut {
     int32_t *orig;
 };
 
-// Writes internal node `lab` over leaves [l, r] split at g, with its box.
-template <typename T, bool AGENT>
+// Writes internal node `lab` over leaves [l, r] split at g, with its box
+// (agent-scope stores: read by later climbers of the same launch).
+template <typename T>
 __device__ inline void write_node(const TreeOut<T> &o, int n, int lab, int l, int r, int g,
                                   T b0, T b1, T b2, T b3, T b4, T b5) {
     const int off = n - 1;
@@ -516,23 +517,16 @@ __device__ inline void write_node(const TreeOut<T> &o, int n, int lab, int l, in
     o.rlast[lab] = r;
     o.parent[li] = lab;
     o.parent[ri] = lab;
-    if constexpr (AGENT) {
-        st_agent(o.mnx + lab, b0); st_agent(o.mxx + lab, b1);
-        st_agent(o.mny + lab, b2); st_agent(o.mxy + lab, b3);
-        st_agent(o.mnz + lab, b4); st_agent(o.mxz + lab, b5);
-    } else {
-        o.mnx[lab] = b0; o.mxx[lab] = b1;
-        o.mny[lab] = b2; o.mxy[lab] = b3;
-        o.mnz[lab] = b4; o.mxz[lab] = b5;
-    }
+    st_agent(o.mnx + lab, b0); st_agent(o.mxx + lab, b1);
+    st_agent(o.mny + lab, b2); st_agent(o.mxy + lab, b3);
+    st_agent(o.mnz + lab, b4); st_agent(o.mxz + lab, b5);
 }
 
 // Pass 1: workgroup w owns leaves [a, a + C).  Every node whose leaves all lie
 // in the chunk is split inside it, and its label lies in it too, so the
 // arrival words (one per split), the boxes (by label) and the leaf points sit
-// in LDS and the climbs synchronise at workgroup scope.  STAGE: node fields
-// and parents are staged in LDS too and leave in coalesced stores; otherwise
-// each completed node is stored directly.
+// in LDS and the climbs synchronise at workgroup scope; node fields and
+// parents are staged in LDS too and leave in coalesced stores.
 // A climber whose parent is split at a chunk edge records its node as a chunk
 // top and stops.  Afterwards, a split inside the chunk reached only once has
 // its other child outside: the child that did arrive is a chunk top too.  The
@@ -541,17 +535,16 @@ __device__ inline void write_node(const TreeOut<T> &o, int n, int lab, int l, in
 // Arrival word of a split: arrivals << 16 | (far end - a + 1) summed.
 constexpr int32_t NO_PARENT = INT32_MIN;  // parent not known inside the chunk
 
-template <typename T, int LEAF, int C, bool STAGE>
+template <typename T, int LEAF, int C>
 __global__ __launch_bounds__(C) void assemble_kernel(int n, const uint64_t *__restrict__ k,
                                                      LeafSrc<T> src, TreeOut<T> o,
                                                      int32_t *__restrict__ tops,
                                                      uint32_t *__restrict__ chunk_tops) {
     static_assert(C <= 1024, "arrival words hold chunk offsets in 16 bits");
-    constexpr int CS = STAGE ? C : 1;
     __shared__ T spt[3][C];
     __shared__ T sbox[6][C];
-    __shared__ int32_t sl[CS], sr[CS], sfirst[CS], slast[CS];  // by label - a; sfirst < 0: not here
-    __shared__ int32_t spar_in[CS], spar_leaf[CS];             // by label - a, by leaf - a
+    __shared__ int32_t sl[C], sr[C], sfirst[C], slast[C];  // by label - a; sfirst < 0: not here
+    __shared__ int32_t spar_in[C], spar_leaf[C];           // by label - a, by leaf - a
     __shared__ uint32_t sarr[C];
     __shared__ int8_t sdel[C + 1];  // sdel[j] = split_delta(a - 1 + j), in [-1, 96]
     __shared__ uint32_t s_ntop;
@@ -591,11 +584,9 @@ __global__ __launch_bounds__(C) void assemble_kernel(int n, const uint64_t *__re
         s_ntop = 0u;
     }
     sarr[t] = 0u;
-    if constexpr (STAGE) {
-        sfirst[t] = -1;
-        spar_in[t] = NO_PARENT;
-        spar_leaf[t] = NO_PARENT;
-    }
+    sfirst[t] = -1;
+    spar_in[t] = NO_PARENT;
+    spar_leaf[t] = NO_PARENT;
     __syncthreads();
 
     if (i < n) {
@@ -607,28 +598,20 @@ __global__ __launch_bounds__(C) void assemble_kernel(int n, const uint64_t *__re
             const int lab = (l == r) ? off + l : (root ? 0 : (up_right ? r : l));
             if (l < r) {  // this climber completed internal node `lab`, split at g
                 const int j = lab - a;
-                if constexpr (STAGE) {
-                    sl[j] = (l == g) ? off + g : g;
-                    sr[j] = (r == g + 1) ? off + g + 1 : g + 1;
-                    sfirst[j] = l;
-                    slast[j] = r;
-                    if (l == g) spar_leaf[g - a] = lab;
-                    else spar_in[g - a] = lab;
-                    if (r == g + 1) spar_leaf[g + 1 - a] = lab;
-                    else spar_in[g + 1 - a] = lab;
-                } else {
-                    write_node<T, false>(o, n, lab, l, r, g, b0, b1, b2, b3, b4, b5);
-                }
+                sl[j] = (l == g) ? off + g : g;
+                sr[j] = (r == g + 1) ? off + g + 1 : g + 1;
+                sfirst[j] = l;
+                slast[j] = r;
+                if (l == g) spar_leaf[g - a] = lab;
+                else spar_in[g - a] = lab;
+                if (r == g + 1) spar_leaf[g + 1 - a] = lab;
+                else spar_in[g + 1 - a] = lab;
                 sbox[0][j] = b0; sbox[1][j] = b1; sbox[2][j] = b2;
                 sbox[3][j] = b3; sbox[4][j] = b4; sbox[5][j] = b5;
             }
             if (root) {
-                if constexpr (STAGE) {
-                    if (l < r) spar_in[0] = -1;
-                    else spar_leaf[0] = -1;  // a single leaf is the root
-                } else {
-                    o.parent[0] = -1;
-                }
+                if (l < r) spar_in[0] = -1;
+                else spar_leaf[0] = -1;  // a single leaf is the root
                 break;
             }
             const int p = up_right ? r : l - 1;
@@ -673,20 +656,18 @@ __global__ __launch_bounds__(C) void assemble_kernel(int n, const uint64_t *__re
             }
         }
     }
-    if constexpr (STAGE) {
-        // Coalesced write-back of what was completed here.
-        if (i < off && sfirst[t] >= 0) {
-            o.left[i] = sl[t];
-            o.right[i] = sr[t];
-            o.rfirst[i] = sfirst[t];
-            o.rlast[i] = slast[t];
-            o.mnx[i] = sbox[0][t]; o.mxx[i] = sbox[1][t];
-            o.mny[i] = sbox[2][t]; o.mxy[i] = sbox[3][t];
-            o.mnz[i] = sbox[4][t]; o.mxz[i] = sbox[5][t];
-        }
-        if (i < off && spar_in[t] != NO_PARENT) o.parent[i] = spar_in[t];
-        if (i < n && spar_leaf[t] != NO_PARENT) o.parent[off + i] = spar_leaf[t];
+    // Coalesced write-back of what was completed here.
+    if (i < off && sfirst[t] >= 0) {
+        o.left[i] = sl[t];
+        o.right[i] = sr[t];
+        o.rfirst[i] = sfirst[t];
+        o.rlast[i] = slast[t];
+        o.mnx[i] = sbox[0][t]; o.mxx[i] = sbox[1][t];
+        o.mny[i] = sbox[2][t]; o.mxy[i] = sbox[3][t];
+        o.mnz[i] = sbox[4][t]; o.mxz[i] = sbox[5][t];
     }
+    if (i < off && spar_in[t] != NO_PARENT) o.parent[i] = spar_in[t];
+    if (i < n && spar_leaf[t] != NO_PARENT) o.parent[off + i] = spar_leaf[t];
     __syncthreads();
     if (t == 0 && a < n) chunk_tops[chunk] = s_ntop;
 }
@@ -753,7 +734,7 @@ __global__ __launch_bounds__(256) void assemble_top_kernel(int n, const uint64_t
                 dr = split_delta(k, n, r);
                 lab = (dr > dl) ? r : l;
             }
-            write_node<T, true>(o, n, lab, l, r, p, b0, b1, b2, b3, b4, b5);
+            write_node<T>(o, n, lab, l, r, p, b0, b1, b2, b3, b4, b5);
             if (root) o.parent[0] = -1;
         }
     }
@@ -923,8 +904,7 @@ int radix_sort_u64(hipStream_t s, Scratch &S, const uint64_t *keys_in, size_t n,
 
 // Assembles the tree over t->sorted_keys: leaves from `src` (LEAF mode),
 // topology, ranges and boxes.  Pass 2 runs only when the tree spans chunks.
-// 1024-leaf chunks: LDS staging of the node fields when the whole chunk still
-// fits twice per CU (4-byte coordinates), direct stores otherwise.
+// 512-leaf chunks: three workgroups per CU with 8-byte coordinates.
 template <typename T, int LEAF>
 static int launch_assemble(hipStream_t s, ft_tree *t, Scratch &S, const LeafSrc<T> &src) {
     const int n = (int)t->num_leaves;
@@ -935,10 +915,9 @@ static int launch_assemble(hipStream_t s, ft_tree *t, Scratch &S, const LeafSrc<
                        t->is_ghost,        t->orig_idx};
     const uint64_t *keys = (const uint64_t *)t->sorted_keys;
     constexpr int C = ASM_CHUNK;
-    constexpr bool STAGE = sizeof(T) <= 4;
     hipLaunchKernelGGL(topo_begin_kernel, dim3(1), dim3(64), 0, s, S.topo_epoch);
     const unsigned chunks = (unsigned)((n + C - 1) / C);
-    hipLaunchKernelGGL((assemble_kernel<T, LEAF, C, STAGE>), dim3(xcd_grid(chunks)), dim3(C), 0, s,
+    hipLaunchKernelGGL((assemble_kernel<T, LEAF, C>), dim3(xcd_grid(chunks)), dim3(C), 0, s,
                        n, keys, src, o, S.tops, S.chunk_tops);
     FT_CHECK_LAUNCH();
     prof_mark(S, s, 4);
