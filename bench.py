@@ -171,14 +171,20 @@ def main():
         same = all(np.array_equal(res.to_host_range(r * k, k), res2.to_host_range(r * k, k)) and
                    np.array_equal(dst.to_host_range(r * k, k), dst2.to_host_range(r * k, k))
                    for r in rows)
+        nodes_seeded = float(ls[0])  # the stats of the last repetition
         learned = {"learned_ms": round(seed_ms + trav_ms, 4), "seed_ms": round(seed_ms, 4),
                    "seeded_traversal_ms": round(trav_ms, 4),
                    "rerun_fraction": round(float(ls[3]) / n, 5),
+                   "inserts_per_query": round(float(ls[4]) / n, 2),
+                   "nodes_per_wave": round(nodes_seeded / ((n + 63) // 64), 2),
+                   "leaves_per_wave": round(float(ls[1]) / ((n + 63) // 64), 2),
+                   "pruned_node_fraction": round(1.0 - nodes_seeded / max(float(st[0]), 1.0), 4),
                    "identical_on_2000_sampled_rows": bool(same),
                    "model": os.path.relpath(args.model, ROOT)}
         ladder["learned_qps"] = n / ((seed_ms + trav_ms) * 1e-3)
-        log(f"learned: seed {seed_ms:.2f} ms + traversal {trav_ms:.2f} ms, reruns {ls[3]}, "
-            f"identical sample {same}")
+        log(f"learned: seed {seed_ms:.2f} ms + traversal {trav_ms:.2f} ms, reruns "
+            f"{learned['rerun_fraction']:.4f}, inserts/q {learned['inserts_per_query']}, "
+            f"nodes/wave {learned['nodes_per_wave']}, identical sample {same}")
         del res2, dst2, feats, seeds
     if args.sweep_buckets:
         for tk in (ft.TOPK_REGISTERS, ft.TOPK_LDS_HEAP):

@@ -315,12 +315,12 @@ __global__ __launch_bounds__(64) void packet_knn_kernel(
     // Self queries over all leaves: the packet is leaf ranks [r0, r0 + 64), so
     // its neighbours along the curve are the leaves on either side.
     int wlo = 0, whi = 0;
+    const int r0 = blk * 64;
     if (a.window > 0) {
-        const int r0 = blk * 64;
         wlo = r0 - a.window > 0 ? r0 - a.window : 0;
         whi = r0 + 64 + a.window < t.n ? r0 + 64 + a.window : t.n;
     }
-    packet_walk<T, H, MK>(t, a.bucket, active, px, py, pz, self, h, stk, c, wlo, whi);
+    packet_walk<T, H, MK>(t, a.bucket, active, px, py, pz, self, h, stk, c, wlo, whi, r0);
     if (active) {
         bool redo = false;
         if constexpr (PACKED) redo = h.needs_exact(k);

@@ -120,6 +120,47 @@ __host__ __device__ __forceinline__ bool wrap_filter_near(uint32_t qx, uint32_t 
     return d <= wf;
 }
 
+// Pre-shifted form of the integer periodic metric (B <= 31): with a' = a << sh
+// and b' = b << sh (32-bit, sh = 32 - B), (int32)(a' - b') is the wrapped
+// difference of a and b times 2^sh, so no per-pair shifts are needed.  Its
+// magnitude, shifted back, is wrap_abs32(a, b, sh) exactly (2^(B-1) included,
+// read as unsigned).
+__host__ __device__ __forceinline__ uint64_t wrap_gap_pre(uint32_t a, uint32_t b, int sh) {
+    const int32_t x = (int32_t)(a - b);
+    const uint32_t u = (uint32_t)x;
+    return (uint64_t)((x < 0 ? 0u - u : u) >> sh);
+}
+// The float32 filter on pre-shifted coordinates: every value is the unshifted
+// one times 2^sh (a power of two, so the conversions and the error bound of
+// wrap_filter_near are unchanged) and the squared bound is scaled by 2^(2 sh).
+__host__ __device__ __forceinline__ float wrap_filter_bound_pre(uint64_t w, int sh) {
+    return wrap_filter_bound(w) * (float)(1ull << (2 * sh));
+}
+// Float32 squared distance of a leaf, pre-shifted coordinates (scaled by
+// 2^(2 sh)); compared with wrap_filter_bound_pre it never rejects a leaf whose
+// exact squared distance is within the bound.
+__host__ __device__ __forceinline__ float wrap_sq_pre(uint32_t qx, uint32_t qy, uint32_t qz,
+                                                      uint32_t x, uint32_t y, uint32_t z) {
+    const float fx = (float)(int32_t)(qx - x);
+    const float fy = (float)(int32_t)(qy - y);
+    const float fz = (float)(int32_t)(qz - z);
+    float d = fx * fx;
+    d = __builtin_fmaf(fy, fy, d);
+    d = __builtin_fmaf(fz, fz, d);
+    return d;
+}
+// One axis of a box's float32 gap (scaled by 2^sh): zero inside [lo, hi],
+// tested on the full values with the query masked to B bits (qm), as in
+// box_d2_k; else the smaller wrapped gap to either face from pre-shifted
+// coordinates (qs = query << sh).
+template <typename T>
+__host__ __device__ __forceinline__ float wrap_box_gap_pre(T qm, uint32_t qs, T lo, T hi, int sh) {
+    const bool in = (qm >= lo) && (qm <= hi);
+    const float a = __builtin_fabsf((float)(int32_t)(qs - ((uint32_t)lo << sh)));
+    const float b = __builtin_fabsf((float)(int32_t)(qs - ((uint32_t)hi << sh)));
+    return in ? 0.0f : __builtin_fminf(a, b);
+}
+
 // leaf_d2 / box_d2 with the metric kind fixed (same values).
 template <typename T, int MK>
 __device__ __forceinline__ typename CoordTraits<T>::D leaf_d2_k(const Metric &m, T px, T py, T pz,
@@ -431,7 +472,8 @@ struct PacketCounters {
 template <typename T, typename H, int MK>
 __device__ __forceinline__ void packet_walk(const DTree<T> &t, int bucket, bool active, T px, T py,
                                             T pz, int self, H &h, int32_t *stk,
-                                            PacketCounters &c, int win_lo = 0, int win_hi = 0) {
+                                            PacketCounters &c, int win_lo = 0, int win_hi = 0,
+                                            int win_mid = 0) {
     using D = typename CoordTraits<T>::D;
     const int lane = threadIdx.x & 63;
     const int n = t.n;
@@ -439,6 +481,46 @@ __device__ __forceinline__ void packet_walk(const DTree<T> &t, int bucket, bool 
     const T *lx = t.mnx + off;
     const T *ly = t.mny + off;
     const T *lz = t.mnz + off;
+    // Integer periodic metric: pre-shifted 32-bit query (wrap_gap_pre) and the
+    // query masked to B bits for the exact inside-the-box tests.
+    constexpr bool PRE = !CoordTraits<T>::fp && MK == MK_WRAP;
+    [[maybe_unused]] const int wsh = PRE ? 32 - t.m.bits : 0;
+    [[maybe_unused]] uint32_t qxs = 0, qys = 0, qzs = 0;
+    [[maybe_unused]] T qmx = px, qmy = py, qmz = pz;
+    if constexpr (PRE) {
+        qxs = (uint32_t)px << wsh;
+        qys = (uint32_t)py << wsh;
+        qzs = (uint32_t)pz << wsh;
+        if constexpr (sizeof(T) >= 8) {
+            const T mask = (T)((T(1) << t.m.bits) - T(1));
+            qmx = px & mask;
+            qmy = py & mask;
+            qmz = pz & mask;
+        }
+    }
+    // Node test.  Integer periodic metric: a float32 squared distance that is
+    // never above the exact one by more than wrap_filter_bound's margin (gaps
+    // from pre-shifted coordinates, the inside test on the full values as in
+    // box_d2_k), compared with the scaled, raised bound, so no node within the
+    // k-th distance is skipped; a few beyond it may be visited, which changes
+    // nothing in the results.  Otherwise the exact distance and bound.
+    auto node_dist = [&](int nd) {
+        if constexpr (PRE) {
+            const float gx = wrap_box_gap_pre<T>(qmx, qxs, t.mnx[nd], t.mxx[nd], wsh);
+            const float gy = wrap_box_gap_pre<T>(qmy, qys, t.mny[nd], t.mxy[nd], wsh);
+            const float gz = wrap_box_gap_pre<T>(qmz, qzs, t.mnz[nd], t.mxz[nd], wsh);
+            float d = gx * gx;
+            d = __builtin_fmaf(gy, gy, d);
+            d = __builtin_fmaf(gz, gz, d);
+            return d;
+        } else {
+            return node_d2_k<T, MK>(t, px, py, pz, nd);
+        }
+    };
+    auto node_bound = [&]() {
+        if constexpr (PRE) return wrap_filter_bound_pre((uint64_t)h.worst_d(), wsh);
+        else return (D)h.worst_d();
+    };
 
     // Scans leaf ranks [first, first+cnt) (cnt <= 64).  One coalesced load
     // puts leaf first+j in lane j.  Phase 1 broadcasts each leaf and marks,
@@ -458,15 +540,17 @@ __device__ __forceinline__ void packet_walk(const DTree<T> &t, int bucket, bool 
         const D w = (D)h.worst_d();
         uint32_t clo = 0, chi = 0;
         const int c0 = cnt < 32 ? cnt : 32;
-        if constexpr (!CoordTraits<T>::fp && MK == MK_WRAP) {
-            // float32 pre-filter (wrap_filter_near): never rejects a leaf within
-            // w; the insertion below tests every passing leaf exactly.
-            const float wf = wrap_filter_bound((uint64_t)w);
-            const int sh = 32 - t.m.bits;
-            const uint32_t qx = (uint32_t)px, qy = (uint32_t)py, qz = (uint32_t)pz;
-            const uint32_t ex = (uint32_t)bx, ey = (uint32_t)by, ez = (uint32_t)bz;
+        [[maybe_unused]] uint32_t ex = 0, ey = 0, ez = 0;  // pre-shifted leaf j in lane j
+        if constexpr (PRE) {
+            // float32 pre-filter (wrap_filter_near on pre-shifted coordinates):
+            // never rejects a leaf within w; the insertion below tests every
+            // passing leaf exactly.
+            ex = (uint32_t)bx << wsh;
+            ey = (uint32_t)by << wsh;
+            ez = (uint32_t)bz << wsh;
+            const float wf = wrap_filter_bound_pre((uint64_t)w, wsh);
             auto near = [&](int j) -> bool {
-                return wrap_filter_near(qx, qy, qz, rl(ex, j), rl(ey, j), rl(ez, j), sh, wf);
+                return wrap_sq_pre(qxs, qys, qzs, rl(ex, j), rl(ey, j), rl(ez, j)) <= wf;
             };
             for (int j = 0; j < c0; ++j) clo |= near(j) ? (1u << j) : 0u;
             for (int j = 32; j < cnt; ++j) chi |= near(j) ? (1u << (j - 32)) : 0u;
@@ -498,18 +582,17 @@ __device__ __forceinline__ void packet_walk(const DTree<T> &t, int bucket, bool 
             cand &= cand - 1;
             const int32_t ri = shfl_any(bo, j);
             D d;
-            if constexpr (!CoordTraits<T>::fp && MK == MK_WRAP) {
-                const int sh = 32 - t.m.bits;
-                const uint64_t gx = wrap_abs32((uint32_t)px, shfl32((uint32_t)bx, j), sh);
-                const uint64_t gy = wrap_abs32((uint32_t)py, shfl32((uint32_t)by, j), sh);
-                const uint64_t gz = wrap_abs32((uint32_t)pz, shfl32((uint32_t)bz, j), sh);
+            if constexpr (PRE) {
+                const uint64_t gx = wrap_gap_pre(qxs, shfl32(ex, j), wsh);
+                const uint64_t gy = wrap_gap_pre(qys, shfl32(ey, j), wsh);
+                const uint64_t gz = wrap_gap_pre(qzs, shfl32(ez, j), wsh);
                 d = gx * gx + gy * gy + gz * gz;
             } else {
                 const T x = shfl_any(bx, j), y = shfl_any(by, j), z = shfl_any(bz, j);
                 d = leaf_d2_k<T, MK>(t.m, px, py, pz, x, y, z);
             }
             const bool take = has && h.beats_worst(d, ri);
-            h.insert(take, d, ri);
+            if (__any(take)) h.insert(take, d, ri);  // a round no lane takes skips the chain
             c.inserts += take ? 1 : 0;
         };
         // Two rounds per trip: the loop-carried top-k is copied once per trip.
@@ -520,11 +603,30 @@ __device__ __forceinline__ void packet_walk(const DTree<T> &t, int bucket, bool 
         }
     };
 
-    // Window pre-scan: leaf ranks [win_lo, win_hi) (the packet's own run of
-    // leaves and its neighbours along the curve) go into h first, which gives
-    // most lanes a tight bound before the walk; the walk then skips them.
+    // Window pre-scan: leaf ranks [win_lo, win_hi) -- the packet's own run of
+    // leaves [win_mid, win_mid + 64) and its neighbours along the curve -- go
+    // into h first, nearest first (own run, then 64-leaf blocks alternately
+    // left and right), which gives most lanes a tight bound before the walk
+    // and keeps the insertions that a tighter bound would reject few; the walk
+    // then skips these leaves.
     const bool window = win_hi > win_lo;
-    for (int f = win_lo; f < win_hi; f += 64) scan_range(f, win_hi - f < 64 ? win_hi - f : 64, false);
+    if (window) {
+        const int own_hi = win_mid + 64 < win_hi ? win_mid + 64 : win_hi;
+        scan_range(win_mid, own_hi - win_mid, false);
+        int le = win_mid, rs = own_hi;
+        while (le > win_lo || rs < win_hi) {
+            if (le > win_lo) {
+                const int lb = le - 64 > win_lo ? le - 64 : win_lo;
+                scan_range(lb, le - lb, false);
+                le = lb;
+            }
+            if (rs < win_hi) {
+                const int re = rs + 64 < win_hi ? rs + 64 : win_hi;
+                scan_range(rs, re - rs, false);
+                rs = re;
+            }
+        }
+    }
 
     int sp = 0;
     int node = 0;  // an internal node some lane needs (or the single leaf when n == 1)
@@ -544,9 +646,9 @@ __device__ __forceinline__ void packet_walk(const DTree<T> &t, int bucket, bool 
             } else {
                 const int l = uni(t.left[node]);
                 const int r = uni(t.right[node]);
-                const D ld = node_d2_k<T, MK>(t, px, py, pz, l);
-                const D rd = node_d2_k<T, MK>(t, px, py, pz, r);
-                const D w = (D)h.worst_d();
+                const auto ld = node_dist(l);
+                const auto rd = node_dist(r);
+                const auto w = node_bound();
                 const bool nl = active && ld <= w;
                 const bool nr = active && rd <= w;
                 const bool anyl = __any(nl), anyr = __any(nr);
@@ -576,8 +678,7 @@ __device__ __forceinline__ void packet_walk(const DTree<T> &t, int bucket, bool 
         if (scan_cnt > 0) scan_range(scan_first, scan_cnt, window);
         while (next < 0 && sp > 0) {
             const int cn = uni(stk[--sp]);
-            const D dc = node_d2_k<T, MK>(t, px, py, pz, cn);
-            if (__any(active && dc <= (D)h.worst_d())) next = cn;
+            if (__any(active && node_dist(cn) <= node_bound())) next = cn;
         }
         if (next < 0) break;
         node = next;

@@ -49,9 +49,11 @@ def test_cxx_dropin_header_builds_and_maps_errors(tmp_path, defs):
 
 @pytest.mark.skipif(not os.path.exists(HIPCC), reason="hipcc not installed")
 def test_float_prefilter_never_rejects_a_leaf_within_the_bound(tmp_path):
-    """The packet walk's float32 pre-filter for integer periodic metrics
-    (wrap_filter_near) passes every leaf whose exact d2 is at most the bound,
-    over bounds equal to, just below and just above d2, and B up to 31."""
+    """The packet walk's float32 filters for integer periodic metrics
+    (wrap_filter_near; wrap_sq_pre and wrap_box_gap_pre on pre-shifted
+    coordinates) pass every leaf and box whose exact d2 is at most the bound,
+    over bounds equal to, just below and just above d2, B up to 31, and box
+    coordinates with bits above B; wrap_gap_pre equals wrap_abs32."""
     src = os.path.join(ROOT, "tests", "host", "wrap_filter_check.cpp")
     exe = str(tmp_path / "wrap_filter_check")
     subprocess.check_call([HIPCC, "-O2", "-std=c++17", "-ffp-contract=off", "--cuda-host-only",
@@ -60,3 +62,4 @@ def test_float_prefilter_never_rejects_a_leaf_within_the_bound(tmp_path):
     print(out.stdout)
     assert out.returncode == 0, out.stdout
     assert "false rejections: 0," in out.stdout
+    assert "gap mismatches: 0," in out.stdout and "box false rejections: 0\n" in out.stdout
