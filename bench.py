@@ -38,6 +38,8 @@ def main():
     ap.add_argument("--k", type=int, default=32)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-ladder", action="store_true", help="skip the plain/grouped kernels")
+    ap.add_argument("--margin-probe", action="store_true",
+                    help="time the learned path at other seed margins (log d2 units)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--bucket", type=int, default=0, help="packet leaf bucket (0 = auto)")
@@ -137,7 +139,8 @@ def main():
     # Traversal statistics of the measured kernel.
     stats = ft.DeviceArray(8, np.uint64).zero()
     ft.self_knn_query(tree, k, results=res, dists=dst, stream=stream.handle, stats=stats,
-                      algorithm=ft.ALGO_PACKET)
+                      algorithm=ft.ALGO_PACKET, leaf_bucket=args.bucket, topk=args.topk,
+                      window=args.window)
     stream.synchronize()
     st = stats.to_host()
 
@@ -185,6 +188,22 @@ def main():
         log(f"learned: seed {seed_ms:.2f} ms + traversal {trav_ms:.2f} ms, reruns "
             f"{learned['rerun_fraction']:.4f}, inserts/q {learned['inserts_per_query']}, "
             f"nodes/wave {learned['nodes_per_wave']}, identical sample {same}")
+        if args.margin_probe:
+            m0 = model.get()[2]
+            for mg in (0.3, 0.4, 0.5, 0.6, 0.7, 0.8, 1.0):
+                model.set(margin=mg)
+                lst.zero()
+                ev[0].record(stream)
+                model.seed(tree, k, seeds, feats, stream=stream.handle)
+                ft.self_knn_query(tree, k, results=res2, dists=dst2, stream=stream.handle,
+                                  seed_bound=seeds, stats=lst, leaf_bucket=args.bucket,
+                                  topk=args.topk, window=args.window)
+                ev[1].record(stream)
+                stream.synchronize()
+                lm = lst.to_host()
+                log(f"margin {mg}: learned {ev[0].elapsed_ms(ev[1]):.2f} ms, reruns "
+                    f"{lm[3] / n:.4f}, inserts/q {lm[4] / n:.1f}")
+            model.set(margin=m0)
         del res2, dst2, feats, seeds
     if args.sweep_buckets:
         for tk in (ft.TOPK_REGISTERS, ft.TOPK_LDS_HEAP):

@@ -845,7 +845,7 @@ static int knn_dispatch(hipStream_t s, const ft_tree *tree, const QSrc<T> &qs, i
     if (o.seed_bound && packet_ok) algo = FT_ALGO_PACKET;
     if (algo == FT_ALGO_PACKET && !packet_ok) algo = FT_ALGO_GROUPED;
     const D *seed = algo == FT_ALGO_PACKET ? (const D *)o.seed_bound : nullptr;
-    const int bucket = o.leaf_bucket > 0 ? std::min(o.leaf_bucket, 64) : 32;
+    const int bucket = o.leaf_bucket > 0 ? std::min(o.leaf_bucket, 64) : 64;
     const uint32_t *order = nullptr;
     if (algo == FT_ALGO_PACKET && (qs.qx || qs.leaf_ids) && nq > 64)
         FT_TRY(query_order<T>(s, dt, qs, nq, order));

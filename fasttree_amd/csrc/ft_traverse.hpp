@@ -409,15 +409,23 @@ template <int M> struct TopKPacked {
     // unshifted slot, so the compares are independent.
     __device__ inline void insert(bool take, uint64_t d, int32_t i) {
         const uint64_t c = take ? make(d, i) : ~0ull;
+        // Slot s becomes key[s-1] if c < key[s-1] (which, the slots being
+        // sorted, implies c < key[s]), else c if c < key[s], else stays: the
+        // select on the previous slot's compare comes first, so it can issue
+        // while the new 64-bit compare's result settles.  cs is c behind an
+        // empty asm, so the compiler does not turn (c < k ? cs : k) into
+        // min(c, k), which it would expand with a second compare per slot.
+        uint64_t cs = c;
+        asm volatile("" : "+v"(cs));
         bool lt_hi = c < key[M - 1];
 #pragma unroll
         for (int s = M - 1; s >= 1; --s) {
             const bool lt_lo = c < key[s - 1];
-            const uint64_t from = lt_lo ? key[s - 1] : c;
-            key[s] = lt_hi ? from : key[s];
+            const uint64_t stay = lt_hi ? cs : key[s];
+            key[s] = lt_lo ? key[s - 1] : stay;
             lt_hi = lt_lo;
         }
-        key[0] = lt_hi ? c : key[0];
+        key[0] = lt_hi ? cs : key[0];
     }
     // Sentinels left in live slots, or a saturated k-th distance.
     __device__ inline bool needs_exact(int k) const {
@@ -552,8 +560,8 @@ __device__ __forceinline__ void packet_walk(const DTree<T> &t, int bucket, bool 
             auto near = [&](int j) -> bool {
                 return wrap_sq_pre(qxs, qys, qzs, rl(ex, j), rl(ey, j), rl(ez, j)) <= wf;
             };
-            for (int j = 0; j < c0; ++j) clo |= near(j) ? (1u << j) : 0u;
-            for (int j = 32; j < cnt; ++j) chi |= near(j) ? (1u << (j - 32)) : 0u;
+            for (int j = 0; j < c0; ++j) clo = near(j) ? (clo | (1u << j)) : clo;
+            for (int j = 32; j < cnt; ++j) chi = near(j) ? (chi | (1u << (j - 32))) : chi;
         } else {
             auto near = [&](int j) -> bool {
                 return leaf_d2_k<T, MK>(t.m, px, py, pz, rl(bx, j), rl(by, j), rl(bz, j)) <= w;
