@@ -178,20 +178,42 @@ __global__ __launch_bounds__(64) void batch_packet_kernel(BatchView<T> b,
             py = b.qy[qi];
             pz = b.qz[qi];
         }
-        H h;
-        if constexpr (HEAP) h.key = heap_mem;
-        if constexpr (PACKED) h.init(k, ~0ull, 32 - __clz(t.n));
-        else h.init(k, dist_max<D>());
-        packet_walk<T, H, MK>(t, q.bucket, active, px, py, pz, -1, h, stk, c);
-        if (active) {
-            bool redo = false;
-            if constexpr (PACKED) redo = h.needs_exact(k);
-            if (redo) {
-                const uint32_t s2 = atomicAdd(b.rerun_count, 1u);
-                b.rerun[s2] = q.entry_first + e;
-            } else {
-                h.store(k, b.res + q.result_first, b.dists + q.result_first, (size_t)local);
+        bool redo = false;
+        if constexpr (PACKED && !HEAP) {
+            // k above M: passes of M, each above the previous pass's last key.
+            uint64_t lo = 0;
+            for (int base = 0; base < k; base += M) {
+                const int kk = k - base < M ? k - base : M;
+                H h;
+                h.init(kk, ~0ull, 32 - __clz(t.n));
+                if (base == 0)
+                    packet_walk<T, H, MK>(t, q.bucket, active, px, py, pz, -1, h, stk, c);
+                else
+                    packet_walk<T, H, MK, true>(t, q.bucket, active, px, py, pz, -1, h, stk, c, 0,
+                                                0, 0, lo);
+                if (active) {
+                    redo = redo || h.needs_exact(kk);
+                    if (!redo)
+                        h.store_part(kk, b.res + q.result_first, b.dists + q.result_first,
+                                     (size_t)local, k, base);
+                }
+                lo = h.next_floor();
             }
+        } else {
+            H h;
+            if constexpr (HEAP) h.key = heap_mem;
+            if constexpr (PACKED) h.init(k, ~0ull, 32 - __clz(t.n));
+            else h.init(k, dist_max<D>());
+            packet_walk<T, H, MK>(t, q.bucket, active, px, py, pz, -1, h, stk, c);
+            if (active) {
+                if constexpr (PACKED) redo = h.needs_exact(k);
+                if (!redo)
+                    h.store(k, b.res + q.result_first, b.dists + q.result_first, (size_t)local);
+            }
+        }
+        if (active && redo) {
+            const uint32_t s2 = atomicAdd(b.rerun_count, 1u);
+            b.rerun[s2] = q.entry_first + e;
         }
     }
     if (lane == 0 && b.stats) {
@@ -419,6 +441,7 @@ static std::vector<int> candidates(int ktier, bool fp) {
     std::vector<int> v;
     const int cap = kTierCap[ktier];
     if (cap <= 32) v.push_back(V_PK1 + (ktier <= 4 ? ktier : 4));
+    else if (!fp) v.push_back(V_PK32);  // passes of 32 (integer metrics)
     if (!fp) {
         if (cap <= 32) v.push_back(V_HP32);
         else if (cap <= 64) v.push_back(V_HP64);
@@ -430,8 +453,8 @@ static std::vector<int> candidates(int ktier, bool fp) {
 
 static bool is_packet(int v) { return v != V_LANE; }
 
-// Default settings: register packets up to k = 32, LDS heap packets up to 128
-// (integer metrics), per-lane heaps otherwise; bucket 32.
+// Default settings: register packets (in passes of 32 above k = 32 for integer
+// metrics), per-lane heaps for larger k with FP metrics; bucket 32.
 static void default_tuning(ft_batch *b) {
     const bool fp = b->coord_type == FT_COORD_F32 || b->coord_type == FT_COORD_F64;
     for (int c = 0; c < N_CLASS; ++c)

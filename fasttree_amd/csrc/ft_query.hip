@@ -308,10 +308,6 @@ __global__ __launch_bounds__(64) void packet_knn_kernel(
     int self = -1;
     if (active) load_query(t, qs, q, px, py, pz, self);
     if (!a.excl) self = -1;
-    H h;
-    if constexpr (HEAP) h.key = heap_mem;
-    if constexpr (PACKED) h.init(k, (SEEDED && active) ? (uint64_t)seed[q] : ~0ull, a.ibits);
-    else h.init(k, (SEEDED && active) ? seed[q] : dist_max<D>());
     // Self queries over all leaves: the packet is leaf ranks [r0, r0 + 64), so
     // its neighbours along the curve are the leaves on either side.
     int wlo = 0, whi = 0;
@@ -320,13 +316,41 @@ __global__ __launch_bounds__(64) void packet_knn_kernel(
         wlo = r0 - a.window > 0 ? r0 - a.window : 0;
         whi = r0 + 64 + a.window < t.n ? r0 + 64 + a.window : t.n;
     }
-    packet_walk<T, H, MK>(t, a.bucket, active, px, py, pz, self, h, stk, c, wlo, whi, r0);
-    if (active) {
+    if constexpr (PACKED && !HEAP) {
+        // k above M (register tier): ceil(k / M) passes, each taking the M
+        // nearest keys above the previous pass's last one, stored in place.
         bool redo = false;
-        if constexpr (PACKED) redo = h.needs_exact(k);
-        else if constexpr (SEEDED) redo = h.missing(k) > 0;
-        if (redo) qm.redo[slot] = 1;
-        else h.store(k, res, dists, (size_t)q);
+        uint64_t lo = 0;
+        for (int base = 0; base < k; base += M) {
+            const int kk = k - base < M ? k - base : M;
+            H h;
+            h.init(kk, (SEEDED && active) ? (uint64_t)seed[q] : ~0ull, a.ibits);
+            if (base == 0)
+                packet_walk<T, H, MK>(t, a.bucket, active, px, py, pz, self, h, stk, c, wlo, whi,
+                                      r0);
+            else
+                packet_walk<T, H, MK, true>(t, a.bucket, active, px, py, pz, self, h, stk, c, wlo,
+                                            whi, r0, lo);
+            if (active) {
+                redo = redo || h.needs_exact(kk);
+                if (!redo) h.store_part(kk, res, dists, (size_t)q, k, base);
+            }
+            lo = h.next_floor();
+        }
+        if (active && redo) qm.redo[slot] = 1;
+    } else {
+        H h;
+        if constexpr (HEAP) h.key = heap_mem;
+        if constexpr (PACKED) h.init(k, (SEEDED && active) ? (uint64_t)seed[q] : ~0ull, a.ibits);
+        else h.init(k, (SEEDED && active) ? seed[q] : dist_max<D>());
+        packet_walk<T, H, MK>(t, a.bucket, active, px, py, pz, self, h, stk, c, wlo, whi, r0);
+        if (active) {
+            bool redo = false;
+            if constexpr (PACKED) redo = h.needs_exact(k);
+            else if constexpr (SEEDED) redo = h.missing(k) > 0;
+            if (redo) qm.redo[slot] = 1;
+            else h.store(k, res, dists, (size_t)q);
+        }
     }
     if (lane == 0 && a.stats) {
         atomicAdd((unsigned long long *)&a.stats[0], (unsigned long long)c.nodes);
@@ -726,7 +750,9 @@ static int run_canonical(hipStream_t s, const DTree<T> &dt, const QSrc<T> &qs, i
         a.ibits = bit_length((uint64_t)dt.n);
         a.window = (!qs.qx && !qs.leaf_ids && !order) ? window : 0;
         a.stats = stats;
-        const bool heap = INTD && (k > 32 || topk == FT_TOPK_LDS_HEAP);
+        // k above 32 with integer coordinates: multi-pass register tier by
+        // default, LDS heap tiers on request.
+        const bool heap = INTD && topk == FT_TOPK_LDS_HEAP;
         if (!heap) {
             FT_BY_M(k, (launch_packet_m<T, M, false, MK>(s, dt, qs, a, qm, seed, res, d)));
         } else if constexpr (INTD) {

@@ -136,6 +136,13 @@ __host__ __device__ __forceinline__ uint64_t wrap_gap_pre(uint32_t a, uint32_t b
 __host__ __device__ __forceinline__ float wrap_filter_bound_pre(uint64_t w, int sh) {
     return wrap_filter_bound(w) * (float)(1ull << (2 * sh));
 }
+// Lower-bound counterpart (multi-pass queries): a leaf whose exact squared
+// distance is at least lo has a pre-shifted float32 distance of at least this
+// (the float sum is within 5 * 2^-24 of the exact one; the bound is lowered by
+// 2^-20 after its own conversion).
+__host__ __device__ __forceinline__ float wrap_filter_floor_pre(uint64_t lo, int sh) {
+    return (float)lo * (1.0f - 0x1p-20f) * (float)(1ull << (2 * sh));
+}
 // Float32 squared distance of a leaf, pre-shifted coordinates (scaled by
 // 2^(2 sh)); compared with wrap_filter_bound_pre it never rejects a leaf whose
 // exact squared distance is within the bound.
@@ -245,6 +252,7 @@ template <typename T> __device__ inline int32_t ret_index(const DTree<T> &t, int
 // hold (-inf, -1) and never move.  Insertion is a fixed compare/shift network.
 // ---------------------------------------------------------------------------
 template <typename D, int M> struct TopK {
+    static constexpr bool kPacked = false;
     D d[M];
     int32_t i[M];
     __device__ inline void init(int k, D bound) {
@@ -386,6 +394,7 @@ template <typename D, bool REF> struct LdsHeap {
 // (and queries left with sentinels) for an exact re-run.
 // ---------------------------------------------------------------------------
 template <int M> struct TopKPacked {
+    static constexpr bool kPacked = true;
     uint64_t key[M];
     int ib;
     uint64_t sat, imask;
@@ -440,6 +449,27 @@ template <int M> struct TopKPacked {
         for (int s = 0; s < M; ++s) bad |= (s >= M - k) & ((key[s] & imask) == imask);
         return bad;
     }
+    // Lower key bound of the next pass of a multi-pass query: above the k-th
+    // key, or nothing when a slot is still empty (no further neighbours).
+    __device__ inline uint64_t next_floor() const {
+        const uint64_t last = key[M - 1];
+        return ((last & imask) == imask) ? ~0ull : last + 1;
+    }
+    // Entries [base, base + k) of row q (row length k_row).
+    template <typename DOut>
+    __device__ inline void store_part(int k, uint64_t *res, DOut *dists, size_t q, int k_row,
+                                      int base) const {
+        const size_t o = q * (size_t)k_row + (size_t)base;
+#pragma unroll
+        for (int s = 0; s < M; ++s) {
+            if (s >= M - k) {
+                const size_t j = o + (size_t)(s - (M - k));
+                const bool ok = (key[s] & imask) != imask;
+                res[j] = ok ? (key[s] & imask) : ~0ull;
+                dists[j] = ok ? (DOut)(key[s] >> ib) : missing_dist<DOut>();
+            }
+        }
+    }
     template <typename DOut>
     __device__ inline void store(int k, uint64_t *res, DOut *dists, size_t q) const {
         const size_t o = q * (size_t)k;
@@ -477,11 +507,11 @@ struct PacketCounters {
     uint64_t nodes = 0, leaves = 0, inserts = 0, overflow = 0;
 };
 
-template <typename T, typename H, int MK>
+template <typename T, typename H, int MK, bool FLOOR = false>
 __device__ __forceinline__ void packet_walk(const DTree<T> &t, int bucket, bool active, T px, T py,
                                             T pz, int self, H &h, int32_t *stk,
                                             PacketCounters &c, int win_lo = 0, int win_hi = 0,
-                                            int win_mid = 0) {
+                                            int win_mid = 0, uint64_t lo_key = 0) {
     using D = typename CoordTraits<T>::D;
     const int lane = threadIdx.x & 63;
     const int n = t.n;
@@ -489,6 +519,11 @@ __device__ __forceinline__ void packet_walk(const DTree<T> &t, int bucket, bool 
     const T *lx = t.mnx + off;
     const T *ly = t.mny + off;
     const T *lz = t.mnz + off;
+    // Multi-pass queries (FLOOR, packed keys): only keys >= lo_key are taken,
+    // so a pass finds the M nearest beyond those of the passes before it.
+    static_assert(!FLOOR || H::kPacked, "a key floor needs packed keys");
+    [[maybe_unused]] D lo_d = D(0);
+    if constexpr (FLOOR) lo_d = (D)(lo_key >> h.ib);
     // Integer periodic metric: pre-shifted 32-bit query (wrap_gap_pre) and the
     // query masked to B bits for the exact inside-the-box tests.
     constexpr bool PRE = !CoordTraits<T>::fp && MK == MK_WRAP;
@@ -557,14 +592,19 @@ __device__ __forceinline__ void packet_walk(const DTree<T> &t, int bucket, bool 
             ey = (uint32_t)by << wsh;
             ez = (uint32_t)bz << wsh;
             const float wf = wrap_filter_bound_pre((uint64_t)w, wsh);
+            [[maybe_unused]] const float lf = wrap_filter_floor_pre((uint64_t)lo_d, wsh);
             auto near = [&](int j) -> bool {
-                return wrap_sq_pre(qxs, qys, qzs, rl(ex, j), rl(ey, j), rl(ez, j)) <= wf;
+                const float d = wrap_sq_pre(qxs, qys, qzs, rl(ex, j), rl(ey, j), rl(ez, j));
+                if constexpr (FLOOR) return d <= wf && d >= lf;
+                else return d <= wf;
             };
             for (int j = 0; j < c0; ++j) clo = near(j) ? (clo | (1u << j)) : clo;
             for (int j = 32; j < cnt; ++j) chi = near(j) ? (chi | (1u << (j - 32))) : chi;
         } else {
             auto near = [&](int j) -> bool {
-                return leaf_d2_k<T, MK>(t.m, px, py, pz, rl(bx, j), rl(by, j), rl(bz, j)) <= w;
+                const D d = leaf_d2_k<T, MK>(t.m, px, py, pz, rl(bx, j), rl(by, j), rl(bz, j));
+                if constexpr (FLOOR) return d <= w && d >= lo_d;
+                else return d <= w;
             };
             for (int j = 0; j < c0; ++j) clo |= near(j) ? (1u << j) : 0u;
             for (int j = 32; j < cnt; ++j) chi |= near(j) ? (1u << (j - 32)) : 0u;
@@ -599,7 +639,8 @@ __device__ __forceinline__ void packet_walk(const DTree<T> &t, int bucket, bool 
                 const T x = shfl_any(bx, j), y = shfl_any(by, j), z = shfl_any(bz, j);
                 d = leaf_d2_k<T, MK>(t.m, px, py, pz, x, y, z);
             }
-            const bool take = has && h.beats_worst(d, ri);
+            bool take = has && h.beats_worst(d, ri);
+            if constexpr (FLOOR) take = take && h.make(d, ri) >= lo_key;
             if (__any(take)) h.insert(take, d, ri);  // a round no lane takes skips the chain
             c.inserts += take ? 1 : 0;
         };
@@ -702,6 +743,7 @@ __device__ __forceinline__ void packet_walk(const DTree<T> &t, int bucket, bool 
 // sifts down (log2 k levels); store() heapsorts in place.
 // ---------------------------------------------------------------------------
 template <int KCAP> struct TopKHeapLds {
+    static constexpr bool kPacked = true;
     uint64_t *key;  // [KCAP][64], owned by the kernel
     uint64_t root, sat, imask;
     int ib, k, lane;

@@ -59,8 +59,10 @@ extern "C" {
 #define FT_ALGO_GROUPED 2 /* SFC-grouped lockstep DFS with a wave vote (reference self-kNN)    */
 #define FT_ALGO_PACKET 3  /* one shared traversal per wavefront of SFC-adjacent queries        */
 
-#define FT_TOPK_AUTO 0      /* registers for k <= 32, an LDS heap above (integer metrics) */
-#define FT_TOPK_REGISTERS 1 /* sorted (d2, index) entries in registers, k <= 32          */
+#define FT_TOPK_AUTO 0      /* = FT_TOPK_REGISTERS                                        */
+#define FT_TOPK_REGISTERS 1 /* sorted (d2, index) entries in registers, up to 32; a larger */
+                            /* k (integer metrics) runs in passes of 32, each above the   */
+                            /* last key of the one before                                  */
 #define FT_TOPK_LDS_HEAP 2  /* a max-heap of packed keys per lane in LDS (integer metrics)*/
 
 /* Stream handle: a hipStream_t passed as an opaque pointer.  NULL = null stream. */
