@@ -65,12 +65,14 @@ template <typename T> struct BatchView {
 // Tier kernels ("variants").  PK_M: packet walk with the register top-k of
 // width M; HP_M: packet walk with an LDS heap of capacity M (integer metrics);
 // LANE: per-lane DFS with LDS heaps, one query per lane (any k, any metric).
+// PASSES: the 32-entry register tier in passes of 32 for k above 32 (integer
+// metrics).
 enum Variant : int {
-    V_PK1, V_PK4, V_PK8, V_PK16, V_PK32, V_HP32, V_HP64, V_HP128, V_LANE, V_COUNT
+    V_PK1, V_PK4, V_PK8, V_PK16, V_PK32, V_HP32, V_HP64, V_HP128, V_LANE, V_PASSES, V_COUNT
 };
-static const char *const kVariantName[V_COUNT] = {"packet-reg1",  "packet-reg4",  "packet-reg8",
-                                                  "packet-reg16", "packet-reg32", "packet-heap32",
-                                                  "packet-heap64", "packet-heap128", "lane-heap"};
+static const char *const kVariantName[V_COUNT] = {
+    "packet-reg1",   "packet-reg4",    "packet-reg8", "packet-reg16",       "packet-reg32",
+    "packet-heap32", "packet-heap64", "packet-heap128", "lane-heap", "packet-reg32-passes"};
 
 // ---------------------------------------------------------------------------
 // Kernels
@@ -140,8 +142,10 @@ __device__ inline int grab_item(uint32_t *ctr, const uint32_t *bnd) {
     return uni(__shfl(it, 0, 64));
 }
 
-// Packet tiers.  HEAP: LDS heap of capacity M, else the register top-k.
-template <typename T, int M, bool HEAP, int MK>
+// Packet tiers.  HEAP: LDS heap of capacity M, else the register top-k;
+// MULTI: the register top-k in passes of M (a separate instantiation, so the
+// single-pass tiers keep their register budget).
+template <typename T, int M, bool HEAP, int MK, bool MULTI = false>
 __global__ __launch_bounds__(64) void batch_packet_kernel(BatchView<T> b,
                                                           const uint2 *__restrict__ items,
                                                           const uint32_t *__restrict__ bnd,
@@ -179,18 +183,17 @@ __global__ __launch_bounds__(64) void batch_packet_kernel(BatchView<T> b,
             pz = b.qz[qi];
         }
         bool redo = false;
-        if constexpr (PACKED && !HEAP) {
+        static_assert(!MULTI || (PACKED && !HEAP), "passes need the packed register tier");
+        if constexpr (MULTI) {
             // k above M: passes of M, each above the previous pass's last key.
             uint64_t lo = 0;
             for (int base = 0; base < k; base += M) {
                 const int kk = k - base < M ? k - base : M;
                 H h;
                 h.init(kk, ~0ull, 32 - __clz(t.n));
-                if (base == 0)
-                    packet_walk<T, H, MK>(t, q.bucket, active, px, py, pz, -1, h, stk, c);
-                else
-                    packet_walk<T, H, MK, true>(t, q.bucket, active, px, py, pz, -1, h, stk, c, 0,
-                                                0, 0, lo);
+                // one walk instance for every pass (the first with floor 0)
+                packet_walk<T, H, MK, true>(t, q.bucket, active, px, py, pz, -1, h, stk, c, 0, 0,
+                                            0, lo);
                 if (active) {
                     redo = redo || h.needs_exact(kk);
                     if (!redo)
@@ -441,7 +444,7 @@ static std::vector<int> candidates(int ktier, bool fp) {
     std::vector<int> v;
     const int cap = kTierCap[ktier];
     if (cap <= 32) v.push_back(V_PK1 + (ktier <= 4 ? ktier : 4));
-    else if (!fp) v.push_back(V_PK32);  // passes of 32 (integer metrics)
+    else if (!fp) v.push_back(V_PASSES);  // passes of 32 (integer metrics)
     if (!fp) {
         if (cap <= 32) v.push_back(V_HP32);
         else if (cap <= 64) v.push_back(V_HP64);
@@ -615,9 +618,9 @@ static int launch_variant_mk(ft_batch *b, hipStream_t s, int v, const uint2 *it,
                              const uint32_t *bnd, uint32_t *ctr) {
     const BatchView<T> bv = view<T>(b);
     constexpr bool FP = CoordTraits<T>::fp;
-#define FT_BPK(M_, HEAP_)                                                                       \
+#define FT_BPK(M_, HEAP_, ...)                                                                  \
     do {                                                                                        \
-        auto kf = batch_packet_kernel<T, M_, HEAP_, MK>;                                        \
+        auto kf = batch_packet_kernel<T, M_, HEAP_, MK, ##__VA_ARGS__>;                         \
         hipLaunchKernelGGL(kf, dim3(persistent_waves(kf, 0, n_items)), dim3(64), 0, s, bv, it,  \
                            bnd, ctr);                                                           \
     } while (0)
@@ -627,6 +630,10 @@ static int launch_variant_mk(ft_batch *b, hipStream_t s, int v, const uint2 *it,
     case V_PK8: FT_BPK(8, false); break;
     case V_PK16: FT_BPK(16, false); break;
     case V_PK32: FT_BPK(32, false); break;
+    case V_PASSES:
+        if constexpr (FP) return fail(FT_EINVAL, "the passes tier needs integer coordinates");
+        else FT_BPK(32, false, true);
+        break;
     case V_HP32:
     case V_HP64:
     case V_HP128:

@@ -280,7 +280,9 @@ struct PacketArgs {
     uint64_t *stats;
 };
 
-template <typename T, int M, bool HEAP, bool SEEDED, bool PACKED, int MK>
+// MULTI (register tier, packed keys, k > M): ceil(k / M) passes, a separate
+// instantiation so that the single-pass kernels keep their register budget.
+template <typename T, int M, bool HEAP, bool SEEDED, bool PACKED, int MK, bool MULTI = false>
 __global__ __launch_bounds__(64) void packet_knn_kernel(
     DTree<T> t, QSrc<T> qs, PacketArgs a, QueryMap qm,
     const typename CoordTraits<T>::D *__restrict__ seed, uint64_t *__restrict__ res,
@@ -316,7 +318,8 @@ __global__ __launch_bounds__(64) void packet_knn_kernel(
         wlo = r0 - a.window > 0 ? r0 - a.window : 0;
         whi = r0 + 64 + a.window < t.n ? r0 + 64 + a.window : t.n;
     }
-    if constexpr (PACKED && !HEAP) {
+    static_assert(!MULTI || (PACKED && !HEAP), "passes need the packed register tier");
+    if constexpr (MULTI) {
         // k above M (register tier): ceil(k / M) passes, each taking the M
         // nearest keys above the previous pass's last one, stored in place.
         bool redo = false;
@@ -325,12 +328,9 @@ __global__ __launch_bounds__(64) void packet_knn_kernel(
             const int kk = k - base < M ? k - base : M;
             H h;
             h.init(kk, (SEEDED && active) ? (uint64_t)seed[q] : ~0ull, a.ibits);
-            if (base == 0)
-                packet_walk<T, H, MK>(t, a.bucket, active, px, py, pz, self, h, stk, c, wlo, whi,
-                                      r0);
-            else
-                packet_walk<T, H, MK, true>(t, a.bucket, active, px, py, pz, self, h, stk, c, wlo,
-                                            whi, r0, lo);
+            // one walk instance for every pass (the first with floor 0)
+            packet_walk<T, H, MK, true>(t, a.bucket, active, px, py, pz, self, h, stk, c, wlo,
+                                        whi, r0, lo);
             if (active) {
                 redo = redo || h.needs_exact(kk);
                 if (!redo) h.store_part(kk, res, dists, (size_t)q, k, base);
@@ -711,9 +711,16 @@ template <typename T, int M, bool HEAP, bool SEEDED, bool PACKED, int MK>
 static void launch_pk(hipStream_t s, const DTree<T> &dt, const QSrc<T> &qs, const PacketArgs &a,
                       const QueryMap &qm, const typename CoordTraits<T>::D *seed, uint64_t *res,
                       typename CoordTraits<T>::D *d) {
-    hipLaunchKernelGGL((packet_knn_kernel<T, M, HEAP, SEEDED, PACKED, MK>),
-                       dim3(xcd_grid(blocks_for(a.nq, 64))), dim3(64), 0, s, dt, qs, a, qm, seed,
-                       res, d);
+    const dim3 g(xcd_grid(blocks_for(a.nq, 64)));
+    if constexpr (PACKED && !HEAP && M == 32) {
+        if (a.k > M) {
+            hipLaunchKernelGGL((packet_knn_kernel<T, M, HEAP, SEEDED, PACKED, MK, true>), g,
+                               dim3(64), 0, s, dt, qs, a, qm, seed, res, d);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((packet_knn_kernel<T, M, HEAP, SEEDED, PACKED, MK>), g, dim3(64), 0, s, dt,
+                       qs, a, qm, seed, res, d);
 }
 
 template <typename T, int M, bool HEAP, int MK>
