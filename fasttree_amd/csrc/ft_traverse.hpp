@@ -424,8 +424,10 @@ template <int M> struct TopKPacked {
         // while the new 64-bit compare's result settles.  cs is c behind an
         // empty asm, so the compiler does not turn (c < k ? cs : k) into
         // min(c, k), which it would expand with a second compare per slot.
+        // Not volatile: a volatile asm counts as a possible memory write, and
+        // later uniform loads of tree nodes would then lose their scalar form.
         uint64_t cs = c;
-        asm volatile("" : "+v"(cs));
+        asm("" : "+v"(cs));
         bool lt_hi = c < key[M - 1];
 #pragma unroll
         for (int s = M - 1; s >= 1; --s) {
