@@ -295,7 +295,9 @@ def main():
         "learned": learned,
         "traversal": {"nodes_per_wave": float(st[0]) / ((n + 63) // 64),
                       "leaves_per_wave": float(st[1]) / ((n + 63) // 64),
-                      "inserts_per_query": float(st[4]) / n, "stack_overflows": int(st[2])},
+                      "inserts_per_query": float(st[4]) / n, "stack_overflows": int(st[2]),
+                      "insert_rounds_per_wave": float(st[5]) / ((n + 63) // 64),
+                      "chains_per_wave": float(st[6]) / ((n + 63) // 64)},
         "roofline": roof,
         "cpu_baseline": cpu,
         "batched": batched,

@@ -356,6 +356,8 @@ __global__ __launch_bounds__(64) void packet_knn_kernel(
         atomicAdd((unsigned long long *)&a.stats[0], (unsigned long long)c.nodes);
         atomicAdd((unsigned long long *)&a.stats[1], (unsigned long long)c.leaves);
         if (c.overflow) atomicAdd((unsigned long long *)&a.stats[2], (unsigned long long)c.overflow);
+        atomicAdd((unsigned long long *)&a.stats[5], (unsigned long long)c.rounds);
+        atomicAdd((unsigned long long *)&a.stats[6], (unsigned long long)c.chains);
     }
     wave_add_stats(a.stats, 4, c.inserts);
 }

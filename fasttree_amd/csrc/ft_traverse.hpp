@@ -416,27 +416,39 @@ template <int M> struct TopKPacked {
     // below s only.  A lane that takes nothing uses the all-ones key, which is
     // below no slot, so no predicate is needed; every compare reads an
     // unshifted slot, so the compares are independent.
-    __device__ inline void insert(bool take, uint64_t d, int32_t i) {
-        const uint64_t c = take ? make(d, i) : ~0ull;
-        // Slot s becomes key[s-1] if c < key[s-1] (which, the slots being
-        // sorted, implies c < key[s]), else c if c < key[s], else stays: the
-        // select on the previous slot's compare comes first, so it can issue
-        // while the new 64-bit compare's result settles.  cs is c behind an
-        // empty asm, so the compiler does not turn (c < k ? cs : k) into
-        // min(c, k), which it would expand with a second compare per slot.
-        // Not volatile: a volatile asm counts as a possible memory write, and
-        // later uniform loads of tree nodes would then lose their scalar form.
-        uint64_t cs = c;
-        asm("" : "+v"(cs));
+    // Slots s >= L of the shift chain, when no taking lane's key is below
+    // key[L-1] (slots below L then keep their keys): slot s becomes key[s-1]
+    // if c < key[s-1] (which, the slots being sorted, implies c < key[s]),
+    // else c if c < key[s], else stays.  The select on the previous slot's
+    // compare comes first, so it can issue while the new 64-bit compare's
+    // result settles.
+    template <int L> __device__ inline void chain_from(uint64_t c, uint64_t cs) {
         bool lt_hi = c < key[M - 1];
 #pragma unroll
-        for (int s = M - 1; s >= 1; --s) {
+        for (int s = M - 1; s >= L + 1; --s) {
             const bool lt_lo = c < key[s - 1];
             const uint64_t stay = lt_hi ? cs : key[s];
             key[s] = lt_lo ? key[s - 1] : stay;
             lt_hi = lt_lo;
         }
-        key[0] = lt_hi ? cs : key[0];
+        key[L] = lt_hi ? cs : key[L];
+    }
+    __device__ inline void insert(bool take, uint64_t d, int32_t i) {
+        const uint64_t c = take ? make(d, i) : ~0ull;
+        // cs is c behind an empty asm, so the compiler does not turn
+        // (c < k ? cs : k) into min(c, k), which it would expand with a second
+        // compare per slot.  Not volatile: a volatile asm counts as a possible
+        // memory write, and later uniform loads of tree nodes would then lose
+        // their scalar form.
+        uint64_t cs = c;
+        asm("" : "+v"(cs));
+        // New keys mostly land near the top: run only the part of the chain a
+        // taking lane can reach (one compare and a wave vote per level).
+        if constexpr (M >= 16) {
+            if (!__any(c < key[3 * M / 4 - 1])) { chain_from<3 * M / 4>(c, cs); return; }
+            if (!__any(c < key[M / 2 - 1])) { chain_from<M / 2>(c, cs); return; }
+        }
+        chain_from<0>(c, cs);
     }
     // Sentinels left in live slots, or a saturated k-th distance.
     __device__ inline bool needs_exact(int k) const {
@@ -507,6 +519,7 @@ constexpr int PK_STACK = 128;
 
 struct PacketCounters {
     uint64_t nodes = 0, leaves = 0, inserts = 0, overflow = 0;
+    uint64_t rounds = 0, chains = 0;  // per wave: insertion rounds, and those that ran the chain
 };
 
 template <typename T, typename H, int MK, bool FLOOR = false>
@@ -643,7 +656,11 @@ __device__ __forceinline__ void packet_walk(const DTree<T> &t, int bucket, bool 
             }
             bool take = has && h.beats_worst(d, ri);
             if constexpr (FLOOR) take = take && h.make(d, ri) >= lo_key;
-            if (__any(take)) h.insert(take, d, ri);  // a round no lane takes skips the chain
+            ++c.rounds;
+            if (__any(take)) {  // a round no lane takes skips the chain
+                h.insert(take, d, ri);
+                ++c.chains;
+            }
             c.inserts += take ? 1 : 0;
         };
         // Two rounds per trip: the loop-carried top-k is copied once per trip.

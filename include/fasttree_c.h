@@ -127,6 +127,8 @@ typedef struct ft_query_opts {
     uint64_t *stats;     /* optional device uint64_t[8] counters, accumulated:           */
                          /* [0] nodes visited [1] leaves tested [2] stack overflows     */
                          /* [3] queries re-run exactly [4] heap insertions              */
+                         /* packet walks, per wave: [5] insertion rounds [6] rounds     */
+                         /* that ran the insertion chain                                */
     int32_t window;      /* FT_ALGO_PACKET self queries over all leaves: leaves scanned  */
                          /* on each side of a packet's own run before the walk (0 =     */
                          /* auto, negative = none)                                       */
