@@ -61,6 +61,10 @@ def main():
                     help="run only the batched multi-tree workload (development)")
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the batched multi-tree workload (64 trees, 4M queries)")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the small plain and the SPH mixed-query configurations")
+    ap.add_argument("--configs-only", action="store_true",
+                    help="run only the small plain and SPH mixed-query configurations")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -70,6 +74,10 @@ def main():
 
     import fasttree_amd as ft
     from fasttree_amd import synth
+
+    if args.configs_only:
+        print(json.dumps({"configs": configs_bench(ft, args.steps)}), flush=True)
+        return
 
     if args.batched_only:
         print(json.dumps({"batched": batched_bench(ft, args.steps, args.warmup, args.seed)}),
@@ -266,9 +274,11 @@ def main():
         cpu = cpu_baseline(ft, tree, n, k, args.cpu_seconds)
 
     batched = None
-    if not args.no_batched:
+    if not args.no_batched or not args.no_configs:
         del res, dst, tree, scratch, p
+    if not args.no_batched:
         batched = batched_bench(ft, args.steps, args.warmup, args.seed)
+    configs = None if args.no_configs else configs_bench(ft, args.steps)
 
     out = {
         "metric": "self-kNN queries/s, N=10M k=32, learned-bound vs plain; BVH build ms",
@@ -301,6 +311,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "batched": batched,
+        "configs": configs,
     }
     print(json.dumps(out), flush=True)
 
@@ -318,6 +329,129 @@ def measured_traffic():
         return None
     m = re.search(r"Self-kNN traffic: ([0-9.]+) GB per launch", text)
     return float(m.group(1)) * 1e9 if m else None
+
+
+def configs_bench(ft, steps):
+    """The small plain configuration and the SPH/Voronoi mixed query, timed with
+    HIP events on one stream, inputs resident in HBM.
+
+    Small plain: N = 100,000 uniform double points (seed 1), 30-bit Morton,
+    non-periodic; build_bvh + self_knn_query k = 16.  SPH mixed query: N = 20M
+    clustered double points in a periodic box L = 100, 63-bit Peano-Hilbert;
+    (a) self_knn_query_subset over a seeded 20% of the leaves at k = 64 (FP
+    k > 32: the grouped and plain per-lane walks, which must agree), (b) the
+    k = 32 self query, h_i = 1.2 x its 32nd neighbour distance, then the CSR
+    range query over every particle with radius h_i (count + scan pass, fill
+    pass).  The learned bound is not used here: its model is trained for the
+    int64 snapshot at k = 32."""
+    from fasttree_amd import synth
+    stream = ft.Stream()
+    ev = [ft.Event() for _ in range(3)]
+    out = {}
+
+    def timed(fn, reps):
+        ts = []
+        for _ in range(reps + 1):
+            ev[0].record(stream)
+            fn()
+            ev[1].record(stream)
+            stream.synchronize()
+            ts.append(ev[0].elapsed_ms(ev[1]))
+        return float(np.median(ts[1:]))
+
+    # small plain
+    n1, k1 = 100_000, 16
+    x, y, z = synth.uniform(n1, 1, 1.0)
+    cfg = ft.Config(sfc=ft.SFC_MORTON, bits_per_dim=10, periodic=False)
+    t1 = ft.TreeSoA(n1, ft.COORD_F64, cfg)
+    sc1 = ft.BvhScratch()
+    p1 = ft.particles(*(ft.DeviceArray.from_host(a) for a in (x, y, z)))
+    r1, d1 = ft.DeviceArray(n1 * k1, np.uint64), ft.DeviceArray(n1 * k1, np.float64)
+
+    def step1():
+        ft.build_bvh(p1, t1, scratch=sc1, stream=stream.handle)
+        ft.self_knn_query(t1, k1, results=r1, dists=d1, stream=stream.handle)
+    ms1 = timed(step1, max(steps, 3))
+    out["small_plain"] = {"workload": "N=100k uniform f64, Morton 10 bits/dim, non-periodic; "
+                                      "build_bvh + self_knn_query k=16",
+                          "step_ms": round(ms1, 4), "queries_per_s": round(n1 / (ms1 * 1e-3), 1)}
+    log(f"small plain: {ms1:.3f} ms per build + self-kNN")
+    del t1, sc1, p1, r1, d1
+
+    # SPH mixed query
+    n5, L = 20_000_000, 100.0
+    t0 = time.time()
+    x, y, z = synth.nfw_clustered(n5, 21, box=L, n_halos=4000)
+    cfg = ft.Config(sfc=ft.SFC_PEANO_HILBERT, bits_per_dim=21, periodic=True, box_length=L)
+    t5 = ft.TreeSoA(n5, ft.COORD_F64, cfg)
+    sc5 = ft.BvhScratch()
+    p5 = ft.particles(*(ft.DeviceArray.from_host(a) for a in (x, y, z)))
+    del x, y, z
+    log(f"SPH snapshot N={n5} generated in {time.time() - t0:.1f}s")
+    build_ms = timed(lambda: ft.build_bvh(p5, t5, scratch=sc5, stream=stream.handle), steps)
+    del p5
+    rng = np.random.default_rng(5)
+    sub = np.sort(rng.permutation(n5)[: n5 // 5]).astype(np.int32)
+    rng.shuffle(sub)
+    dsub = ft.DeviceArray.from_host(sub)
+    ns, ks = len(sub), 64
+    ra, da = ft.DeviceArray(ns * ks, np.uint64), ft.DeviceArray(ns * ks, np.float64)
+    rb, db = ft.DeviceArray(ns * ks, np.uint64), ft.DeviceArray(ns * ks, np.float64)
+    g_ms = timed(lambda: ft.self_knn_query_subset(t5, dsub, ks, results=ra, dists=da,
+                                                  stream=stream.handle,
+                                                  algorithm=ft.ALGO_GROUPED), steps)
+    p_ms = timed(lambda: ft.self_knn_query_subset(t5, dsub, ks, results=rb, dists=db,
+                                                  stream=stream.handle,
+                                                  algorithm=ft.ALGO_PLAIN), steps)
+    rows = rng.choice(ns, 2000, replace=False)
+    same = all(np.array_equal(ra.to_host_range(int(r) * ks, ks), rb.to_host_range(int(r) * ks, ks))
+               for r in rows)
+    del ra, da, rb, db, dsub
+    log(f"SPH subset k=64: grouped {g_ms:.2f} ms, plain {p_ms:.2f} ms, same {same}")
+    # smoothing lengths from the k = 32 self query, then the CSR range query
+    k32 = 32
+    r32, d32 = ft.DeviceArray(n5 * k32, np.uint64), ft.DeviceArray(n5 * k32, np.float64)
+    s_ms = timed(lambda: ft.self_knn_query(t5, k32, results=r32, dists=d32,
+                                           stream=stream.handle), 1)
+    h = 1.2 * np.sqrt(d32.to_host().reshape(n5, k32)[:, k32 - 1])
+    del r32, d32
+    g = t5.to_host()
+    qx, qy, qz = (ft.DeviceArray.from_host(np.ascontiguousarray(g[c][n5 - 1:]))
+                  for c in ("min_x", "min_y", "min_z"))
+    del g
+    rmin = ft.DeviceArray.from_host(np.zeros(n5))
+    rmax = ft.DeviceArray.from_host(h)
+    rp = ft.DeviceArray(n5 + 1, np.int64)
+    lib = ft.lib()
+
+    def count_pass():
+        ft._check(lib.ft_range_query_csr(stream.handle, ft.C.byref(t5.c), qx.ptr, qy.ptr, qz.ptr,
+                                         rmin.ptr, rmax.ptr, n5, rp.ptr, None, 0))
+    c_ms = timed(count_pass, 1)
+    total = int(rp.to_host_range(n5, 1)[0])
+    cols = ft.DeviceArray(max(total, 1), np.int32)
+
+    def fill_pass():
+        ft._check(lib.ft_range_query_csr(stream.handle, ft.C.byref(t5.c), qx.ptr, qy.ptr, qz.ptr,
+                                         rmin.ptr, rmax.ptr, n5, rp.ptr, cols.ptr, total))
+    f_ms = timed(fill_pass, 1)
+    min_row = int(np.diff(rp.to_host()).min())
+    log(f"SPH range CSR: {total} hits ({total / n5:.1f} per query), count+scan {c_ms:.1f} ms, "
+        f"count+scan+fill {f_ms:.1f} ms")
+    out["sph_mixed"] = {
+        "workload": "N=20M NFW-clustered f64, periodic L=100, PH 21 bits/dim",
+        "build_ms": round(build_ms, 3),
+        "subset_k64": {"queries": ns, "grouped_ms": round(g_ms, 3), "plain_ms": round(p_ms, 3),
+                       "queries_per_s": round(ns / (min(g_ms, p_ms) * 1e-3), 1),
+                       "grouped_equals_plain_on_2000_sampled_rows": bool(same)},
+        "self_knn_k32_ms": round(s_ms, 3),
+        "range_csr": {"queries": n5, "hits": total, "hits_per_query": round(total / n5, 2),
+                      "count_scan_ms": round(c_ms, 3), "count_scan_fill_ms": round(f_ms, 3),
+                      "queries_per_s": round(n5 / (f_ms * 1e-3), 1),
+                      "fewest_hits_in_a_row": min_row,
+                      "every_row_holds_itself_and_32_neighbours": bool(min_row >= 33)}}
+    del t5, sc5, qx, qy, qz, rmin, rmax, rp, cols
+    return out
 
 
 def batched_bench(ft, steps, warmup, seed):

@@ -183,19 +183,20 @@ __global__ __launch_bounds__(64) void batch_packet_kernel(BatchView<T> b,
             pz = b.qz[qi];
         }
         bool redo = false;
-        static_assert(!MULTI || (PACKED && !HEAP), "passes need the packed register tier");
+        static_assert(!MULTI || !HEAP, "passes need the register tier");
         if constexpr (MULTI) {
             // k above M: passes of M, each above the previous pass's last key.
-            uint64_t lo = 0;
+            typename H::Floor lo = H::no_floor();
             for (int base = 0; base < k; base += M) {
                 const int kk = k - base < M ? k - base : M;
                 H h;
-                h.init(kk, ~0ull, 32 - __clz(t.n));
+                if constexpr (PACKED) h.init(kk, ~0ull, 32 - __clz(t.n));
+                else h.init(kk, dist_max<D>());
                 // one walk instance for every pass (the first with floor 0)
                 packet_walk<T, H, MK, true>(t, q.bucket, active, px, py, pz, -1, h, stk, c, 0, 0,
                                             0, lo);
                 if (active) {
-                    redo = redo || h.needs_exact(kk);
+                    if constexpr (PACKED) redo = redo || h.needs_exact(kk);
                     if (!redo)
                         h.store_part(kk, b.res + q.result_first, b.dists + q.result_first,
                                      (size_t)local, k, base);
@@ -444,7 +445,7 @@ static std::vector<int> candidates(int ktier, bool fp) {
     std::vector<int> v;
     const int cap = kTierCap[ktier];
     if (cap <= 32) v.push_back(V_PK1 + (ktier <= 4 ? ktier : 4));
-    else if (!fp) v.push_back(V_PASSES);  // passes of 32 (integer metrics)
+    else if (cap <= 128) v.push_back(V_PASSES);  // passes of 32
     if (!fp) {
         if (cap <= 32) v.push_back(V_HP32);
         else if (cap <= 64) v.push_back(V_HP64);
@@ -630,10 +631,7 @@ static int launch_variant_mk(ft_batch *b, hipStream_t s, int v, const uint2 *it,
     case V_PK8: FT_BPK(8, false); break;
     case V_PK16: FT_BPK(16, false); break;
     case V_PK32: FT_BPK(32, false); break;
-    case V_PASSES:
-        if constexpr (FP) return fail(FT_EINVAL, "the passes tier needs integer coordinates");
-        else FT_BPK(32, false, true);
-        break;
+    case V_PASSES: FT_BPK(32, false, true); break;
     case V_HP32:
     case V_HP64:
     case V_HP128:

@@ -318,21 +318,26 @@ __global__ __launch_bounds__(64) void packet_knn_kernel(
         wlo = r0 - a.window > 0 ? r0 - a.window : 0;
         whi = r0 + 64 + a.window < t.n ? r0 + 64 + a.window : t.n;
     }
-    static_assert(!MULTI || (PACKED && !HEAP), "passes need the packed register tier");
+    static_assert(!MULTI || !HEAP, "passes need the register tier");
     if constexpr (MULTI) {
         // k above M (register tier): ceil(k / M) passes, each taking the M
-        // nearest keys above the previous pass's last one, stored in place.
+        // nearest (d2, index) above the previous pass's last one, stored in
+        // place.
         bool redo = false;
-        uint64_t lo = 0;
+        typename H::Floor lo = H::no_floor();
         for (int base = 0; base < k; base += M) {
             const int kk = k - base < M ? k - base : M;
             H h;
-            h.init(kk, (SEEDED && active) ? (uint64_t)seed[q] : ~0ull, a.ibits);
-            // one walk instance for every pass (the first with floor 0)
+            if constexpr (PACKED) h.init(kk, (SEEDED && active) ? (uint64_t)seed[q] : ~0ull, a.ibits);
+            else h.init(kk, (SEEDED && active) ? seed[q] : dist_max<D>());
+            // one walk instance for every pass (the first with no floor)
             packet_walk<T, H, MK, true>(t, a.bucket, active, px, py, pz, self, h, stk, c, wlo,
                                         whi, r0, lo);
             if (active) {
-                redo = redo || h.needs_exact(kk);
+                bool r = false;
+                if constexpr (PACKED) r = h.needs_exact(kk);
+                else if constexpr (SEEDED) r = h.missing(kk) > 0;
+                redo = redo || r;
                 if (!redo) h.store_part(kk, res, dists, (size_t)q, k, base);
             }
             lo = h.next_floor();
@@ -714,7 +719,7 @@ static void launch_pk(hipStream_t s, const DTree<T> &dt, const QSrc<T> &qs, cons
                       const QueryMap &qm, const typename CoordTraits<T>::D *seed, uint64_t *res,
                       typename CoordTraits<T>::D *d) {
     const dim3 g(xcd_grid(blocks_for(a.nq, 64)));
-    if constexpr (PACKED && !HEAP && M == 32) {
+    if constexpr (!HEAP && M == 32) {
         if (a.k > M) {
             hipLaunchKernelGGL((packet_knn_kernel<T, M, HEAP, SEEDED, PACKED, MK, true>), g,
                                dim3(64), 0, s, dt, qs, a, qm, seed, res, d);
@@ -875,7 +880,7 @@ static int knn_dispatch(hipStream_t s, const ft_tree *tree, const QSrc<T> &qs, i
     // Canonical mode.  Packet tiers: registers for k <= 32; an LDS heap per
     // lane for integer metrics up to k = 128.  Larger k (or FP metrics above
     // 32) take the per-lane heap walk.
-    const bool packet_ok = k <= 32 || (!CoordTraits<T>::fp && k <= 128);
+    const bool packet_ok = k <= 128;  // above 32: passes of 32
     if (algo == FT_ALGO_AUTO) algo = packet_ok ? FT_ALGO_PACKET : FT_ALGO_GROUPED;
     if (o.seed_bound && packet_ok) algo = FT_ALGO_PACKET;
     if (algo == FT_ALGO_PACKET && !packet_ok) algo = FT_ALGO_GROUPED;

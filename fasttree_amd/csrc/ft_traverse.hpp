@@ -284,6 +284,35 @@ template <typename D, int M> struct TopK {
         }
     }
     __device__ inline D worst_d() const { return d[M - 1]; }
+    // Multi-pass queries: a pass takes only (d, index) at or above the floor.
+    struct Floor {
+        D d;
+        int32_t i;
+    };
+    __device__ static inline Floor no_floor() { return Floor{D(0), INT32_MIN}; }
+    __device__ inline D floor_d(const Floor &lo) const { return lo.d; }
+    __device__ inline bool above(const Floor &lo, D cd, int32_t ci) const {
+        return (cd > lo.d) | ((cd == lo.d) & (ci >= lo.i));
+    }
+    // Above the k-th entry, or nothing when a slot is still empty.
+    __device__ inline Floor next_floor() const {
+        if (i[M - 1] == INT32_MAX) return Floor{dist_max<D>(), INT32_MAX};
+        return Floor{d[M - 1], i[M - 1] + 1};
+    }
+    template <typename DOut>
+    __device__ inline void store_part(int k, uint64_t *res, DOut *dists, size_t q, int k_row,
+                                      int base) const {
+        const size_t o = q * (size_t)k_row + (size_t)base;
+#pragma unroll
+        for (int s = 0; s < M; ++s) {
+            if (s >= M - k) {
+                const size_t j = o + (size_t)(s - (M - k));
+                const bool ok = i[s] != INT32_MAX;
+                res[j] = ok ? (uint64_t)(uint32_t)i[s] : ~0ull;
+                dists[j] = ok ? (DOut)d[s] : missing_dist<DOut>();
+            }
+        }
+    }
     // Number of live slots still holding the initial sentinel.
     __device__ inline int missing(int k) const {
         int c = 0;
@@ -463,6 +492,11 @@ template <int M> struct TopKPacked {
         for (int s = 0; s < M; ++s) bad |= (s >= M - k) & ((key[s] & imask) == imask);
         return bad;
     }
+    // Multi-pass queries: a pass takes only keys at or above the floor.
+    using Floor = uint64_t;
+    __device__ static inline Floor no_floor() { return 0ull; }
+    __device__ inline uint64_t floor_d(Floor lo) const { return lo >> ib; }
+    __device__ inline bool above(Floor lo, uint64_t d, int32_t i) const { return make(d, i) >= lo; }
     // Lower key bound of the next pass of a multi-pass query: above the k-th
     // key, or nothing when a slot is still empty (no further neighbours).
     __device__ inline uint64_t next_floor() const {
@@ -526,7 +560,8 @@ template <typename T, typename H, int MK, bool FLOOR = false>
 __device__ __forceinline__ void packet_walk(const DTree<T> &t, int bucket, bool active, T px, T py,
                                             T pz, int self, H &h, int32_t *stk,
                                             PacketCounters &c, int win_lo = 0, int win_hi = 0,
-                                            int win_mid = 0, uint64_t lo_key = 0) {
+                                            int win_mid = 0,
+                                            typename H::Floor lo = H::no_floor()) {
     using D = typename CoordTraits<T>::D;
     const int lane = threadIdx.x & 63;
     const int n = t.n;
@@ -534,11 +569,11 @@ __device__ __forceinline__ void packet_walk(const DTree<T> &t, int bucket, bool 
     const T *lx = t.mnx + off;
     const T *ly = t.mny + off;
     const T *lz = t.mnz + off;
-    // Multi-pass queries (FLOOR, packed keys): only keys >= lo_key are taken,
-    // so a pass finds the M nearest beyond those of the passes before it.
-    static_assert(!FLOOR || H::kPacked, "a key floor needs packed keys");
+    // Multi-pass queries (FLOOR): only (d2, index) at or above the floor lo
+    // are taken, so a pass finds the M nearest beyond those of the passes
+    // before it.
     [[maybe_unused]] D lo_d = D(0);
-    if constexpr (FLOOR) lo_d = (D)(lo_key >> h.ib);
+    if constexpr (FLOOR) lo_d = (D)h.floor_d(lo);
     // Integer periodic metric: pre-shifted 32-bit query (wrap_gap_pre) and the
     // query masked to B bits for the exact inside-the-box tests.
     constexpr bool PRE = !CoordTraits<T>::fp && MK == MK_WRAP;
@@ -655,7 +690,7 @@ __device__ __forceinline__ void packet_walk(const DTree<T> &t, int bucket, bool 
                 d = leaf_d2_k<T, MK>(t.m, px, py, pz, x, y, z);
             }
             bool take = has && h.beats_worst(d, ri);
-            if constexpr (FLOOR) take = take && h.make(d, ri) >= lo_key;
+            if constexpr (FLOOR) take = take && h.above(lo, d, ri);
             ++c.rounds;
             if (__any(take)) {  // a round no lane takes skips the chain
                 h.insert(take, d, ri);
@@ -763,6 +798,8 @@ __device__ __forceinline__ void packet_walk(const DTree<T> &t, int bucket, bool 
 // ---------------------------------------------------------------------------
 template <int KCAP> struct TopKHeapLds {
     static constexpr bool kPacked = true;
+    using Floor = uint64_t;  // (single pass only)
+    __device__ static inline Floor no_floor() { return 0ull; }
     uint64_t *key;  // [KCAP][64], owned by the kernel
     uint64_t root, sat, imask;
     int ib, k, lane;

@@ -54,9 +54,8 @@ DATASETS = [
 def test_self_knn_canonical(ft, orc, ds, algo):
     name, ct, sfc, bits, periodic, gen = ds
     tree, t = make(ft, ct, sfc, bits, periodic, gen())
-    # packet: register tiers (k <= 32) and, for integer metrics, k in passes of
-    # 32 (32 < k <= 128; "packet-heap": the LDS heap tiers); FP metrics above 32
-    # take the per-lane heap walk.
+    # packet: register tiers (k <= 32) and k in passes of 32 (32 < k <= 128,
+    # every metric; "packet-heap": the LDS heap tiers of integer metrics).
     ks = (1, 5, 16, 32, 33, 64, 128) if algo.startswith("packet") else (1, 16, 32, 33, 64)
     for k in ks:
         for excl in (True, False):
