@@ -503,6 +503,170 @@ __global__ __launch_bounds__(LANE_BLOCK) void range_kernel(DTree<T> t, const T *
     if (counts) counts[q] = count;
 }
 
+// Range query, 64 queries per wave on one shared DFS (range_query,
+// src/hlbvh.hpp:2006-2136), with the reference's results exactly:
+//  * The reference's per-query DFS pushes right then left, so every query
+//    meets its hits in ascending leaf rank; the shared walk is left-first too,
+//    and each lane appends its hits in the order the walk meets them.
+//  * The reference pushes a node's children only while its stack holds fewer
+//    than 62 entries.  When it pops node v its stack holds the right siblings
+//    pending along v's path, one per left turn, so that count -- the same for
+//    every query that reaches v -- decides: the shared walk carries each
+//    node's left turns and does not descend below a node with 62.
+//  * A subtree of at most `bucket` leaves is tested leaf by leaf without
+//    descending (its nodes add at most cnt - 1 left turns, so this is done
+//    only when the reference could not have stopped inside it).
+//  * A node is skipped only when no lane needs it (exact box distance, or the
+//    conservative float32 form for integer periodic metrics); every leaf is
+//    tested exactly: r_min^2 <= d2 <= r_max^2.
+// Queries are taken in the order `order` (slot -> query; Morton order of the
+// points inside the root box), so a wave's 64 queries are neighbours.
+constexpr int RANGE_DROP_DEPTH = STACK_DEPTH - 2;  // 62
+
+template <typename T, int MK>
+__global__ __launch_bounds__(64) void range_packet_kernel(
+    DTree<T> t, const T *__restrict__ qx, const T *__restrict__ qy, const T *__restrict__ qz,
+    const T *__restrict__ rmin, const T *__restrict__ rmax, int nq,
+    const uint32_t *__restrict__ order, int bucket, int32_t *__restrict__ res,
+    int32_t *__restrict__ counts, int max_res, const int64_t *__restrict__ row_ptr,
+    int32_t *__restrict__ cols, int64_t col_cap) {
+    using D = typename CoordTraits<T>::D;
+    __shared__ int32_t stk[PK_STACK];
+    __shared__ uint8_t stk_lt[PK_STACK];
+    const int lane = threadIdx.x;
+    const int blk = xcd_block();
+    if (blk * 64 >= nq) return;  // padding block of the XCD-ordered grid
+    const int slot = blk * 64 + lane;
+    const bool active = slot < nq;
+    const int q = active ? (order ? (int)order[slot] : slot) : 0;
+    const int n = t.n;
+    const int off = n - 1;
+    T px = T(0), py = T(0), pz = T(0);
+    D rm2 = D(0), RM2 = D(0);
+    if (active) {
+        px = qx[q]; py = qy[q]; pz = qz[q];
+        if constexpr (CoordTraits<T>::fp) {
+            const double a = (double)rmin[q], b = (double)rmax[q];
+            rm2 = a * a;
+            RM2 = b * b;
+        } else {
+            const uint64_t a = (uint64_t)rmin[q] >> t.m.dshift, b = (uint64_t)rmax[q] >> t.m.dshift;
+            rm2 = a * a;
+            RM2 = b * b;
+        }
+    }
+    constexpr bool PRE = !CoordTraits<T>::fp && MK == MK_WRAP;
+    [[maybe_unused]] const int wsh = PRE ? 32 - t.m.bits : 0;
+    [[maybe_unused]] uint32_t qxs = 0, qys = 0, qzs = 0;
+    [[maybe_unused]] T qmx = px, qmy = py, qmz = pz;
+    [[maybe_unused]] float RM2f = 0.f;
+    if constexpr (PRE) {
+        qxs = (uint32_t)px << wsh;
+        qys = (uint32_t)py << wsh;
+        qzs = (uint32_t)pz << wsh;
+        if constexpr (sizeof(T) >= 8) {
+            const T mask = (T)((T(1) << t.m.bits) - T(1));
+            qmx = px & mask;
+            qmy = py & mask;
+            qmz = pz & mask;
+        }
+        RM2f = wrap_filter_bound_pre((uint64_t)RM2, wsh);
+    }
+    auto node_near = [&](int nd) -> bool {
+        if (!active) return false;
+        if constexpr (PRE) {
+            const float gx = wrap_box_gap_pre<T>(qmx, qxs, t.mnx[nd], t.mxx[nd], wsh);
+            const float gy = wrap_box_gap_pre<T>(qmy, qys, t.mny[nd], t.mxy[nd], wsh);
+            const float gz = wrap_box_gap_pre<T>(qmz, qzs, t.mnz[nd], t.mxz[nd], wsh);
+            float d = gx * gx;
+            d = __builtin_fmaf(gy, gy, d);
+            d = __builtin_fmaf(gz, gz, d);
+            return d <= RM2f;
+        } else {
+            return node_d2_k<T, MK>(t, px, py, pz, nd) <= RM2;
+        }
+    };
+    const bool writes = res || cols;
+    const int64_t w0 = (active && row_ptr) ? row_ptr[q] : 0;
+    int count = 0;
+    // Tests leaf ranks [first, first + cnt) (cnt <= 64): lane j loads leaf j.
+    auto scan = [&](int first, int cnt) {
+        T bx = T(0), by = T(0), bz = T(0);
+        int32_t bo = 0;
+        if (lane < cnt) {
+            bx = t.mnx[off + first + lane];
+            by = t.mny[off + first + lane];
+            bz = t.mnz[off + first + lane];
+            bo = ret_index(t, off + first + lane);
+        }
+        uint64_t hit = 0;
+        if constexpr (PRE) {
+            const uint32_t ex = (uint32_t)bx << wsh, ey = (uint32_t)by << wsh,
+                           ez = (uint32_t)bz << wsh;
+            for (int j = 0; j < cnt; ++j) {
+                const uint64_t gx = wrap_gap_pre(qxs, rl(ex, j), wsh);
+                const uint64_t gy = wrap_gap_pre(qys, rl(ey, j), wsh);
+                const uint64_t gz = wrap_gap_pre(qzs, rl(ez, j), wsh);
+                const uint64_t d = gx * gx + gy * gy + gz * gz;
+                hit |= (d >= rm2 && d <= RM2) ? (1ull << j) : 0ull;
+            }
+        } else {
+            for (int j = 0; j < cnt; ++j) {
+                const D d = leaf_d2_k<T, MK>(t.m, px, py, pz, rl(bx, j), rl(by, j), rl(bz, j));
+                hit |= (d >= rm2 && d <= RM2) ? (1ull << j) : 0ull;
+            }
+        }
+        if (!active) hit = 0;
+        if (!writes) {
+            count += __popcll(hit);
+            return;
+        }
+        while (__any(hit != 0)) {  // one hit per lane per round, ascending rank
+            const bool has = hit != 0;
+            const int j = has ? (__ffsll((long long)hit) - 1) : 0;
+            hit &= hit - 1;
+            const int32_t ri = shfl_any(bo, j);
+            if (has) {
+                if (res && count < max_res) res[(size_t)q * max_res + count] = ri;
+                if (cols && w0 + count < col_cap) cols[w0 + count] = ri;
+                ++count;
+            }
+        }
+    };
+    int sp = 0;
+    stk[0] = 0;
+    stk_lt[0] = 0;
+    sp = 1;
+    while (sp > 0) {
+        --sp;
+        const int v = uni(stk[sp]);
+        const int ltv = uni((int)stk_lt[sp]);
+        if (!__any(node_near(v))) continue;
+        if (n == 1) {
+            scan(0, 1);
+            continue;
+        }
+        if (v >= off) {
+            scan(v - off, 1);
+            continue;
+        }
+        const int first = uni(t.rfirst[v]);
+        const int cnt = uni(t.rlast[v]) - first + 1;
+        if (cnt <= bucket && ltv + cnt - 1 < RANGE_DROP_DEPTH) {
+            scan(first, cnt);
+            continue;
+        }
+        if (ltv >= RANGE_DROP_DEPTH) continue;  // the reference stops here
+        // a Karras tree is less than 97 levels deep: the stack never overflows
+        stk[sp] = uni(t.right[v]);
+        stk_lt[sp] = (uint8_t)ltv;
+        stk[sp + 1] = uni(t.left[v]);
+        stk_lt[sp + 1] = (uint8_t)(ltv + 1);
+        sp += 2;
+    }
+    if (active && counts) counts[q] = count;
+}
+
 // Exclusive scan of int32 counts into int64 offsets (Q+1 entries), one block
 // per 1024 counts, then block totals, then a fix-up pass.
 __global__ __launch_bounds__(256) void scan_block_kernel(const int32_t *__restrict__ c, int nq,
@@ -913,18 +1077,46 @@ int knn_entry(hipStream_t s, const ft_tree *tree, const void *qx, const void *qy
     return FT_OK;
 }
 
+// Range query launch: queries in Morton order of the points, packet walk with
+// the metric kind fixed at compile time.
+template <typename T>
+static int range_launch(hipStream_t s, const ft_tree &tree, const T *qx, const T *qy, const T *qz,
+                        const T *rmin, const T *rmax, int nq, int32_t *res, int32_t *counts,
+                        int max_res, const int64_t *row_ptr, int32_t *cols, int64_t col_cap) {
+    const DTree<T> dt = make_dtree<T>(tree);
+    const QSrc<T> qs{qx, qy, qz, nullptr};
+    const uint32_t *order = nullptr;
+    FT_TRY(query_order<T>(s, dt, qs, nq, order));
+    const dim3 g(xcd_grid(blocks_for(nq, 64))), b(64);
+    constexpr int BUCKET = 32;
+    switch (metric_kind<T>(dt.m)) {
+    case MK_WRAP:
+        hipLaunchKernelGGL((range_packet_kernel<T, MK_WRAP>), g, b, 0, s, dt, qx, qy, qz, rmin,
+                           rmax, nq, order, BUCKET, res, counts, max_res, row_ptr, cols, col_cap);
+        break;
+    case MK_OPEN:
+        hipLaunchKernelGGL((range_packet_kernel<T, MK_OPEN>), g, b, 0, s, dt, qx, qy, qz, rmin,
+                           rmax, nq, order, BUCKET, res, counts, max_res, row_ptr, cols, col_cap);
+        break;
+    default:
+        hipLaunchKernelGGL((range_packet_kernel<T, MK_ANY>), g, b, 0, s, dt, qx, qy, qz, rmin,
+                           rmax, nq, order, BUCKET, res, counts, max_res, row_ptr, cols, col_cap);
+        break;
+    }
+    FT_CHECK_LAUNCH();
+    return FT_OK;
+}
+
 int range_entry(hipStream_t s, const ft_tree *tree, const void *qx, const void *qy,
                 const void *qz, const void *rmin, const void *rmax, int nq, int32_t *res,
                 int32_t *counts, int max_res, const int64_t *row_ptr, int32_t *cols,
                 int64_t col_cap) {
-    FT_DISPATCH_COORD(tree->coord_type, {
-        const DTree<T> dt = make_dtree<T>(*tree);
-        hipLaunchKernelGGL(range_kernel<T>, dim3(blocks_for(nq, LANE_BLOCK)), dim3(LANE_BLOCK), 0,
-                           s, dt, (const T *)qx, (const T *)qy, (const T *)qz, (const T *)rmin,
-                           (const T *)rmax, nq, res, counts, max_res, row_ptr, cols, col_cap);
-        FT_CHECK_LAUNCH();
-    });
-    return FT_OK;
+    int rc = FT_OK;
+    FT_DISPATCH_COORD(tree->coord_type,
+                      rc = range_launch<T>(s, *tree, (const T *)qx, (const T *)qy, (const T *)qz,
+                                           (const T *)rmin, (const T *)rmax, nq, res, counts,
+                                           max_res, row_ptr, cols, col_cap));
+    return rc;
 }
 
 int scan_entry(hipStream_t s, const int32_t *counts, int nq, int64_t *row_ptr) {

@@ -82,3 +82,44 @@ def test_fp_periodic_range_and_knn(ft, orc):
     assert np.array_equal(rp, erp) and np.array_equal(cols, ecols)
     # each query finds itself and at least its 24 nearest neighbours
     assert np.all(np.diff(rp) >= 25)
+
+
+def comb_points(ct):
+    """63 points whose Morton keys are the 63 powers of two below 2^63, plus the
+    origin: the radix tree is a comb 63 levels deep whose every step goes left,
+    so the reference's per-query stack is full at its deepest nodes and it drops
+    their children."""
+    pts = [(0, 0, 0)]
+    for i in range(21):
+        pts += [(1 << i, 0, 0), (0, 1 << i, 0), (0, 0, 1 << i)]
+    a = np.array(pts, dtype=O.COORD_DTYPES[ct])
+    return a[:, 0].copy(), a[:, 1].copy(), a[:, 2].copy()
+
+
+@pytest.mark.parametrize("ct", [O.CT_U64, O.CT_U32])
+def test_range_deep_comb_tree(ft, orc, ct):
+    """The shared-walk range query reproduces the reference's stack limit: on a
+    tree whose deepest nodes sit below 62 left turns, the leaves the reference
+    never reaches are missing from the GPU output too, in the same order."""
+    pts = comb_points(ct)
+    tree, t = make(ft, ct, O.SFC_MORTON, 21, 0, pts)
+    dt = O.COORD_DTYPES[ct]
+    x, y, z = pts
+    qx = np.concatenate([x, np.array([3, 0, 1 << 20], dt)])
+    qy = np.concatenate([y, np.array([0, 5, 0], dt)])
+    qz = np.concatenate([z, np.array([1, 0, 0], dt)])
+    nq = len(qx)
+    rmax = np.full(nq, (1 << 21) - 1, dt)  # every point inside
+    rmax[::4] = 4
+    rmin = np.zeros(nq, dt)
+    rmin[::7] = 1
+    res, cnt = ft.range_query(tree, qx, qy, qz, rmin, rmax, 64)
+    rr, rc = O.range_reference(t, qx, qy, qz, rmin, rmax, 64)
+    assert np.array_equal(cnt, rc)
+    for i in range(nq):
+        m = min(cnt[i], 64)
+        assert np.array_equal(res[i, :m], rr[i, :m])
+    assert rc.max() == len(x) - 2  # the reference never reaches the two deepest leaves
+    rp, cols = ft.range_query_csr(tree, qx, qy, qz, rmin, rmax)
+    erp, ecols = O.range_csr(t, qx, qy, qz, rmin, rmax)
+    assert np.array_equal(rp, erp) and np.array_equal(cols, ecols)
