@@ -444,64 +444,6 @@ __global__ __launch_bounds__(256) void flag_scatter_kernel(const uint8_t *__rest
     }
 }
 
-// ===========================================================================
-// range_query (src/hlbvh.hpp:2046-2119): per-lane DFS, right child pushed
-// before left so leaves arrive in ascending rank; accept rm2 <= d2 <= RM2;
-// store while count < max; count = all hits.  CSR mode writes at row offsets.
-// ===========================================================================
-template <typename T>
-__global__ __launch_bounds__(LANE_BLOCK) void range_kernel(DTree<T> t, const T *__restrict__ qx,
-                                                           const T *__restrict__ qy,
-                                                           const T *__restrict__ qz,
-                                                           const T *__restrict__ rmin,
-                                                           const T *__restrict__ rmax, int nq,
-                                                           int32_t *__restrict__ res,
-                                                           int32_t *__restrict__ counts,
-                                                           int max_res,
-                                                           const int64_t *__restrict__ row_ptr,
-                                                           int32_t *__restrict__ cols,
-                                                           int64_t col_cap) {
-    using D = typename CoordTraits<T>::D;
-    __shared__ int32_t stk[STACK_DEPTH][LANE_BLOCK];
-    const int lane = threadIdx.x;
-    const int q = blockIdx.x * LANE_BLOCK + lane;
-    if (q >= nq) return;
-    const int n = t.n;
-    const T px = qx[q], py = qy[q], pz = qz[q];
-    D rm2, RM2;
-    if constexpr (CoordTraits<T>::fp) {
-        const double a = (double)rmin[q], b = (double)rmax[q];
-        rm2 = a * a;
-        RM2 = b * b;
-    } else {
-        const uint64_t a = (uint64_t)rmin[q] >> t.m.dshift, b = (uint64_t)rmax[q] >> t.m.dshift;
-        rm2 = a * a;
-        RM2 = b * b;
-    }
-    int sp = 0;
-    stk[sp++][lane] = 0;
-    int count = 0;
-    const int64_t w0 = row_ptr ? row_ptr[q] : 0;
-    while (sp > 0) {
-        const int node = stk[--sp][lane];
-        const D d2 = node_d2(t, px, py, pz, node);
-        if (d2 <= RM2) {
-            if (node >= n - 1 || n == 1) {
-                if (d2 >= rm2) {
-                    const int leaf = (n == 1) ? 0 : node;
-                    const int32_t ri = ret_index(t, leaf);
-                    if (res && count < max_res) res[(size_t)q * max_res + count] = ri;
-                    if (cols && w0 + count < col_cap) cols[w0 + count] = ri;
-                    count++;
-                }
-            } else if (sp < STACK_DEPTH - 2) {
-                stk[sp++][lane] = t.right[node];
-                stk[sp++][lane] = t.left[node];
-            }
-        }
-    }
-    if (counts) counts[q] = count;
-}
 
 // Range query, 64 queries per wave on one shared DFS (range_query,
 // src/hlbvh.hpp:2006-2136), with the reference's results exactly:
