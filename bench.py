@@ -52,6 +52,10 @@ def main():
                                                     "nfw10m_int64_ph21_k32.npz"),
                     help="trained learned-radius model (npz); the learned step is skipped "
                          "when it is missing")
+    ap.add_argument("--sph-model", default=os.path.join(ROOT, "fasttree_amd", "models",
+                                                        "nfw20m_f64_ph21_k32k64.npz"),
+                    help="learned-radius model for the SPH mixed query (double coordinates, "
+                         "k = 32 and 64); its seeded legs are skipped when it is missing")
     ap.add_argument("--seed-probe", action="store_true",
                     help="time the packet kernel seeded with scaled exact k-th distances")
     ap.add_argument("--pmc-probe", action="store_true",
@@ -76,7 +80,7 @@ def main():
     from fasttree_amd import synth
 
     if args.configs_only:
-        print(json.dumps({"configs": configs_bench(ft, args.steps)}), flush=True)
+        print(json.dumps({"configs": configs_bench(ft, args.steps, args.sph_model)}), flush=True)
         return
 
     if args.batched_only:
@@ -278,7 +282,7 @@ def main():
         del res, dst, tree, scratch, p
     if not args.no_batched:
         batched = batched_bench(ft, args.steps, args.warmup, args.seed)
-    configs = None if args.no_configs else configs_bench(ft, args.steps)
+    configs = None if args.no_configs else configs_bench(ft, args.steps, args.sph_model)
 
     out = {
         "metric": "self-kNN queries/s, N=10M k=32, learned-bound vs plain; BVH build ms",
@@ -331,7 +335,7 @@ def measured_traffic():
     return float(m.group(1)) * 1e9 if m else None
 
 
-def configs_bench(ft, steps):
+def configs_bench(ft, steps, sph_model=None):
     """The small plain configuration and the SPH/Voronoi mixed query, timed with
     HIP events on one stream, inputs resident in HBM.
 
@@ -342,8 +346,11 @@ def configs_bench(ft, steps):
     k > 32: the grouped and plain per-lane walks, which must agree), (b) the
     k = 32 self query, h_i = 1.2 x its 32nd neighbour distance, then the CSR
     range query over every particle with radius h_i (count + scan pass, fill
-    pass).  The learned bound is not used here: its model is trained for the
-    int64 snapshot at k = 32."""
+    pass).  Both queries also run seeded by the learned radius (a model trained
+    on double-coordinate snapshots of other seeds at k = 32 and 64): features +
+    MLP seeds, the seeded walk, the exact re-run of queries whose seed held
+    fewer than k; their rows must equal the unseeded ones, so the radii h_i
+    and the range query are the same either way."""
     from fasttree_amd import synth
     stream = ft.Stream()
     ev = [ft.Event() for _ in range(3)]
@@ -413,6 +420,35 @@ def configs_bench(ft, steps):
                                  rb.to_host_range(int(r) * ks, ks)) and
                   np.array_equal(da.to_host_range(int(r) * ks, ks),
                                  db.to_host_range(int(r) * ks, ks)) for r in rows)
+    model = None
+    if sph_model and os.path.exists(sph_model):
+        from fasttree_amd import learned as L
+        model = L.MLP.load(sph_model)
+    sub_learned = None
+    if model is not None:
+        sd = ft.DeviceArray(ns, np.float64)
+        sf = ft.DeviceArray(ns * L.NF, np.float32)
+        st = ft.DeviceArray(8, np.uint64)
+
+        def learned_subset():
+            model.seed(t5, ks, sd, sf, leaf_ids=dsub, stream=stream.handle)
+            ft.self_knn_query_subset(t5, dsub, ks, results=ra, dists=da, stream=stream.handle,
+                                     seed_bound=sd, stats=st)
+        seed_ms = timed(lambda: model.seed(t5, ks, sd, sf, leaf_ids=dsub, stream=stream.handle),
+                        steps)
+        st.zero()
+        l_ms = timed(learned_subset, steps)
+        reruns = int(st.to_host()[3]) / ((steps + 1) * ns)
+        same_l = all(np.array_equal(ra.to_host_range(int(r) * ks, ks),
+                                    rb.to_host_range(int(r) * ks, ks)) and
+                     np.array_equal(da.to_host_range(int(r) * ks, ks),
+                                    db.to_host_range(int(r) * ks, ks)) for r in rows)
+        sub_learned = {"learned_ms": round(l_ms, 3), "seed_ms": round(seed_ms, 3),
+                       "rerun_fraction": round(reruns, 5), "vs_packet": round(k_ms / l_ms, 3),
+                       "equals_unseeded_on_2000_sampled_rows": bool(same_l)}
+        log(f"SPH subset k=64 learned: {l_ms:.2f} ms (seed {seed_ms:.2f} ms), reruns "
+            f"{reruns:.4f}, same {same_l}")
+        del sd, sf
     del ra, da, rb, db, dsub
     log(f"SPH subset k=64: packet (passes of 32) {k_ms:.2f} ms, grouped {g_ms:.2f} ms, plain "
         f"{p_ms:.2f} ms, same {same} {same_pk}")
@@ -422,6 +458,32 @@ def configs_bench(ft, steps):
     s_ms = timed(lambda: ft.self_knn_query(t5, k32, results=r32, dists=d32,
                                            stream=stream.handle), 1)
     h = 1.2 * np.sqrt(d32.to_host().reshape(n5, k32)[:, k32 - 1])
+    self_learned = None
+    if model is not None:
+        sd = ft.DeviceArray(n5, np.float64)
+        sf = ft.DeviceArray(n5 * L.NF, np.float32)
+        st = ft.DeviceArray(8, np.uint64).zero()
+        r32b, d32b = ft.DeviceArray(n5 * k32, np.uint64), ft.DeviceArray(n5 * k32, np.float64)
+
+        def learned_self():
+            model.seed(t5, k32, sd, sf, stream=stream.handle)
+            ft.self_knn_query(t5, k32, results=r32b, dists=d32b, stream=stream.handle,
+                              seed_bound=sd, stats=st)
+        ls_ms = timed(learned_self, 1)
+        reruns = int(st.to_host()[3]) / (2 * n5)
+        rows5 = np.random.default_rng(6).choice(n5, 2000, replace=False)
+        same_s = all(np.array_equal(r32.to_host_range(int(r) * k32, k32),
+                                    r32b.to_host_range(int(r) * k32, k32)) and
+                     np.array_equal(d32.to_host_range(int(r) * k32, k32),
+                                    d32b.to_host_range(int(r) * k32, k32)) for r in rows5)
+        hb = 1.2 * np.sqrt(d32b.to_host().reshape(n5, k32)[:, k32 - 1])
+        self_learned = {"learned_ms": round(ls_ms, 3), "rerun_fraction": round(reruns, 5),
+                        "vs_packet": round(s_ms / ls_ms, 3),
+                        "equals_unseeded_on_2000_sampled_rows": bool(same_s),
+                        "smoothing_lengths_identical": bool(np.array_equal(h, hb))}
+        log(f"SPH self k=32 learned: {ls_ms:.2f} ms vs {s_ms:.2f} ms, reruns {reruns:.4f}, "
+            f"same {same_s}, h identical {self_learned['smoothing_lengths_identical']}")
+        del sd, sf, r32b, d32b, hb
     del r32, d32
     g = t5.to_host()
     qx, qy, qz = (ft.DeviceArray.from_host(np.ascontiguousarray(g[c][n5 - 1:]))
@@ -453,8 +515,10 @@ def configs_bench(ft, steps):
                        "plain_ms": round(p_ms, 3),
                        "queries_per_s": round(ns / (min(k_ms, g_ms, p_ms) * 1e-3), 1),
                        "grouped_equals_plain_on_2000_sampled_rows": bool(same),
-                       "packet_equals_grouped_on_2000_sampled_rows": bool(same_pk)},
+                       "packet_equals_grouped_on_2000_sampled_rows": bool(same_pk),
+                       "learned": sub_learned},
         "self_knn_k32_ms": round(s_ms, 3),
+        "self_knn_k32_learned": self_learned,
         "range_csr": {"queries": n5, "hits": total, "hits_per_query": round(total / n5, 2),
                       "count_scan_ms": round(c_ms, 3), "count_scan_fill_ms": round(f_ms, 3),
                       "queries_per_s": round(n5 / (f_ms * 1e-3), 1),

@@ -328,8 +328,13 @@ __global__ __launch_bounds__(64) void packet_knn_kernel(
         for (int base = 0; base < k; base += M) {
             const int kk = k - base < M ? k - base : M;
             H h;
-            if constexpr (PACKED) h.init(kk, (SEEDED && active) ? (uint64_t)seed[q] : ~0ull, a.ibits);
-            else h.init(kk, (SEEDED && active) ? seed[q] : dist_max<D>());
+            // The seed is read again in every pass (an opaque copy of q keeps
+            // the load in the loop): holding it across the passes would cost
+            // the kernel two registers.
+            int qv = q;
+            asm volatile("" : "+v"(qv));
+            if constexpr (PACKED) h.init(kk, (SEEDED && active) ? (uint64_t)seed[qv] : ~0ull, a.ibits);
+            else h.init(kk, (SEEDED && active) ? seed[qv] : dist_max<D>());
             // one walk instance for every pass (the first with no floor)
             packet_walk<T, H, MK, true>(t, a.bucket, active, px, py, pz, self, h, stk, c, wlo,
                                         whi, r0, lo);

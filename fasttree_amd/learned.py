@@ -21,8 +21,9 @@ import time
 
 import numpy as np
 
-from . import (ALGO_PACKET, COORD_U64, SFC_PEANO_HILBERT, BvhScratch, Config, DeviceArray,
-               TreeSoA, _check, _ptr, _sh, lib, particles, self_knn_query)
+from . import (ALGO_PACKET, COORD_F32, COORD_F64, COORD_U64, SFC_PEANO_HILBERT, BvhScratch,
+               Config, DeviceArray, TreeSoA, _check, _ptr, _sh, lib, particles,
+               self_knn_query_subset)
 from . import synth
 
 NF = 16
@@ -180,10 +181,18 @@ def norm_from(feats: np.ndarray, y: np.ndarray):
 # ---------------------------------------------------------------------------
 # Training data: exact kNN of seeded clustered snapshots
 # ---------------------------------------------------------------------------
-def snapshot_tree(n, seed, bits=21):
-    x, y, z = synth.clustered_fixed(n, seed, bits=bits, box=100.0)
-    cfg = Config(sfc=SFC_PEANO_HILBERT, bits_per_dim=bits, periodic=True)
-    tree = TreeSoA(n, COORD_U64, cfg)
+def snapshot_tree(n, seed, bits=21, coord="u64", n_halos=2000):
+    """Seeded NFW-clustered snapshot in a periodic box L = 100, built on the
+    device: 21-bit fixed-point int64 coordinates ("u64") or the double
+    coordinates themselves ("f64"), 63-bit Peano-Hilbert keys."""
+    if coord == "f64":
+        x, y, z = synth.nfw_clustered(n, seed, box=100.0, n_halos=n_halos)
+        cfg = Config(sfc=SFC_PEANO_HILBERT, bits_per_dim=bits, periodic=True, box_length=100.0)
+        tree = TreeSoA(n, COORD_F64, cfg)
+    else:
+        x, y, z = synth.clustered_fixed(n, seed, bits=bits, box=100.0, n_halos=n_halos)
+        cfg = Config(sfc=SFC_PEANO_HILBERT, bits_per_dim=bits, periodic=True)
+        tree = TreeSoA(n, COORD_U64, cfg)
     p = particles(*(DeviceArray.from_host(a) for a in (x, y, z)))
     build_bvh_dev(p, tree)
     return tree
@@ -194,31 +203,50 @@ def build_bvh_dev(p, tree):
     build_bvh(p, tree)
 
 
+def target_eps(tree: TreeSoA) -> float:
+    """The offset of the regression target log(d2_k + eps): 1 for integer
+    coordinates; for FP coordinates 1e-12 * (largest root extent)^2, the
+    features' own offset (fasttree_amd/csrc/ft_learned.hip, features_kernel)."""
+    if tree.coord_type not in (COORD_F32, COORD_F64):
+        return 1.0
+    dt = np.float64 if tree.coord_type == COORD_F64 else np.float32
+    ext = []
+    for lo, hi in (("min_x", "max_x"), ("min_y", "max_y"), ("min_z", "max_z")):
+        a, b = np.empty(1, dt), np.empty(1, dt)
+        _check(lib().ft_memcpy_d2h(a.ctypes.data, getattr(tree.c, lo), a.nbytes, None))
+        _check(lib().ft_memcpy_d2h(b.ctypes.data, getattr(tree.c, hi), b.nbytes, None))
+        ext.append(float(b[0]) - float(a[0]))
+    e = max(ext)
+    return 1e-12 * e * e if e > 0 else 1e-300
+
+
 def examples(tree: TreeSoA, k: int, sample: int, rng) -> tuple:
-    """(features[sample,16], log(d2_k + 1)[sample]) for a random sample of leaves."""
+    """(features[sample,16], log(d2_k + eps)[sample]) for a random sample of leaves."""
     n = tree.num_leaves
-    res = DeviceArray(n * k, np.uint64)
-    dst = DeviceArray(n * k, np.uint64)
-    self_knn_query(tree, k, results=res, dists=dst, algorithm=ALGO_PACKET)
-    dk = dst.to_host().reshape(n, k)[:, k - 1].astype(np.float64)
-    del res, dst
     ids = np.sort(rng.choice(n, size=min(sample, n), replace=False)).astype(np.int32)
-    f = features(tree, k, leaf_ids=ids).to_host().reshape(-1, NF)
-    return f, np.log(dk[ids] + 1.0).astype(np.float32)
+    dids = DeviceArray.from_host(ids)
+    # the subset query returns exactly the self query's rows for these leaves
+    _, dst = self_knn_query_subset(tree, dids, k, algorithm=ALGO_PACKET)
+    dk = dst[:, k - 1].astype(np.float64)
+    f = features(tree, k, leaf_ids=dids).to_host().reshape(-1, NF)
+    return f, np.log(dk + target_eps(tree)).astype(np.float32)
 
 
 def train(train_seeds, k=32, n=10_000_000, sample=1_000_000, steps=3000, batch=65536, lr=2e-3,
-          fallback=0.002, log=print):
-    """Trains on `sample` queries from each seeded snapshot; returns (MLP, report)."""
+          fallback=0.002, log=print, coord="u64", ks=None, n_halos=2000):
+    """Trains on `sample` queries from each seeded snapshot (split evenly over
+    the k values in `ks`, default just `k`); returns (MLP, report)."""
+    ks = list(ks) if ks else [k]
     rng = np.random.default_rng(1234)
     feats, ys = [], []
     t0 = time.time()
     for s in train_seeds:
-        tree = snapshot_tree(n, s)
-        f, y = examples(tree, k, sample, rng)
-        feats.append(f)
-        ys.append(y)
-        log(f"snapshot seed {s}: {len(y)} examples ({time.time() - t0:.1f}s)")
+        tree = snapshot_tree(n, s, coord=coord, n_halos=n_halos)
+        for kk in ks:
+            f, y = examples(tree, kk, sample // len(ks), rng)
+            feats.append(f)
+            ys.append(y)
+            log(f"snapshot seed {s}, k={kk}: {len(y)} examples ({time.time() - t0:.1f}s)")
         del tree
     F = np.concatenate(feats)
     Y = np.concatenate(ys)
@@ -243,7 +271,8 @@ def train(train_seeds, k=32, n=10_000_000, sample=1_000_000, steps=3000, batch=6
     margin = float(np.quantile(resid, 1.0 - fallback))
     m.set(margin=margin)
     rep = {"train_examples": total, "resid_std": float(resid.std()), "margin": margin,
-           "train_fallback": float(np.mean(resid > margin)), "k": k, "n": n,
+           "train_fallback": float(np.mean(resid > margin)), "k": ks[0] if len(ks) == 1 else ks,
+           "n": n, "coord": coord, "n_halos": n_halos,
            "train_seeds": list(train_seeds), "steps": steps, "batch": batch}
     return m, rep
 
@@ -256,6 +285,10 @@ def main(argv=None):
     t.add_argument("--seeds", default="1,2,3,4,5,6,7,8")
     t.add_argument("--n", type=int, default=10_000_000)
     t.add_argument("--k", type=int, default=32)
+    t.add_argument("--ks", default="", help="comma-separated k values to train on (overrides --k)")
+    t.add_argument("--coord", choices=("u64", "f64"), default="u64",
+                   help="snapshot coordinates: 21-bit fixed-point int64 or double")
+    t.add_argument("--halos", type=int, default=2000, help="NFW halos in each snapshot")
     t.add_argument("--sample", type=int, default=1_000_000)
     t.add_argument("--steps", type=int, default=3000)
     t.add_argument("--fallback", type=float, default=0.002,
@@ -264,8 +297,10 @@ def main(argv=None):
     a = ap.parse_args(argv)
     if a.cmd == "train":
         seeds = [int(s) for s in a.seeds.split(",")]
+        ks = [int(v) for v in a.ks.split(",")] if a.ks else [a.k]
         m, rep = train(seeds, k=a.k, n=a.n, sample=a.sample, steps=a.steps, fallback=a.fallback,
-                       log=lambda s: print(s, flush=True))
+                       log=lambda s: print(s, flush=True), coord=a.coord, ks=ks,
+                       n_halos=a.halos)
         os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)
         m.save(a.out, rep)
         print(json.dumps(rep), flush=True)

@@ -190,6 +190,31 @@ def test_seeded_bound_never_changes_results(ft, orc):
         assert st[3] == int(np.sum(seed < db[:, -1]))  # exactly the too-small seeds re-ran
 
 
+def test_seeded_bound_fp_periodic_subset_and_self(ft, orc):
+    """Seeds on the SPH shape at test size (double coordinates, periodic box,
+    Peano-Hilbert): a random 20% subset at k = 64 (passes of 32) and every leaf
+    at k = 32, each query seeded too small, exactly right or loose: results
+    equal the brute force and exactly the queries whose seed is below their
+    k-th distance re-run."""
+    n = 20000
+    pts = nfw_points(n, 23, O.CT_F64)
+    tree, t = make(ft, O.CT_F64, O.SFC_PH, 21, 1, pts, box=100.0)
+    rng = np.random.default_rng(4)
+    ids = rng.permutation(n)[: n // 5].astype(np.int32)
+    for k, leaf_ids in ((64, ids), (32, None)):
+        rb, db = O.knn_bruteforce(t, k, True, leaf_ids=leaf_ids)
+        scale = rng.choice([0.25, 0.9, 1.0, 1.5, 4.0], size=len(db))
+        seed = db[:, -1] * scale
+        stats = ft.DeviceArray(8, np.uint64).zero()
+        sd = ft.DeviceArray.from_host(seed)
+        if leaf_ids is None:
+            r, d = ft.self_knn_query(tree, k, seed_bound=sd, stats=stats)
+        else:
+            r, d = ft.self_knn_query_subset(tree, ids, k, seed_bound=sd, stats=stats)
+        assert np.array_equal(d, db) and np.array_equal(r, rb), k
+        assert stats.to_host()[3] == int(np.sum(seed < db[:, -1])), k
+
+
 def comb_points(ct):
     """63 points whose Morton keys are the 63 powers of two below 2^63, plus the
     origin: the radix tree is a comb 63 levels deep, deeper than the

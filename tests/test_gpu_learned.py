@@ -146,3 +146,59 @@ def test_training_learns_and_seeds_are_exact(ft):
           f"{s1[4] / n:.1f} seeded vs {s0[4] / n:.1f} without the network")
     assert reruns < 0.2 * n
     assert s1[4] < 0.9 * s0[4]
+
+
+def test_fp_model_seeds_subset_and_self_queries_exactly(ft):
+    """The learned bound on double coordinates (the SPH shape at test size):
+    a network trained on the GPU on exact k = 32 and k = 64 distances of one
+    periodic f64 snapshot learns (loss well below the constant predictor's);
+    its seeds leave the k = 64 subset query and the k = 32 self query
+    bit-identical to the unseeded packet walk, with few re-runs."""
+    from fasttree_amd import learned as L
+    tree = L.snapshot_tree(200_000, 12, coord="f64", n_halos=40)
+    assert tree.coord_type == ft.COORD_F64
+    eps = L.target_eps(tree)
+    assert 0 < eps < 1e-6  # 1e-12 * (root extent ~ 100)^2
+    rng = np.random.default_rng(0)
+    F = []
+    Y = []
+    for k in (32, 64):
+        f, y = L.examples(tree, k, 49_152, rng)
+        F.append(f)
+        Y.append(y)
+    F, Y = np.concatenate(F), np.concatenate(Y)
+    perm = rng.permutation(len(Y))
+    F, Y = F[perm], Y[perm]
+    m = L.MLP(L.init_params(5), L.norm_from(F, Y), 0.0)
+    dF = ft.DeviceArray.from_host(F.ravel(), np.float32)
+    dY = ft.DeviceArray.from_host(Y, np.float32)
+    first = None
+    for step in range(400):
+        b = step % 6
+        m.train_step(dF.ptr + b * 16384 * 16 * 4, dY.ptr + b * 16384 * 4, 16384, 3e-3)
+        if step == 0:
+            first = m.last_loss()
+    last = m.last_loss()
+    print(f"[learned f64] loss {first:.3f} -> {last:.3f}")
+    assert last < 0.5 * first and last < 0.6
+    resid = Y - m.forward(dF, len(Y))
+    m.set(margin=float(np.quantile(resid, 0.98)))
+    n = tree.num_leaves
+    ids = rng.permutation(n)[: n // 5].astype(np.int32)
+    dids = ft.DeviceArray.from_host(ids)
+    for k, sub in ((64, dids), (32, None)):
+        nq = n if sub is None else len(ids)
+        seeds = ft.DeviceArray(nq, np.float64)
+        scratch = ft.DeviceArray(nq * 16, np.float32)
+        m.seed(tree, k, seeds, scratch, leaf_ids=sub)
+        stats = ft.DeviceArray(8, np.uint64).zero()
+        if sub is None:
+            r1, d1 = ft.self_knn_query(tree, k, seed_bound=seeds, stats=stats)
+            r0, d0 = ft.self_knn_query(tree, k, algorithm=ft.ALGO_PACKET)
+        else:
+            r1, d1 = ft.self_knn_query_subset(tree, sub, k, seed_bound=seeds, stats=stats)
+            r0, d0 = ft.self_knn_query_subset(tree, sub, k, algorithm=ft.ALGO_PACKET)
+        assert np.array_equal(d1, d0) and np.array_equal(r1, r0), k
+        reruns = int(stats.to_host()[3])
+        print(f"[learned f64] k={k}: re-run fraction {reruns / nq:.4f}")
+        assert reruns < 0.2 * nq
