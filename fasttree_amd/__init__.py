@@ -86,6 +86,7 @@ EXPORTS = [
     "ft_stream_synchronize", "ft_event_create", "ft_event_destroy", "ft_event_record",
     "ft_event_elapsed_ms", "ft_tree_alloc", "ft_tree_ensure", "ft_tree_free",
     "ft_scratch_create", "ft_scratch_destroy", "ft_scratch_set_profiling", "ft_scratch_stage_ms",
+    "ft_scratch_set_radix_bits",
     "ft_compute_bbox", "ft_sfc_encode", "ft_build_tree", "ft_build_bvh", "ft_self_knn_query",
     "ft_self_knn_query_subset", "ft_knn_query", "ft_range_query", "ft_range_query_csr",
     "ft_mlp_num_params", "ft_mlp_create", "ft_mlp_destroy", "ft_mlp_set", "ft_mlp_get",
@@ -127,6 +128,7 @@ def lib():
         L.ft_scratch_create.argtypes = [C.POINTER(C.c_void_p)]
         L.ft_scratch_destroy.argtypes = [vp]
         L.ft_scratch_set_profiling.argtypes = [vp, i32]
+        L.ft_scratch_set_radix_bits.argtypes = [vp, i32]
         L.ft_scratch_stage_ms.argtypes = [vp, C.POINTER(C.c_double), i32, C.POINTER(C.c_int)]
         L.ft_compute_bbox.argtypes = [vp, C.c_int32, vp, vp, vp, sz, vp]
         L.ft_sfc_encode.argtypes = [vp, C.c_int32, C.POINTER(Config), vp, vp, vp, sz, vp, vp]
@@ -389,6 +391,10 @@ class BvhScratch:
 
     def set_profiling(self, on: bool):
         _check(lib().ft_scratch_set_profiling(self.handle, int(bool(on))))
+
+    def set_radix_bits(self, bits: int):
+        """Digit width of the build's radix sort: 8 (default) or 9 bits per pass."""
+        _check(lib().ft_scratch_set_radix_bits(self.handle, int(bits)))
 
     def stage_ms(self):
         ms = (C.c_double * 8)()

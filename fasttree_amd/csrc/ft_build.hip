@@ -168,19 +168,22 @@ __device__ inline uint64_t ph_encode(uint32_t ix, uint32_t iy, uint32_t iz, int 
     return key;
 }
 
-// Also accumulates the radix-sort digit histograms of every pass (8-bit
-// digits) so the sort does not read the keys once more just to count them.
+// Also accumulates the radix-sort digit histograms of every pass (digits of
+// rbits bits; pass p's bins at hist[p << rbits ...]) so the sort does not read
+// the keys once more just to count them.
+constexpr int ENC_HIST_WORDS = 8 * 512;  // up to 8 passes of 9-bit digits
 template <typename T>
 __global__ __launch_bounds__(256) void encode_kernel(const T *__restrict__ x,
                                                      const T *__restrict__ y,
                                                      const T *__restrict__ z, size_t n, int B,
                                                      int sfc, const BboxDev<T> *__restrict__ bb,
                                                      uint64_t *__restrict__ keys, int passes,
-                                                     uint32_t *__restrict__ hist) {
+                                                     int rbits, uint32_t *__restrict__ hist) {
     __shared__ uint16_t tab[384];
-    __shared__ uint32_t h[8][256];
+    __shared__ uint32_t h[ENC_HIST_WORDS];
+    const uint32_t dmask = (1u << rbits) - 1u;
     for (int i = threadIdx.x; i < 384; i += blockDim.x) tab[i] = c_ph_table[i];
-    for (int i = threadIdx.x; i < 8 * 256; i += blockDim.x) (&h[0][0])[i] = 0;
+    for (int i = threadIdx.x; i < ENC_HIST_WORDS; i += blockDim.x) h[i] = 0;
     __syncthreads();
     T lo[3] = {T(0), T(0), T(0)}, inv[3] = {T(0), T(0), T(0)};
     if constexpr (CoordTraits<T>::fp) {
@@ -215,64 +218,76 @@ __global__ __launch_bounds__(256) void encode_kernel(const T *__restrict__ x,
             const uint64_t act = __ballot(1);
             const int lead = __ffsll((long long)act) - 1;
             for (int p = 0; p < passes; ++p) {
-                const uint32_t dg = (uint32_t)(key >> (8 * p)) & 255u;
+                const uint32_t dg = (uint32_t)(key >> (rbits * p)) & dmask;
                 const uint32_t d0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)dg);
                 if (__ballot(dg == d0) == act) {
-                    if ((int)(threadIdx.x & 63) == lead) atomicAdd(&h[p][d0], (uint32_t)__popcll(act));
+                    if ((int)(threadIdx.x & 63) == lead)
+                        atomicAdd(&h[(p << rbits) + d0], (uint32_t)__popcll(act));
                 } else {
-                    atomicAdd(&h[p][dg], 1u);
+                    atomicAdd(&h[(p << rbits) + dg], 1u);
                 }
             }
         }
     }
     if (hist) {
         __syncthreads();
-        for (int i = threadIdx.x; i < passes * 256; i += blockDim.x) {
-            const uint32_t v = (&h[0][0])[i];
+        for (int i = threadIdx.x; i < (passes << rbits); i += blockDim.x) {
+            const uint32_t v = h[i];
             if (v) atomicAdd(&hist[i], v);
         }
     }
 }
 
 // ===========================================================================
-// K7: stable LSD radix sort of (key u64, index u32), 8-bit digits, one sweep per
-// digit with decoupled look-back (tile prefixes published per digit, read back
-// by later tiles), wave-level multisplit ranking (8 ballots per item) and an
-// LDS-staged scatter so each digit's run is written contiguously.
-// Equal keys keep their input order (stable).
+// K7: stable LSD radix sort of (key u64, index u32), digits of BITS = 8 or 9
+// bits, one sweep per digit with decoupled look-back (tile prefixes published
+// per digit, read back by later tiles), wave-level multisplit ranking (BITS
+// ballots per item) and an LDS-staged scatter so each digit's run is written
+// contiguously.  Equal keys keep their input order (stable), so every digit
+// width gives the same permutation.
 // ===========================================================================
-constexpr int RS_BLOCK = 256;
+constexpr int RS_BLOCK = 512;
 constexpr int RS_WAVES = RS_BLOCK / 64;
-constexpr int RS_ITEMS = 24;                   // keys per thread
+constexpr int RS_ITEMS = 12;                   // keys per thread
 constexpr int RS_TILE = RS_BLOCK * RS_ITEMS;   // 6144 keys per tile
-constexpr int RS_TILE_MIN = RS_BLOCK * 8;      // smallest tile: sizes the look-back words
+constexpr int RS_TILE_MIN = 2048;              // smallest tile: sizes the look-back words
+static_assert(RS_TILE_MIN <= RS_TILE, "look-back words are sized for tiles of at least RS_TILE_MIN keys");
 constexpr int RS_MAX_PASSES = 8;
+constexpr int RS_MAX_RADIX = 512;              // 9-bit digits
+constexpr int RS_HIST_WORDS = RS_MAX_PASSES * RS_MAX_RADIX;
+static_assert(RS_MAX_RADIX <= RS_BLOCK, "one thread per digit");
+static_assert(RS_HIST_WORDS <= ENC_HIST_WORDS, "encode histograms");
 
 __global__ void fill_u32_kernel(uint32_t *p, uint32_t v, size_t n) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
 }
 
-// Global digit histograms of every pass in one read of the keys.
+// Global digit histograms of every pass in one read of the keys: pass p's
+// bins are hist[p << rbits ...].
 __global__ __launch_bounds__(256) void radix_hist_kernel(const uint64_t *__restrict__ keys,
-                                                         size_t n, int passes,
+                                                         size_t n, int passes, int rbits,
                                                          uint32_t *__restrict__ hist) {
-    __shared__ uint32_t h[RS_MAX_PASSES][256];
-    for (int i = threadIdx.x; i < RS_MAX_PASSES * 256; i += blockDim.x) (&h[0][0])[i] = 0;
+    __shared__ uint32_t h[RS_HIST_WORDS];
+    const uint32_t mask = (1u << rbits) - 1u;
+    for (int i = threadIdx.x; i < RS_HIST_WORDS; i += blockDim.x) h[i] = 0;
     __syncthreads();
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (size_t)gridDim.x * blockDim.x) {
         uint64_t k = keys[i];
-        for (int p = 0; p < passes; ++p) atomicAdd(&h[p][(k >> (8 * p)) & 255u], 1u);
+        for (int p = 0; p < passes; ++p)
+            atomicAdd(&h[(p << rbits) + ((uint32_t)(k >> (rbits * p)) & mask)], 1u);
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < passes * 256; i += blockDim.x) {
-        uint32_t v = (&h[0][0])[i];
+    for (int i = threadIdx.x; i < (passes << rbits); i += blockDim.x) {
+        uint32_t v = h[i];
         if (v) atomicAdd(&hist[i], v);
     }
 }
 
-__device__ inline uint32_t block_excl_scan256(uint32_t v, uint32_t *s_tmp /*>= 4*/) {
+// Exclusive scan over the block (any number of waves): s_tmp holds one word
+// per wave.
+__device__ inline uint32_t block_excl_scan(uint32_t v, uint32_t *s_tmp) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     uint32_t x = v;
 #pragma unroll
@@ -288,17 +303,17 @@ __device__ inline uint32_t block_excl_scan256(uint32_t v, uint32_t *s_tmp /*>= 4
     return x - v + add;
 }
 
-// Exclusive scan of each pass's histogram -> digit base offsets; resets the
-// per-pass tile counters and advances the sort counter (epoch) that tags this
-// sort's look-back words.  Epochs run 1 .. 2^27-1 and then wrap to 1.
-__global__ __launch_bounds__(256) void radix_base_kernel(const uint32_t *__restrict__ hist,
-                                                         int passes, uint32_t *__restrict__ base,
-                                                         uint32_t *__restrict__ tile_ctr,
-                                                         uint32_t *__restrict__ epoch) {
-    __shared__ uint32_t tmp[4];
+// Exclusive scan of each pass's histogram -> digit base offsets (one thread
+// per digit: launched with 1 << rbits threads); resets the per-pass tile
+// counters and advances the sort counter (epoch) that tags this sort's
+// look-back words.  Epochs run 1 .. 2^27-1 and then wrap to 1.
+__global__ __launch_bounds__(RS_MAX_RADIX) void radix_base_kernel(
+    const uint32_t *__restrict__ hist, int passes, int rbits, uint32_t *__restrict__ base,
+    uint32_t *__restrict__ tile_ctr, uint32_t *__restrict__ epoch) {
+    __shared__ uint32_t tmp[RS_MAX_RADIX / 64];
     for (int p = 0; p < passes; ++p) {
-        uint32_t v = hist[p * 256 + threadIdx.x];
-        base[p * 256 + threadIdx.x] = block_excl_scan256(v, tmp);
+        uint32_t v = hist[(p << rbits) + threadIdx.x];
+        base[(p << rbits) + threadIdx.x] = block_excl_scan(v, tmp);
     }
     if (threadIdx.x < RS_MAX_PASSES) tile_ctr[threadIdx.x] = 0;
     if (threadIdx.x == 0) {
@@ -312,26 +327,33 @@ __device__ inline uint64_t lanemask_lt() {
     return lane ? (~0ull >> (64 - lane)) : 0ull;
 }
 
-template <bool FIRST>
+template <bool FIRST, int BITS>
 __global__ __launch_bounds__(RS_BLOCK) void onesweep_kernel(
     const uint64_t *__restrict__ keys_in, const uint32_t *__restrict__ vals_in,
     uint64_t *__restrict__ keys_out, uint32_t *__restrict__ vals_out, int n, int shift,
     const uint32_t *__restrict__ digit_base, uint64_t *__restrict__ lookback,
     const uint32_t *__restrict__ epoch, int pass, uint32_t *__restrict__ tile_ctr) {
-    // One staging buffer: keys first, then the values (as 32-bit words), with
-    // each position's digit kept for the second phase.
-    __shared__ uint64_t s_keys[RS_TILE];
-    __shared__ uint8_t s_dig[RS_TILE];
-    uint32_t *s_vals = (uint32_t *)s_keys;
-    __shared__ uint32_t s_wave[RS_WAVES][256];
-    __shared__ uint32_t s_local[256];
-    __shared__ uint32_t s_global[256];
-    __shared__ uint32_t s_tmp[4];
+    constexpr int RADIX = 1 << BITS;
+    constexpr uint32_t MASK = RADIX - 1;
+    static_assert(RADIX <= RS_MAX_RADIX, "digit width");
+    using dig_t = typename std::conditional<(BITS <= 8), uint8_t, uint16_t>::type;
+    // One staging buffer: the per-wave digit counters while the tile is
+    // ranked, then the keys, then the values (as 32-bit words); each
+    // position's digit is kept for the values.
+    constexpr int STAGE_BYTES = RS_TILE * 8 > RS_WAVES * RADIX * 4 ? RS_TILE * 8 : RS_WAVES * RADIX * 4;
+    __shared__ __align__(16) uint8_t s_stage[STAGE_BYTES];
+    uint64_t *s_keys = (uint64_t *)s_stage;
+    uint32_t *s_vals = (uint32_t *)s_stage;
+    uint32_t (*s_wave)[RADIX] = (uint32_t (*)[RADIX])s_stage;
+    __shared__ dig_t s_dig[RS_TILE];
+    __shared__ uint32_t s_local[RADIX];
+    __shared__ uint32_t s_global[RADIX];
+    __shared__ uint32_t s_tmp[RS_WAVES];
     __shared__ int s_tile;
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     if (tid == 0) s_tile = (int)atomicAdd(tile_ctr, 1u);
-    for (int i = tid; i < RS_WAVES * 256; i += RS_BLOCK) (&s_wave[0][0])[i] = 0;
+    for (int i = tid; i < RS_WAVES * RADIX; i += RS_BLOCK) (&s_wave[0][0])[i] = 0;
     __syncthreads();
     const int tile = s_tile;
     const int base = tile * RS_TILE + wave * (64 * RS_ITEMS);
@@ -353,10 +375,10 @@ __global__ __launch_bounds__(RS_BLOCK) void onesweep_kernel(
     for (int j = 0; j < RS_ITEMS; ++j) {
         const int idx = base + j * 64 + lane;
         const bool ok = idx < n;
-        const uint32_t d = (uint32_t)(k[j] >> shift) & 255u;
+        const uint32_t d = (uint32_t)(k[j] >> shift) & MASK;
         uint64_t peers = __ballot(ok);
 #pragma unroll
-        for (int b = 0; b < 8; ++b) {
+        for (int b = 0; b < BITS; ++b) {
             const bool bit = (d >> b) & 1u;
             const uint64_t m = __ballot(bit);
             peers &= bit ? m : ~m;
@@ -370,60 +392,72 @@ __global__ __launch_bounds__(RS_BLOCK) void onesweep_kernel(
     }
     __syncthreads();
 
-    // Per digit (thread = digit): wave offsets, tile count, look-back prefix.
-    const uint32_t dg = (uint32_t)tid;
+    // Per digit (thread = digit, the first RADIX threads): wave offsets, tile
+    // count, look-back prefix.
+    const uint32_t dg = (uint32_t)tid & MASK;
+    const bool dthr = tid < RADIX;
     uint32_t count = 0;
-#pragma unroll
-    for (int w = 0; w < RS_WAVES; ++w) {
-        uint32_t c = s_wave[w][dg];
-        s_wave[w][dg] = count;
-        count += c;
-    }
-    const uint64_t tag = (uint64_t)((*epoch * 8u + (uint32_t)pass) & 0x3FFFFFFFu) << 34;
-    uint64_t *slot = lookback + (size_t)tile * 256 + dg;
     uint32_t excl = 0;
-    if (tile == 0) {
-        __hip_atomic_store(slot, tag | (2ull << 32) | count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    } else {
-        __hip_atomic_store(slot, tag | (1ull << 32) | count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        int p = tile - 1;
-        while (true) {
-            const uint64_t w = __hip_atomic_load(lookback + (size_t)p * 256 + dg, __ATOMIC_RELAXED,
-                                                 __HIP_MEMORY_SCOPE_AGENT);
-            if ((w & ~0x3FFFFFFFFull) != tag || ((w >> 32) & 3u) == 0) {
-                __builtin_amdgcn_s_sleep(1);
-                continue;
-            }
-            excl += (uint32_t)w;
-            if (((w >> 32) & 3u) == 2u) break;
-            --p;
+    if (dthr) {
+#pragma unroll
+        for (int w = 0; w < RS_WAVES; ++w) {
+            uint32_t c = s_wave[w][dg];
+            s_wave[w][dg] = count;
+            count += c;
         }
-        __hip_atomic_store(slot, tag | (2ull << 32) | (uint32_t)(excl + count), __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
+        const uint64_t tag = (uint64_t)((*epoch * 8u + (uint32_t)pass) & 0x3FFFFFFFu) << 34;
+        uint64_t *slot = lookback + (size_t)tile * RADIX + dg;
+        if (tile == 0) {
+            __hip_atomic_store(slot, tag | (2ull << 32) | count, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            __hip_atomic_store(slot, tag | (1ull << 32) | count, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+            int p = tile - 1;
+            while (true) {
+                const uint64_t w = __hip_atomic_load(lookback + (size_t)p * RADIX + dg,
+                                                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if ((w & ~0x3FFFFFFFFull) != tag || ((w >> 32) & 3u) == 0) {
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                excl += (uint32_t)w;
+                if (((w >> 32) & 3u) == 2u) break;
+                --p;
+            }
+            __hip_atomic_store(slot, tag | (2ull << 32) | (uint32_t)(excl + count),
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
     }
-    const uint32_t local = block_excl_scan256(count, s_tmp);
-    s_local[dg] = local;
-    s_global[dg] = digit_base[dg] + excl;
+    const uint32_t local = block_excl_scan(count, s_tmp);  // the first RADIX threads hold the digits
+    if (dthr) {
+        s_local[dg] = local;
+        s_global[dg] = digit_base[dg] + excl;
+    }
     __syncthreads();
 
-    // Scatter into the tile's digit-sorted order in LDS: keys, then values.
+    // Tile positions first (the counters share the staging buffer), then the
+    // keys into the tile's digit-sorted order in LDS, then the values.
     uint32_t lpos[RS_ITEMS];
 #pragma unroll
     for (int j = 0; j < RS_ITEMS; ++j) {
         const int idx = base + j * 64 + lane;
         lpos[j] = 0xFFFFFFFFu;
         if (idx < n) {
-            const uint32_t d = (uint32_t)(k[j] >> shift) & 255u;
+            const uint32_t d = (uint32_t)(k[j] >> shift) & MASK;
             lpos[j] = s_local[d] + s_wave[wave][d] + rank[j];
-            s_keys[lpos[j]] = k[j];
         }
     }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < RS_ITEMS; ++j)
+        if (lpos[j] != 0xFFFFFFFFu) s_keys[lpos[j]] = k[j];
     __syncthreads();
     const int tile_n = min(RS_TILE, n - tile * RS_TILE);
     for (int i = tid; i < tile_n; i += RS_BLOCK) {
         const uint64_t key = s_keys[i];
-        const uint32_t d = (uint32_t)(key >> shift) & 255u;
-        s_dig[i] = (uint8_t)d;
+        const uint32_t d = (uint32_t)(key >> shift) & MASK;
+        s_dig[i] = (dig_t)d;
         keys_out[s_global[d] + ((uint32_t)i - s_local[d])] = key;
     }
     __syncthreads();
@@ -774,8 +808,8 @@ static int scratch_ensure(Scratch &S, size_t n) {
     const size_t o_perm = off; off += al(cap * 4);
     const size_t o_words = off; off += al(cap * 8);
     const size_t o_tops = off; off += al(cap * 4);
-    const size_t o_hist = off; off += al(RS_MAX_PASSES * 256 * 4 * 2);
-    const size_t o_look = off; off += al(tiles * 256 * 8);
+    const size_t o_hist = off; off += al(RS_HIST_WORDS * 4 * 2);
+    const size_t o_look = off; off += al(tiles * RS_MAX_RADIX * 8);
     const size_t o_tile = off; off += al(64);
     const size_t o_epoch = off; off += al(64);
     const size_t o_ctops = off; off += al(4 * (cap / ASM_CHUNK_MIN + 2));
@@ -806,7 +840,7 @@ static int scratch_ensure(Scratch &S, size_t n) {
     S.topo_epoch = (uint32_t *)(b + o_tepoch);
     S.hist = (uint32_t *)(b + o_hist);
     S.lookback = (uint64_t *)(b + o_look);
-    S.lookback_words = tiles * 256;
+    S.lookback_words = tiles * RS_MAX_RADIX;
     S.tile_ctr = (uint32_t *)(b + o_tile);
     S.epoch = (uint32_t *)(b + o_epoch);
     S.bbox_dev = (void *)(b + o_bbox);
@@ -843,34 +877,24 @@ static int launch_bbox(hipStream_t s, Scratch &S, const T *x, const T *y, const 
 template <typename T>
 static int launch_encode(hipStream_t s, Scratch &S, const ft_config &cfg, const T *x, const T *y,
                          const T *z, size_t n, uint64_t *keys, uint32_t *hist = nullptr,
-                         int passes = 0) {
+                         int passes = 0, int rbits = 8) {
     const unsigned g = grid_for(n, 256, 4096);
     hipLaunchKernelGGL(encode_kernel<T>, dim3(g), dim3(256), 0, s, x, y, z, n, cfg.bits_per_dim,
-                       cfg.sfc, (const BboxDev<T> *)S.bbox_dev, keys, passes, hist);
+                       cfg.sfc, (const BboxDev<T> *)S.bbox_dev, keys, passes, rbits, hist);
     FT_CHECK_LAUNCH();
     return FT_OK;
 }
 
 // Sorts keys_in (n keys of `key_bits` significant bits) into sorted_out, and
-// the permutation into S.perm.
-static inline int radix_passes(int key_bits) {
-    const int p = (key_bits + 7) / 8;
+// the permutation into S.perm, in passes of S.radix_bits-bit digits.
+static inline int radix_passes(int key_bits, int rbits) {
+    const int p = (key_bits + rbits - 1) / rbits;
     return p < 1 ? 1 : p;
 }
 
-static int radix_sort(hipStream_t s, Scratch &S, const uint64_t *keys_in, size_t n, int key_bits,
-                      uint64_t *sorted_out, bool hist_ready) {
-    const int passes = radix_passes(key_bits);
-    uint32_t *base = S.hist + RS_MAX_PASSES * 256;
-    if (!hist_ready) {
-        hipLaunchKernelGGL(fill_u32_kernel, dim3(grid_for(RS_MAX_PASSES * 256, 256)), dim3(256), 0,
-                           s, S.hist, 0u, (size_t)(RS_MAX_PASSES * 256));
-        hipLaunchKernelGGL(radix_hist_kernel, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s,
-                           keys_in, n, passes, S.hist);
-    }
-    hipLaunchKernelGGL(radix_base_kernel, dim3(1), dim3(256), 0, s, (const uint32_t *)S.hist,
-                       passes, base, S.tile_ctr, S.epoch);
-    FT_CHECK_LAUNCH();
+template <int BITS>
+static void launch_sweeps(hipStream_t s, Scratch &S, const uint64_t *keys_in, size_t n,
+                          int passes, const uint32_t *base, uint64_t *sorted_out) {
     const unsigned tiles = (unsigned)((n + RS_TILE - 1) / RS_TILE);
     const uint64_t *kin = keys_in;
     const uint32_t *vin = nullptr;
@@ -879,17 +903,36 @@ static int radix_sort(hipStream_t s, Scratch &S, const uint64_t *keys_in, size_t
         uint64_t *kout = last ? sorted_out : ((p & 1) ? S.keys_b : S.keys_a);
         uint32_t *vout = last ? S.perm : ((p & 1) ? S.vals_b : S.vals_a);
         if (p == 0)
-            hipLaunchKernelGGL(onesweep_kernel<true>, dim3(tiles), dim3(RS_BLOCK), 0, s, kin, vin,
-                               kout, vout, (int)n, 8 * p, base + p * 256, S.lookback,
-                               (const uint32_t *)S.epoch, p, S.tile_ctr + p);
+            hipLaunchKernelGGL((onesweep_kernel<true, BITS>), dim3(tiles), dim3(RS_BLOCK), 0, s,
+                               kin, vin, kout, vout, (int)n, BITS * p, base + (p << BITS),
+                               S.lookback, (const uint32_t *)S.epoch, p, S.tile_ctr + p);
         else
-            hipLaunchKernelGGL(onesweep_kernel<false>, dim3(tiles), dim3(RS_BLOCK), 0, s, kin, vin,
-                               kout, vout, (int)n, 8 * p, base + p * 256, S.lookback,
-                               (const uint32_t *)S.epoch, p, S.tile_ctr + p);
-        FT_CHECK_LAUNCH();
+            hipLaunchKernelGGL((onesweep_kernel<false, BITS>), dim3(tiles), dim3(RS_BLOCK), 0, s,
+                               kin, vin, kout, vout, (int)n, BITS * p, base + (p << BITS),
+                               S.lookback, (const uint32_t *)S.epoch, p, S.tile_ctr + p);
         kin = kout;
         vin = vout;
     }
+}
+
+static int radix_sort(hipStream_t s, Scratch &S, const uint64_t *keys_in, size_t n, int key_bits,
+                      uint64_t *sorted_out, bool hist_ready) {
+    const int rbits = S.radix_bits;
+    const int passes = radix_passes(key_bits, rbits);
+    if (passes > RS_MAX_PASSES) return fail(FT_EINVAL, "radix sort: too many passes");
+    uint32_t *base = S.hist + RS_HIST_WORDS;
+    if (!hist_ready) {
+        hipLaunchKernelGGL(fill_u32_kernel, dim3(grid_for(RS_HIST_WORDS, 256)), dim3(256), 0, s,
+                           S.hist, 0u, (size_t)RS_HIST_WORDS);
+        hipLaunchKernelGGL(radix_hist_kernel, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s,
+                           keys_in, n, passes, rbits, S.hist);
+    }
+    hipLaunchKernelGGL(radix_base_kernel, dim3(1), dim3(1 << rbits), 0, s, (const uint32_t *)S.hist,
+                       passes, rbits, base, S.tile_ctr, S.epoch);
+    FT_CHECK_LAUNCH();
+    if (rbits == 9) launch_sweeps<9>(s, S, keys_in, n, passes, base, sorted_out);
+    else launch_sweeps<8>(s, S, keys_in, n, passes, base, sorted_out);
+    FT_CHECK_LAUNCH();
     return FT_OK;
 }
 
@@ -942,9 +985,10 @@ int build_bvh_t(hipStream_t s, ft_tree *t, Scratch &S, const T *x, const T *y, c
     if (!keys) {
         if constexpr (CoordTraits<T>::fp) FT_TRY(launch_bbox<T>(s, S, x, y, z, n, bbox6));
         prof_mark(S, s, 1);
-        hipLaunchKernelGGL(fill_u32_kernel, dim3(grid_for(RS_MAX_PASSES * 256, 256)), dim3(256),
-                           0, s, S.hist, 0u, (size_t)(RS_MAX_PASSES * 256));
-        FT_TRY(launch_encode<T>(s, S, cfg, x, y, z, n, S.keys_in, S.hist, radix_passes(key_bits)));
+        hipLaunchKernelGGL(fill_u32_kernel, dim3(grid_for(RS_HIST_WORDS, 256)), dim3(256), 0, s,
+                           S.hist, 0u, (size_t)RS_HIST_WORDS);
+        FT_TRY(launch_encode<T>(s, S, cfg, x, y, z, n, S.keys_in, S.hist,
+                                radix_passes(key_bits, S.radix_bits), S.radix_bits));
         kin = S.keys_in;
     } else {
         prof_mark(S, s, 1);

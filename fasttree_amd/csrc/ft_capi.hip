@@ -324,6 +324,12 @@ int ft_scratch_set_profiling(ft_scratch *scratch, int on) {
     scratch->s.profiling = on;
     return FT_OK;
 }
+int ft_scratch_set_radix_bits(ft_scratch *scratch, int bits) {
+    if (!scratch) return fail(FT_EINVAL, "null scratch");
+    if (bits != 8 && bits != 9) return fail(FT_EINVAL, "radix bits must be 8 or 9");
+    scratch->s.radix_bits = bits;
+    return FT_OK;
+}
 int ft_scratch_stage_ms(ft_scratch *scratch, double *ms, int max_stages, int *n_stages) {
     if (!scratch) return fail(FT_EINVAL, "null scratch");
     FT_TRY(collect_profile(scratch->s));

@@ -285,13 +285,14 @@ struct Scratch {
     int32_t *tops = nullptr;         // tree assembly: labels of the chunk tops, by chunk (n)
     uint32_t *chunk_tops = nullptr;  // tree assembly: chunk tops per chunk
     uint32_t *topo_epoch = nullptr;  // tree assembly counter: tags the arrival words
-    uint32_t *hist = nullptr;     // 8 passes x 256
-    uint64_t *lookback = nullptr; // tiles x 256 status words
+    uint32_t *hist = nullptr;     // 8 passes x 512 digit counts, then 8 x 512 digit bases
+    uint64_t *lookback = nullptr; // tiles x 512 status words
     size_t lookback_words = 0;
     uint32_t *tile_ctr = nullptr; // 8 passes
     uint32_t *epoch = nullptr;    // sort counter on the device: look-back tags of a sort
                                   // are 8 * epoch + pass, so no tag is reused and a
                                   // captured graph that replays the sort stays correct
+    int radix_bits = 8;           // sort digit width: 8 or 9 bits per pass
     void *bbox_dev = nullptr;     // 6 coords + 3 inverse extents (16 x 8 bytes)
     void *partials = nullptr;     // bbox partials (1024 x 6 x 8 bytes)
     // staging of inputs (host pointers)

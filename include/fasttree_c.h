@@ -174,6 +174,10 @@ int ft_scratch_set_profiling(ft_scratch *scratch, int on);
 /* ms of the last profiled build per stage: bbox, encode, sort, assemble (leaf gather,  */
 /* topology and in-chunk boxes), assemble_top (nodes spanning chunks).                  */
 int ft_scratch_stage_ms(ft_scratch *scratch, double *ms, int max_stages, int *n_stages);
+/* Digit width of the next builds' radix sort: 8 (default) or 9 bits per pass (63-bit */
+/* keys: 8 or 7 passes).  The sorted order is the same; only the speed differs.  No   */
+/* reference counterpart (its sort is oneDPL's); the batched tuner sets it per size.  */
+int ft_scratch_set_radix_bits(ft_scratch *scratch, int bits);
 
 /* ---- build ------------------------------------------------------------- */
 /* compute_bbox (src/hlbvh.hpp:462-578).  out6 (host) = min_x,max_x,min_y,max_y,min_z,max_z. */

@@ -83,6 +83,41 @@ def test_build_matches_oracle(ft, orc, case):
     compare_trees(g, t, n)
 
 
+RADIX_CASES = [c for c in CASES if c[0] in ("ref-clustered-f64-ph", "ref-degenerate-u32-morton",
+                                             "uniform-f64-morton10", "nfw-u64-ph21", "tiny-2")]
+
+
+@pytest.mark.parametrize("case", RADIX_CASES, ids=[c[0] for c in RADIX_CASES])
+def test_build_with_9_bit_digits_matches_oracle(ft, orc, case):
+    """The radix sort in passes of 9-bit digits (7 passes for 63-bit keys, 4 for
+    30-bit) gives the same stable order: every array equals the oracle's,
+    including the degenerate set's runs of equal keys; so do keys given by the
+    caller with many duplicates."""
+    name, ct, sfc, bits, periodic, gen = case
+    print(f"[build 9-bit] {name}", flush=True)
+    x, y, z = make_points(gen, ct)
+    sc = ft.BvhScratch()
+    sc.set_radix_bits(9)
+    tree = gpu_build(ft, ct, x, y, z, sfc, bits, periodic, scratch=sc)
+    t = O.build_bvh(ct, x, y, z, sfc=sfc, bits=bits, periodic=periodic)
+    compare_trees(tree.to_host(), t, len(x))
+    if len(x) > 1000:
+        keys = (np.arange(len(x), dtype=np.uint64) * 2654435761) % 997
+        tree2 = gpu_build(ft, ct, x, y, z, sfc, bits, periodic, scratch=sc,
+                          keys=ft.DeviceArray.from_host(keys, np.uint64))
+        t2 = O.build_bvh(ct, x, y, z, sfc=sfc, bits=bits, periodic=periodic, keys=keys)
+        compare_trees(tree2.to_host(), t2, len(x))
+
+
+def test_radix_bits_argument(ft):
+    sc = ft.BvhScratch()
+    for bad in (0, 7, 10, 16):
+        with pytest.raises(Exception):
+            sc.set_radix_bits(bad)
+    sc.set_radix_bits(9)
+    sc.set_radix_bits(8)
+
+
 def test_single_particle(ft, orc):
     x = np.array([5.0]); y = np.array([6.0]); z = np.array([7.0])
     tree = gpu_build(ft, O.CT_F64, x, y, z, O.SFC_MORTON, 10)
@@ -185,3 +220,10 @@ def test_full_size_build_properties(ft, orc, n):
     assert np.array_equal(g["min_x"][o:], x[perm])
     t = O.build_bvh(O.CT_U64, x, y, z, sfc=O.SFC_PH, bits=21, periodic=1)
     compare_trees(g, t, n)
+    # the same snapshot sorted in 7 passes of 9-bit digits: the same tree
+    sc = ft.BvhScratch()
+    sc.set_radix_bits(9)
+    g9 = gpu_build(ft, O.CT_U64, x, y, z, O.SFC_PH, 21, periodic=1, scratch=sc).to_host()
+    for a in ("sorted_keys", "orig_idx", "left_child", "right_child", "parent", "range_first",
+              "range_last", *ARRAYS):
+        assert np.array_equal(g9[a], g[a]), a
