@@ -393,7 +393,8 @@ class BvhScratch:
         _check(lib().ft_scratch_set_profiling(self.handle, int(bool(on))))
 
     def set_radix_bits(self, bits: int):
-        """Digit width of the build's radix sort: 8 (default) or 9 bits per pass."""
+        """Digit width of the build's radix sort: 8 or 9 bits per pass, or 0 (the
+        default) for the width with fewer passes."""
         _check(lib().ft_scratch_set_radix_bits(self.handle, int(bits)))
 
     def stage_ms(self):

@@ -16,8 +16,9 @@
 //   4. re-runs exactly, one lane per query, every query whose packed keys
 //      could not order its results.
 // The autotuner times candidate settings per (tree-size class, k tier) on the
-// bound data -- tier and leaf bucket for the queries, branch count for the
-// builds -- and keeps the fastest.  Results are canonical (ascending (d2,
+// bound data -- tier and leaf bucket for the queries; per size class the
+// radix width of the builds' sort; the branch count of the builds -- and
+// keeps the fastest.  Results are canonical (ascending (d2,
 // index)), so every setting gives the same output.
 #include "ft_traverse.hpp"
 
@@ -380,6 +381,7 @@ struct ft_batch {
     uint64_t *stats = nullptr;
     // tuning
     ft::GroupTune tune[ft::N_CLASS][ft::N_KTIER];
+    int radix_bits[ft::N_CLASS] = {0, 0, 0, 0};  // sort digit width per tree-size class (0: auto)
     int branches = 4;
     bool tuned = false;
     // device state
@@ -725,6 +727,7 @@ template <typename T> static int enqueue_builds(ft_batch *b, hipStream_t s, int 
         tr->num_leaves = n;
         tr->num_internal = n - 1;
         hipStream_t st = nb > 1 ? b->side[br[i]] : s;
+        b->build_scratch[br[i]].radix_bits = b->radix_bits[size_class(n)];
         FT_TRY(build_bvh_t<T>(st, tr, b->build_scratch[br[i]], (const T *)b->bx[i],
                               (const T *)b->by[i], (const T *)b->bz[i], nullptr, nullptr, n,
                               nullptr, nullptr));
@@ -804,8 +807,9 @@ static float elapsed(hipEvent_t a, hipEvent_t c) {
 
 // Autotuner.  Builds once, sorts the queries once, then for every (size class,
 // k tier) group with work times each candidate tier kernel (and leaf bucket for
-// packet tiers) on that group's items alone, keeping the fastest; then times
-// the build phase at 1, 2, 4 and 8 branches.
+// packet tiers) on that group's items alone, keeping the fastest; then, per
+// size class, that class's builds with 8- and 9-bit sort digits; then the
+// build phase at 1, 2, 4 and 8 branches.
 template <typename T> static int tune_t(ft_batch *b, hipStream_t s) {
     const bool fp = CoordTraits<T>::fp;
     default_tuning(b);
@@ -892,6 +896,45 @@ template <typename T> static int tune_t(ft_batch *b, hipStream_t s) {
             g.ms_per_item = best / std::max(best_items, 1);
             g.tuned = true;
         }
+    }
+    // Radix width per size class: that class's builds, in order on s, with
+    // 8- and 9-bit digits (the sorted order is the same).
+    if (rc == FT_OK && b->build) {
+        const int nt = (int)b->trees.size();
+        if (b->build_scratch.empty()) b->build_scratch.emplace_back();
+        Scratch &S0 = b->build_scratch[0];
+        for (int c = 0; c < N_CLASS && rc == FT_OK; ++c) {
+            size_t maxn = 0;
+            for (int i = 0; i < nt; ++i)
+                if (b->bn[i] > 0 && size_class(b->bn[i]) == c) maxn = std::max(maxn, b->bn[i]);
+            if (maxn == 0) continue;
+            rc = scratch_ensure_public(S0, maxn);
+            float best = 1e30f;
+            int best_bits = 8;
+            for (int bits : {8, 9}) {
+                S0.radix_bits = bits;
+                float t_best = 1e30f;
+                for (int rep = 0; rep < 2 && rc == FT_OK; ++rep) {
+                    (void)hipEventRecord(e0, s);
+                    for (int i = 0; i < nt && rc == FT_OK; ++i) {
+                        const size_t n = b->bn[i];
+                        if (n == 0 || size_class(n) != c) continue;
+                        rc = build_bvh_t<T>(s, b->trees[i], S0, (const T *)b->bx[i],
+                                            (const T *)b->by[i], (const T *)b->bz[i], nullptr,
+                                            nullptr, n, nullptr, nullptr);
+                    }
+                    (void)hipEventRecord(e1, s);
+                    (void)hipEventSynchronize(e1);
+                    t_best = std::min(t_best, elapsed(e0, e1));
+                }
+                if (t_best < best) {
+                    best = t_best;
+                    best_bits = bits;
+                }
+            }
+            b->radix_bits[c] = best_bits;
+        }
+        S0.radix_bits = 0;
     }
     // Build branches.
     if (rc == FT_OK && b->build && b->trees.size() > 1) {
@@ -1099,9 +1142,12 @@ int ft_batch_tuning(ft_batch *b, char *buf, size_t len) {
     std::string out = "{\"tuned\": ";
     out += b->tuned ? "true" : "false";
     out += ", \"build_branches\": " + std::to_string(b->branches);
-    out += ", \"radix_bits\": 8, \"groups\": [";
-    bool first = true;
     static const char *const cls[N_CLASS] = {"<40k", "<160k", "<640k", ">=640k"};
+    out += ", \"radix_bits\": {";
+    for (int c = 0; c < N_CLASS; ++c)
+        out += std::string(c ? ", " : "") + "\"" + cls[c] + "\": " + std::to_string(b->radix_bits[c]);
+    out += "}, \"groups\": [";
+    bool first = true;
     for (int c = 0; c < N_CLASS; ++c)
         for (int t = 0; t < N_KTIER; ++t) {
             const GroupTune &g = b->tune[c][t];

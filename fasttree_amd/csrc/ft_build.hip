@@ -891,6 +891,12 @@ static inline int radix_passes(int key_bits, int rbits) {
     const int p = (key_bits + rbits - 1) / rbits;
     return p < 1 ? 1 : p;
 }
+// The digit width a sort of key_bits-bit keys uses: the scratch's setting, or
+// 9 bits when that saves a pass (63-bit keys: 7 passes instead of 8).
+static inline int radix_width(const Scratch &S, int key_bits) {
+    if (S.radix_bits) return S.radix_bits;
+    return radix_passes(key_bits, 9) < radix_passes(key_bits, 8) ? 9 : 8;
+}
 
 template <int BITS>
 static void launch_sweeps(hipStream_t s, Scratch &S, const uint64_t *keys_in, size_t n,
@@ -917,7 +923,7 @@ static void launch_sweeps(hipStream_t s, Scratch &S, const uint64_t *keys_in, si
 
 static int radix_sort(hipStream_t s, Scratch &S, const uint64_t *keys_in, size_t n, int key_bits,
                       uint64_t *sorted_out, bool hist_ready) {
-    const int rbits = S.radix_bits;
+    const int rbits = radix_width(S, key_bits);
     const int passes = radix_passes(key_bits, rbits);
     if (passes > RS_MAX_PASSES) return fail(FT_EINVAL, "radix sort: too many passes");
     uint32_t *base = S.hist + RS_HIST_WORDS;
@@ -987,8 +993,9 @@ int build_bvh_t(hipStream_t s, ft_tree *t, Scratch &S, const T *x, const T *y, c
         prof_mark(S, s, 1);
         hipLaunchKernelGGL(fill_u32_kernel, dim3(grid_for(RS_HIST_WORDS, 256)), dim3(256), 0, s,
                            S.hist, 0u, (size_t)RS_HIST_WORDS);
+        const int rbits = radix_width(S, key_bits);
         FT_TRY(launch_encode<T>(s, S, cfg, x, y, z, n, S.keys_in, S.hist,
-                                radix_passes(key_bits, S.radix_bits), S.radix_bits));
+                                radix_passes(key_bits, rbits), rbits));
         kin = S.keys_in;
     } else {
         prof_mark(S, s, 1);

@@ -326,7 +326,7 @@ int ft_scratch_set_profiling(ft_scratch *scratch, int on) {
 }
 int ft_scratch_set_radix_bits(ft_scratch *scratch, int bits) {
     if (!scratch) return fail(FT_EINVAL, "null scratch");
-    if (bits != 8 && bits != 9) return fail(FT_EINVAL, "radix bits must be 8 or 9");
+    if (bits != 0 && bits != 8 && bits != 9) return fail(FT_EINVAL, "radix bits must be 0, 8 or 9");
     scratch->s.radix_bits = bits;
     return FT_OK;
 }

@@ -292,7 +292,8 @@ struct Scratch {
     uint32_t *epoch = nullptr;    // sort counter on the device: look-back tags of a sort
                                   // are 8 * epoch + pass, so no tag is reused and a
                                   // captured graph that replays the sort stays correct
-    int radix_bits = 8;           // sort digit width: 8 or 9 bits per pass
+    int radix_bits = 0;           // sort digit width: 8 or 9 bits per pass, 0 = the one
+                                  // with fewer passes (8 on a tie)
     void *bbox_dev = nullptr;     // 6 coords + 3 inverse extents (16 x 8 bytes)
     void *partials = nullptr;     // bbox partials (1024 x 6 x 8 bytes)
     // staging of inputs (host pointers)

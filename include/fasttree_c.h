@@ -174,9 +174,10 @@ int ft_scratch_set_profiling(ft_scratch *scratch, int on);
 /* ms of the last profiled build per stage: bbox, encode, sort, assemble (leaf gather,  */
 /* topology and in-chunk boxes), assemble_top (nodes spanning chunks).                  */
 int ft_scratch_stage_ms(ft_scratch *scratch, double *ms, int max_stages, int *n_stages);
-/* Digit width of the next builds' radix sort: 8 (default) or 9 bits per pass (63-bit */
-/* keys: 8 or 7 passes).  The sorted order is the same; only the speed differs.  No   */
-/* reference counterpart (its sort is oneDPL's); the batched tuner sets it per size.  */
+/* Digit width of the next builds' radix sort: 8 or 9 bits per pass (63-bit keys: 8  */
+/* or 7 passes), or 0 (default) for the width with fewer passes, 8 on a tie.  The     */
+/* sorted order is the same; only the speed differs.  No reference counterpart (its   */
+/* sort is oneDPL's); the batched tuner sets it per tree-size class.                  */
 int ft_scratch_set_radix_bits(ft_scratch *scratch, int bits);
 
 /* ---- build ------------------------------------------------------------- */

@@ -115,6 +115,11 @@ def test_batch_run_tune_and_graph_replays(ft, orc, ct):
     b.tune()
     tun = b.tuning()
     assert tun["tuned"] and tun["groups"]
+    assert set(tun["radix_bits"]) == {"<40k", "<160k", "<640k", ">=640k"}
+    # 8 or 9 for the size classes the batch's trees fall in; 0 (automatic)
+    # for classes with no tree
+    assert all(v in (0, 8, 9) for v in tun["radix_bits"].values())
+    assert any(v in (8, 9) for v in tun["radix_bits"].values())
     print(f"[batch] tuning {tun}", flush=True)
     for rep in range(3):
         res.zero()

@@ -89,8 +89,8 @@ RADIX_CASES = [c for c in CASES if c[0] in ("ref-clustered-f64-ph", "ref-degener
 
 @pytest.mark.parametrize("case", RADIX_CASES, ids=[c[0] for c in RADIX_CASES])
 def test_build_with_9_bit_digits_matches_oracle(ft, orc, case):
-    """The radix sort in passes of 9-bit digits (7 passes for 63-bit keys, 4 for
-    30-bit) gives the same stable order: every array equals the oracle's,
+    """The radix sort forced to 9-bit digits (7 passes for 63-bit keys, 4 for
+    30-bit keys, where the default is 8-bit) gives the same stable order: every array equals the oracle's,
     including the degenerate set's runs of equal keys; so do keys given by the
     caller with many duplicates."""
     name, ct, sfc, bits, periodic, gen = case
@@ -111,11 +111,12 @@ def test_build_with_9_bit_digits_matches_oracle(ft, orc, case):
 
 def test_radix_bits_argument(ft):
     sc = ft.BvhScratch()
-    for bad in (0, 7, 10, 16):
+    for bad in (-1, 7, 10, 16):
         with pytest.raises(Exception):
             sc.set_radix_bits(bad)
     sc.set_radix_bits(9)
     sc.set_radix_bits(8)
+    sc.set_radix_bits(0)
 
 
 def test_single_particle(ft, orc):
@@ -220,9 +221,10 @@ def test_full_size_build_properties(ft, orc, n):
     assert np.array_equal(g["min_x"][o:], x[perm])
     t = O.build_bvh(O.CT_U64, x, y, z, sfc=O.SFC_PH, bits=21, periodic=1)
     compare_trees(g, t, n)
-    # the same snapshot sorted in 7 passes of 9-bit digits: the same tree
+    # the same snapshot sorted in 8 passes of 8-bit digits (the default for
+    # 63-bit keys is 7 passes of 9 bits): the same tree
     sc = ft.BvhScratch()
-    sc.set_radix_bits(9)
+    sc.set_radix_bits(8)
     g9 = gpu_build(ft, O.CT_U64, x, y, z, O.SFC_PH, 21, periodic=1, scratch=sc).to_host()
     for a in ("sorted_keys", "orig_idx", "left_child", "right_child", "parent", "range_first",
               "range_last", *ARRAYS):
