@@ -168,9 +168,15 @@ __device__ inline uint64_t ph_encode(uint32_t ix, uint32_t iy, uint32_t iz, int 
     return key;
 }
 
+// A point as one aligned record of four coordinates (x, y, z, 0).
+template <typename T> struct alignas(4 * sizeof(T)) Point4 {
+    T v[4];
+};
+
 // Also accumulates the radix-sort digit histograms of every pass (digits of
 // rbits bits; pass p's bins at hist[p << rbits ...]) so the sort does not read
-// the keys once more just to count them.
+// the keys once more just to count them.  With `pts`, it also writes each
+// point as one record for the leaf gather.
 constexpr int ENC_HIST_WORDS = 8 * 512;  // up to 8 passes of 9-bit digits
 template <typename T>
 __global__ __launch_bounds__(256) void encode_kernel(const T *__restrict__ x,
@@ -178,7 +184,8 @@ __global__ __launch_bounds__(256) void encode_kernel(const T *__restrict__ x,
                                                      const T *__restrict__ z, size_t n, int B,
                                                      int sfc, const BboxDev<T> *__restrict__ bb,
                                                      uint64_t *__restrict__ keys, int passes,
-                                                     int rbits, uint32_t *__restrict__ hist) {
+                                                     int rbits, uint32_t *__restrict__ hist,
+                                                     Point4<T> *__restrict__ pts) {
     __shared__ uint16_t tab[384];
     __shared__ uint32_t h[ENC_HIST_WORDS];
     const uint32_t dmask = (1u << rbits) - 1u;
@@ -196,6 +203,7 @@ __global__ __launch_bounds__(256) void encode_kernel(const T *__restrict__ x,
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (size_t)gridDim.x * blockDim.x) {
         const T p[3] = {x[i], y[i], z[i]};
+        if (pts) pts[i] = Point4<T>{{p[0], p[1], p[2], T(0)}};
         uint32_t q[3];
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
@@ -527,6 +535,7 @@ template <typename T> struct LeafSrc {
     const int8_t *ghost;
     const int32_t *orig;    // LEAF_SORTED only
     const uint32_t *perm;   // LEAF_GATHER only
+    const Point4<T> *pts;   // LEAF_GATHER: the points as records (null: x, y, z)
 };
 
 template <typename T> struct TreeOut {
@@ -593,7 +602,12 @@ __global__ __launch_bounds__(C) void assemble_kernel(int n, const uint64_t *__re
         const int leaf = off + i;
         if constexpr (LEAF == LEAF_GATHER) {
             const uint32_t q = src.perm[i];
-            px = src.x[q]; py = src.y[q]; pz = src.z[q];
+            if (src.pts) {
+                const Point4<T> p4 = src.pts[q];
+                px = p4.v[0]; py = p4.v[1]; pz = p4.v[2];
+            } else {
+                px = src.x[q]; py = src.y[q]; pz = src.z[q];
+            }
             o.id[leaf] = src.id ? src.id[q] : q;
             o.ghost[leaf] = src.ghost ? src.ghost[q] : (int8_t)0;
             o.orig[leaf] = (int32_t)q;
@@ -816,6 +830,7 @@ static int scratch_ensure(Scratch &S, size_t n) {
     const size_t o_tepoch = off; off += al(64);
     const size_t o_bbox = off; off += al(16 * 8);
     const size_t o_part = off; off += al(1024 * 6 * 8);
+    const size_t o_pts = off; off += al(cap * 32);
     void *blk = nullptr;
     if (hipMalloc(&blk, off) != hipSuccess) return fail(FT_ENOMEM, "scratch allocation failed");
     // One-time clear: look-back and arrival words carry an epoch, so they never
@@ -845,6 +860,7 @@ static int scratch_ensure(Scratch &S, size_t n) {
     S.epoch = (uint32_t *)(b + o_epoch);
     S.bbox_dev = (void *)(b + o_bbox);
     S.partials = (void *)(b + o_part);
+    S.points = (void *)(b + o_pts);
     return FT_OK;
 }
 
@@ -877,10 +893,10 @@ static int launch_bbox(hipStream_t s, Scratch &S, const T *x, const T *y, const 
 template <typename T>
 static int launch_encode(hipStream_t s, Scratch &S, const ft_config &cfg, const T *x, const T *y,
                          const T *z, size_t n, uint64_t *keys, uint32_t *hist = nullptr,
-                         int passes = 0, int rbits = 8) {
+                         int passes = 0, int rbits = 8, Point4<T> *pts = nullptr) {
     const unsigned g = grid_for(n, 256, 4096);
     hipLaunchKernelGGL(encode_kernel<T>, dim3(g), dim3(256), 0, s, x, y, z, n, cfg.bits_per_dim,
-                       cfg.sfc, (const BboxDev<T> *)S.bbox_dev, keys, passes, rbits, hist);
+                       cfg.sfc, (const BboxDev<T> *)S.bbox_dev, keys, passes, rbits, hist, pts);
     FT_CHECK_LAUNCH();
     return FT_OK;
 }
@@ -995,7 +1011,7 @@ int build_bvh_t(hipStream_t s, ft_tree *t, Scratch &S, const T *x, const T *y, c
                            S.hist, 0u, (size_t)RS_HIST_WORDS);
         const int rbits = radix_width(S, key_bits);
         FT_TRY(launch_encode<T>(s, S, cfg, x, y, z, n, S.keys_in, S.hist,
-                                radix_passes(key_bits, rbits), rbits));
+                                radix_passes(key_bits, rbits), rbits, (Point4<T> *)S.points));
         kin = S.keys_in;
     } else {
         prof_mark(S, s, 1);
@@ -1003,7 +1019,9 @@ int build_bvh_t(hipStream_t s, ft_tree *t, Scratch &S, const T *x, const T *y, c
     prof_mark(S, s, 2);
     FT_TRY(radix_sort(s, S, kin, n, key_bits, t->sorted_keys, keys == nullptr));
     prof_mark(S, s, 3);
-    LeafSrc<T> src{x, y, z, id, ghost, nullptr, (const uint32_t *)S.perm};
+    // keys encoded here: the encode pass also wrote the points as records
+    LeafSrc<T> src{x, y, z, id, ghost, nullptr, (const uint32_t *)S.perm,
+                   keys ? nullptr : (const Point4<T> *)S.points};
     FT_TRY((launch_assemble<T, LEAF_GATHER>(s, t, S, src)));
     prof_mark(S, s, 5);
     if (S.profiling) S.n_stages = 5;

@@ -281,6 +281,9 @@ struct Scratch {
     uint64_t *keys_a = nullptr, *keys_b = nullptr;
     uint32_t *vals_a = nullptr, *vals_b = nullptr;
     uint32_t *perm = nullptr;     // final permutation (n)
+    void *points = nullptr;       // encode's copy of the input points, (x, y, z, 0) per
+                                  // point (4 coordinates, 32 bytes at most): the leaf
+                                  // gather reads one record instead of three arrays
     uint64_t *split_words = nullptr; // tree assembly: arrival word per split (n), epoch-tagged
     int32_t *tops = nullptr;         // tree assembly: labels of the chunk tops, by chunk (n)
     uint32_t *chunk_tops = nullptr;  // tree assembly: chunk tops per chunk
