@@ -160,6 +160,8 @@ struct TreePool {
 struct BvhScratch {
     ft_scratch *s = nullptr;
     BvhScratch() { detail::check(ft_scratch_create(&s)); }
+    // Radix sort digit width of the builds: 8 or 9 bits, 0 = fewer passes.
+    void set_radix_bits(int bits) { detail::check(ft_scratch_set_radix_bits(s, bits)); }
     void free_all(hipStream_t) {
         ft_scratch_destroy(s);
         s = nullptr;
