@@ -49,8 +49,13 @@ __device__ inline int slot_query(const QueryMap &qm, int slot) {
 // Per-lane DFS with the canonical register top-k (k <= M <= 32): the
 // reference's grouped self-kNN / knn_query_small_k walk on wave64 lanes.
 // Direct passes use the reference's 64-entry stack and flag a query whose
-// walk would need more; LISTED passes (the exact re-runs) use 128 entries.
+// walk would need more; LISTED passes (the exact re-runs) use 128 entries,
+// and for self queries first insert the leaves within LISTED_WINDOW ranks of
+// the query's own leaf, so that the walk starts with a finite bound and skips
+// those leaves (the top-k set does not depend on the order of insertion).
 // ===========================================================================
+constexpr int LISTED_WINDOW = 32;
+
 template <typename T, int M, bool GROUPED, bool LISTED, int MK>
 __global__ __launch_bounds__(LANE_BLOCK) void lane_knn_reg_kernel(
     DTree<T> t, QSrc<T> qs, int nq, int k, int excl, QueryMap qm, uint64_t *__restrict__ res,
@@ -73,9 +78,27 @@ __global__ __launch_bounds__(LANE_BLOCK) void lane_knn_reg_kernel(
         if (!excl) self = -1;
         TopK<D, M> h;
         h.init(k, dist_max<D>());
+        int wlo = 0, whi = 0;
+        if constexpr (LISTED) {
+            if (active && self >= 0) {
+                const int off = t.n - 1;
+                wlo = self - LISTED_WINDOW > 0 ? self - LISTED_WINDOW : 0;
+                whi = self + LISTED_WINDOW + 1 < t.n ? self + LISTED_WINDOW + 1 : t.n;
+                for (int r = wlo; r < whi; ++r) {
+                    if (r == self) continue;
+                    const int leaf = off + r;
+                    const D d2 = leaf_d2_k<T, MK>(t.m, px, py, pz, t.mnx[leaf], t.mny[leaf],
+                                                  t.mnz[leaf]);
+                    const int32_t ri = ret_index(t, leaf);
+                    const bool take = h.beats_worst(d2, ri);
+                    h.insert(take, d2, ri);
+                    inserts += take ? 1 : 0;
+                }
+            }
+        }
         const bool ovf = lane_reg_walk<T, M, GROUPED, DEPTH, MK>(t, active, px, py, pz, self, h,
                                                                  stk, LANE_BLOCK, lane, visits,
-                                                                 inserts);
+                                                                 inserts, wlo, whi);
         if (active) {
             if (!LISTED && ovf) qm.redo[slot] = 1;
             else h.store(k, res, dists, (size_t)q);

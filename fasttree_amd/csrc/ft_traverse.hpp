@@ -891,7 +891,8 @@ template <typename T, int M, bool GROUPED, int DEPTH, int MK>
 __device__ __forceinline__ bool lane_reg_walk(const DTree<T> &t, bool active, T px, T py, T pz,
                                               int self, TopK<typename CoordTraits<T>::D, M> &h,
                                               int32_t *stk, int lanes, int lane,
-                                              uint64_t &visits, uint64_t &inserts) {
+                                              uint64_t &visits, uint64_t &inserts,
+                                              int wlo = 0, int whi = 0) {
     using D = typename CoordTraits<T>::D;
     const int n = t.n;
     bool overflow = false;
@@ -908,7 +909,9 @@ __device__ __forceinline__ bool lane_reg_walk(const DTree<T> &t, bool active, T 
         if (pruned) continue;
         if (node >= n - 1 || n == 1) {
             const int leaf = (n == 1) ? 0 : node;
-            if (leaf - (n - 1) == self) continue;
+            const int rk = leaf - (n - 1);
+            if (rk == self) continue;
+            if (rk >= wlo && rk < whi) continue;  // already in h (pre-scanned)
             const int32_t ri = ret_index(t, leaf);
             const bool take = h.beats_worst(d2, ri);
             h.insert(take, d2, ri);
