@@ -53,6 +53,10 @@ __device__ __host__ inline uint16_t to_bf16(float f) {
     return (uint16_t)(u >> 16);
 }
 __device__ inline float from_bf16(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+// The same rounding by the hardware conversion (v_cvt_pk_bf16_f32, round to
+// nearest even; equal to to_bf16 for every finite value), for the inference
+// loop's activations.
+__device__ inline short bf16_hw(float f) { return __builtin_bit_cast(short, (__bf16)f); }
 
 __device__ inline f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
@@ -342,19 +346,19 @@ __global__ __launch_bounds__(256) void mlp_infer_kernel(const float *__restrict_
         if (s < n) v = ((const float4 *)(feat + (size_t)s * NF))[lg];
         const float vi[4] = {v.x, v.y, v.z, v.w};
         bf16x4 x;
-        for (int i = 0; i < 4; ++i) x[i] = (short)to_bf16((vi[i] - fm[i]) * fi[i]);
+        for (int i = 0; i < 4; ++i) x[i] = bf16_hw((vi[i] - fm[i]) * fi[i]);
         bf16x4 h1[4], h2[4];
 #pragma unroll
         for (int nt = 0; nt < 4; ++nt) {
             const f32x4 z = mfma16(a1[nt], x, b1[nt]);
-            for (int i = 0; i < 4; ++i) h1[nt][i] = (short)to_bf16(z[i] > 0.f ? z[i] : 0.f);
+            for (int i = 0; i < 4; ++i) h1[nt][i] = bf16_hw(z[i] > 0.f ? z[i] : 0.f);
         }
 #pragma unroll
         for (int nt = 0; nt < 4; ++nt) {
             f32x4 z = b2[nt];
 #pragma unroll
             for (int kc = 0; kc < 4; ++kc) z = mfma16(a2[nt][kc], h1[kc], z);
-            for (int i = 0; i < 4; ++i) h2[nt][i] = (short)to_bf16(z[i] > 0.f ? z[i] : 0.f);
+            for (int i = 0; i < 4; ++i) h2[nt][i] = bf16_hw(z[i] > 0.f ? z[i] : 0.f);
         }
         float part = 0.f;
 #pragma unroll
