@@ -43,6 +43,10 @@ int range_entry(hipStream_t s, const ft_tree *tree, const void *qx, const void *
                 int32_t *counts, int max_res, const int64_t *row_ptr, int32_t *cols,
                 int64_t col_cap);
 int scan_entry(hipStream_t s, const int32_t *counts, int nq, int64_t *row_ptr);
+int range_csr_entry(hipStream_t s, const ft_tree *tree, const void *qx, const void *qy,
+                    const void *qz, const void *rmin, const void *rmax, int nq, int64_t *row_ptr,
+                    int32_t *(*cols_for)(int64_t total, void *ctx), void *ctx, int64_t col_cap,
+                    int64_t *total);
 
 // ---------------------------------------------------------------------------
 // Staging
@@ -598,28 +602,33 @@ int ft_range_query_csr(ft_stream s, const ft_tree *tree, const void *qx, const v
     FT_TRY(stage_in(r_min, qb, hs, sa));
     FT_TRY(stage_in(r_max, qb, hs, sb));
     FT_TRY(stage_out(row_offsets, ((size_t)num_queries + 1) * 8, hs, so));
-    int32_t *counts = nullptr;
-    FT_HIP(hipMallocAsync((void **)&counts, (size_t)num_queries * 4, hs));
-    int rc = range_entry(hs, tree, sx.dev, sy.dev, sz.dev, sa.dev, sb.dev, num_queries, nullptr,
+    int rc = FT_OK;
+    if (cols) {
+        // counts, offsets and columns from one walk; the column buffer is
+        // staged once the total is known
+        struct ColsCtx {
+            int32_t *user;
+            Staged *sc;
+            hipStream_t hs;
+        } cctx{cols, &sc, hs};
+        auto cols_for = [](int64_t total, void *p) -> int32_t * {
+            ColsCtx *c = (ColsCtx *)p;
+            if (stage_out(c->user, (size_t)(total > 0 ? total : 1) * 4, c->hs, *c->sc) != FT_OK)
+                return nullptr;
+            return (int32_t *)c->sc->dev;
+        };
+        int64_t total = 0;
+        rc = range_csr_entry(hs, tree, sx.dev, sy.dev, sz.dev, sa.dev, sb.dev, num_queries,
+                             (int64_t *)so.dev, cols_for, &cctx, col_capacity, &total);
+        if (rc == FT_OK) rc = unstage(sc, hs, true);
+    } else {
+        int32_t *counts = nullptr;
+        FT_HIP(hipMallocAsync((void **)&counts, (size_t)num_queries * 4, hs));
+        rc = range_entry(hs, tree, sx.dev, sy.dev, sz.dev, sa.dev, sb.dev, num_queries, nullptr,
                          counts, 0, nullptr, nullptr, 0);
-    if (rc == FT_OK) rc = scan_entry(hs, counts, num_queries, (int64_t *)so.dev);
-    int64_t total = 0;
-    if (rc == FT_OK) {
-        FT_HIP(hipMemcpyAsync(&total, (int64_t *)so.dev + num_queries, 8, hipMemcpyDeviceToHost, hs));
-        FT_HIP(hipStreamSynchronize(hs));
+        if (rc == FT_OK) rc = scan_entry(hs, counts, num_queries, (int64_t *)so.dev);
+        hipFreeAsync(counts, hs);
     }
-    if (rc == FT_OK && cols) {
-        if (total > col_capacity) {
-            rc = fail(FT_ERANGE, "CSR column capacity is smaller than the number of hits");
-        } else {
-            FT_TRY(stage_out(cols, (size_t)(total > 0 ? total : 1) * 4, hs, sc));
-            rc = range_entry(hs, tree, sx.dev, sy.dev, sz.dev, sa.dev, sb.dev, num_queries,
-                             nullptr, nullptr, 0, (const int64_t *)so.dev, (int32_t *)sc.dev,
-                             col_capacity);
-            if (rc == FT_OK) rc = unstage(sc, hs, true);
-        }
-    }
-    hipFreeAsync(counts, hs);
     int rc2 = unstage(so, hs, true);
     if (rc == FT_OK) rc = rc2;
     for (Staged *st : {&sx, &sy, &sz, &sa, &sb, &sc}) unstage(*st, hs, false);

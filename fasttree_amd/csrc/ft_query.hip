@@ -1049,16 +1049,14 @@ int knn_entry(hipStream_t s, const ft_tree *tree, const void *qx, const void *qy
     return FT_OK;
 }
 
-// Range query launch: queries in Morton order of the points, packet walk with
-// the metric kind fixed at compile time.
+// Range query launch: queries in the order `order` (slot -> query), packet
+// walk with the metric kind fixed at compile time.
 template <typename T>
-static int range_launch(hipStream_t s, const ft_tree &tree, const T *qx, const T *qy, const T *qz,
-                        const T *rmin, const T *rmax, int nq, int32_t *res, int32_t *counts,
-                        int max_res, const int64_t *row_ptr, int32_t *cols, int64_t col_cap) {
-    const DTree<T> dt = make_dtree<T>(tree);
-    const QSrc<T> qs{qx, qy, qz, nullptr};
-    const uint32_t *order = nullptr;
-    FT_TRY(query_order<T>(s, dt, qs, nq, order));
+static int range_walk(hipStream_t s, const DTree<T> &dt, const T *qx, const T *qy, const T *qz,
+                      const T *rmin, const T *rmax, int nq, const uint32_t *order, int32_t *res,
+                      int32_t *counts, int max_res, const int64_t *row_ptr, int32_t *cols,
+                      int64_t col_cap) {
+    if (nq <= 0) return FT_OK;
     const dim3 g(xcd_grid(blocks_for(nq, 64))), b(64);
     constexpr int BUCKET = 32;
     switch (metric_kind<T>(dt.m)) {
@@ -1079,6 +1077,19 @@ static int range_launch(hipStream_t s, const ft_tree &tree, const T *qx, const T
     return FT_OK;
 }
 
+// Queries in Morton order of the points inside the root box.
+template <typename T>
+static int range_launch(hipStream_t s, const ft_tree &tree, const T *qx, const T *qy, const T *qz,
+                        const T *rmin, const T *rmax, int nq, int32_t *res, int32_t *counts,
+                        int max_res, const int64_t *row_ptr, int32_t *cols, int64_t col_cap) {
+    const DTree<T> dt = make_dtree<T>(tree);
+    const QSrc<T> qs{qx, qy, qz, nullptr};
+    const uint32_t *order = nullptr;
+    FT_TRY(query_order<T>(s, dt, qs, nq, order));
+    return range_walk<T>(s, dt, qx, qy, qz, rmin, rmax, nq, order, res, counts, max_res, row_ptr,
+                         cols, col_cap);
+}
+
 int range_entry(hipStream_t s, const ft_tree *tree, const void *qx, const void *qy,
                 const void *qz, const void *rmin, const void *rmax, int nq, int32_t *res,
                 int32_t *counts, int max_res, const int64_t *row_ptr, int32_t *cols,
@@ -1088,6 +1099,140 @@ int range_entry(hipStream_t s, const ft_tree *tree, const void *qx, const void *
                       rc = range_launch<T>(s, *tree, (const T *)qx, (const T *)qy, (const T *)qz,
                                            (const T *)rmin, (const T *)rmax, nq, res, counts,
                                            max_res, row_ptr, cols, col_cap));
+    return rc;
+}
+
+// ---------------------------------------------------------------------------
+// CSR range query with the columns in one walk (no reference counterpart).
+// The walk writes every query's hit count and its first CSR_POOL hits into a
+// fixed-stride pool (the fixed-stride range_query output); after the scan of
+// the counts, the pool rows are copied to their CSR rows, and only queries
+// with more than CSR_POOL hits are walked again, writing their whole rows.
+// A walk meets a query's hits in the same (ascending rank) order every time,
+// so the rows equal those of a counting walk followed by a filling walk.
+// ---------------------------------------------------------------------------
+constexpr int CSR_POOL = 128;
+
+// One wave per row (grid-stride over rows): rows of at most CSR_POOL hits.
+__global__ __launch_bounds__(256) void csr_copy_kernel(const int32_t *__restrict__ pool,
+                                                       const int32_t *__restrict__ counts,
+                                                       const int64_t *__restrict__ row_ptr, int nq,
+                                                       int32_t *__restrict__ cols,
+                                                       int64_t col_cap) {
+    const int lane = threadIdx.x & 63;
+    const int w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int nw = gridDim.x * (blockDim.x >> 6);
+    for (int q = w; q < nq; q += nw) {
+        const int c = counts[q];
+        if (c > CSR_POOL) continue;
+        const int64_t base = row_ptr[q];
+        for (int j = lane; j < c; j += 64)
+            if (base + j < col_cap) cols[base + j] = pool[(size_t)q * CSR_POOL + j];
+    }
+}
+
+// flags[slot] = the slot's query has more hits than the pool holds.
+__global__ void csr_overflow_flags_kernel(const uint32_t *__restrict__ order,
+                                          const int32_t *__restrict__ counts, int nq,
+                                          uint8_t *__restrict__ flags) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nq; i += gridDim.x * blockDim.x) {
+        const int q = order ? (int)order[i] : i;
+        flags[i] = counts[q] > CSR_POOL ? 1 : 0;
+    }
+}
+
+// order2[i] = the query of the i-th listed slot.
+__global__ void csr_listed_order_kernel(const uint32_t *__restrict__ order,
+                                        const int32_t *__restrict__ list,
+                                        const uint32_t *__restrict__ count,
+                                        uint32_t *__restrict__ order2) {
+    const uint32_t m = *count;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x)
+        order2[i] = order ? order[list[i]] : (uint32_t)list[i];
+}
+
+int scan_entry(hipStream_t s, const int32_t *counts, int nq, int64_t *row_ptr);
+
+template <typename T>
+static int range_csr_t(hipStream_t s, const ft_tree &tree, const T *qx, const T *qy, const T *qz,
+                       const T *rmin, const T *rmax, int nq, int64_t *row_ptr,
+                       int32_t *(*cols_for)(int64_t total, void *ctx), void *ctx,
+                       int64_t col_cap, int64_t *total_out) {
+    const DTree<T> dt = make_dtree<T>(tree);
+    const QSrc<T> qs{qx, qy, qz, nullptr};
+    const uint32_t *order = nullptr;
+    FT_TRY(query_order<T>(s, dt, qs, nq, order));
+    int32_t *counts = nullptr, *pool = nullptr;
+    uint32_t *order2 = nullptr;
+    FT_HIP(hipMallocAsync((void **)&counts, (size_t)nq * 4, s));
+    if (hipMallocAsync((void **)&pool, (size_t)nq * CSR_POOL * 4, s) != hipSuccess) {
+        (void)hipFreeAsync(counts, s);
+        return fail(FT_ENOMEM, "CSR range query: pool allocation failed");
+    }
+    auto release = [&]() {
+        (void)hipFreeAsync(counts, s);
+        (void)hipFreeAsync(pool, s);
+        if (order2) (void)hipFreeAsync(order2, s);
+    };
+    int rc = range_walk<T>(s, dt, qx, qy, qz, rmin, rmax, nq, order, pool, counts, CSR_POOL,
+                           nullptr, nullptr, 0);
+    if (rc == FT_OK) rc = scan_entry(s, counts, nq, row_ptr);
+    int64_t total = 0;
+    if (rc == FT_OK && hipMemcpyAsync(&total, row_ptr + nq, 8, hipMemcpyDeviceToHost, s) != hipSuccess)
+        rc = fail(FT_EHIP, "CSR range query: reading the total failed");
+    if (rc == FT_OK && hipStreamSynchronize(s) != hipSuccess)
+        rc = fail(FT_EHIP, "CSR range query: synchronize failed");
+    if (rc == FT_OK) *total_out = total;
+    if (rc == FT_OK && total > col_cap)
+        rc = fail(FT_ERANGE, "CSR column capacity is smaller than the number of hits");
+    int32_t *cols = nullptr;
+    if (rc == FT_OK) {
+        cols = cols_for(total, ctx);
+        if (!cols) rc = fail(FT_ENOMEM, "CSR range query: column buffer");
+    }
+    if (rc == FT_OK) {
+        hipLaunchKernelGGL(csr_copy_kernel, dim3(std::min<unsigned>(blocks_for(nq, 4), 16384u)),
+                           dim3(256), 0, s, (const int32_t *)pool, (const int32_t *)counts,
+                           (const int64_t *)row_ptr, nq, cols, col_cap);
+        rc = rerun_ensure((size_t)nq);
+    }
+    uint32_t n_over = 0;
+    if (rc == FT_OK) {
+        hipLaunchKernelGGL(csr_overflow_flags_kernel, dim3(std::min<unsigned>(blocks_for(nq, 256), 4096u)),
+                           dim3(256), 0, s, order, (const int32_t *)counts, nq, g_rerun.flags);
+        rc = compact_flags(s, nq);
+    }
+    if (rc == FT_OK &&
+        (hipMemcpyAsync(&n_over, g_rerun.count, 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
+         hipStreamSynchronize(s) != hipSuccess))
+        rc = fail(FT_EHIP, "CSR range query: reading the overflow count failed");
+    if (rc == FT_OK && n_over > 0) {
+        if (hipMallocAsync((void **)&order2, (size_t)n_over * 4, s) != hipSuccess) {
+            rc = fail(FT_ENOMEM, "CSR range query: overflow list allocation failed");
+        } else {
+            hipLaunchKernelGGL(csr_listed_order_kernel, dim3(std::min<unsigned>(blocks_for(n_over, 256), 4096u)),
+                               dim3(256), 0, s, order, (const int32_t *)g_rerun.list,
+                               (const uint32_t *)g_rerun.count, order2);
+            if (hipGetLastError() != hipSuccess) rc = fail(FT_EHIP, "CSR range query: launch failed");
+        }
+        if (rc == FT_OK) {
+            rc = range_walk<T>(s, dt, qx, qy, qz, rmin, rmax, (int)n_over, order2, nullptr,
+                               nullptr, 0, row_ptr, cols, col_cap);
+        }
+    }
+    release();
+    return rc;
+}
+
+int range_csr_entry(hipStream_t s, const ft_tree *tree, const void *qx, const void *qy,
+                    const void *qz, const void *rmin, const void *rmax, int nq, int64_t *row_ptr,
+                    int32_t *(*cols_for)(int64_t total, void *ctx), void *ctx, int64_t col_cap,
+                    int64_t *total) {
+    int rc = FT_OK;
+    FT_DISPATCH_COORD(tree->coord_type,
+                      rc = range_csr_t<T>(s, *tree, (const T *)qx, (const T *)qy, (const T *)qz,
+                                          (const T *)rmin, (const T *)rmax, nq, row_ptr, cols_for,
+                                          ctx, col_cap, total));
     return rc;
 }
 

@@ -221,6 +221,8 @@ int ft_range_query(ft_stream s, const ft_tree *tree, const void *qx, const void 
 /* Range query with CSR output (no reference counterpart): row_offsets int64_t[Q+1],   */
 /* cols int32_t[capacity].  Pass cols = NULL to get only counts/offsets (total in      */
 /* row_offsets[Q]); hits beyond col_capacity return FT_ERANGE after filling offsets.   */
+/* With cols, one walk fills both (it uses 512 bytes of scratch per query; rows of   */
+/* more than 128 hits are walked a second time).                                      */
 int ft_range_query_csr(ft_stream s, const ft_tree *tree, const void *qx, const void *qy,
                        const void *qz, const void *r_min, const void *r_max, int num_queries,
                        int64_t *row_offsets, int32_t *cols, int64_t col_capacity);

@@ -44,6 +44,30 @@ def test_range_matches_reference(ft, orc, ct, periodic):
     rp, cols = ft.range_query_csr(tree, *q, rmin, rmax)
     erp, ecols = O.range_csr(t, *q, rmin, rmax)
     assert np.array_equal(rp, erp) and np.array_equal(cols, ecols)
+    # the CSR fill copies rows of at most 128 hits from its first walk and walks
+    # the longer rows again: both kinds occur here
+    rows = np.diff(rp)
+    assert (rows > 128).any() and (rows <= 128).any()
+
+
+def test_csr_column_capacity_too_small(ft, orc):
+    """A column buffer smaller than the number of hits is an error (FT_ERANGE)
+    and leaves the row offsets filled."""
+    import ctypes as C
+    x, y, z = synth.nfw_clustered(5000, 8, box=100.0, n_halos=20)
+    tree, t = make(ft, O.CT_F64, O.SFC_PH, 21, 0, (x, y, z))
+    q = tuple(np.ascontiguousarray(a[::10]) for a in (x, y, z))
+    nq = len(q[0])
+    rmin, rmax = np.zeros(nq), np.full(nq, 2.0)
+    erp, _ = O.range_csr(t, *q, rmin, rmax)
+    assert erp[-1] > 10
+    rp = np.zeros(nq + 1, np.int64)
+    cols = np.zeros(10, np.int32)
+    rc = ft.lib().ft_range_query_csr(None, C.byref(tree.c), *(a.ctypes.data for a in q),
+                                     rmin.ctypes.data, rmax.ctypes.data, nq, rp.ctypes.data,
+                                     cols.ctypes.data, 10)
+    assert rc == ft.FT_ERANGE
+    assert np.array_equal(rp, erp)
 
 
 def test_range_grid_kats(ft, orc):
