@@ -143,6 +143,38 @@ __host__ __device__ __forceinline__ float wrap_filter_bound_pre(uint64_t w, int 
 __host__ __device__ __forceinline__ float wrap_filter_floor_pre(uint64_t lo, int sh) {
     return (float)lo * (1.0f - 0x1p-20f) * (float)(1ull << (2 * sh));
 }
+// Float32 pre-filter for double coordinates.  With u = 2^-24, coordinates of
+// magnitude at most M and period L, the float gap of one axis (both points
+// rounded to float, one subtraction, for a periodic metric min(g, fl(L) - g))
+// is within (1 + u) g + E of the double gap g, E = 1.01 u (L + 4M) (periodic)
+// or 2.02 u M (open); the float sum of the three squares (a product, two
+// fmas) is within a factor 1 + 3.01u.  So a leaf with d2 <= w has a float
+// squared distance at most fpf_upper(w, E), and one with d2 >= lo at least
+// fpf_lower(lo, E): the filter never rejects a leaf the exact test accepts.
+__host__ __device__ __forceinline__ float fpf_gap(float q, float x, float L, bool wrap) {
+    const float g = fabsf(q - x);
+    return wrap ? fminf(g, L - g) : g;
+}
+__host__ __device__ __forceinline__ float fpf_sq(float gx, float gy, float gz) {
+    float d = gx * gx;
+    d = __builtin_fmaf(gy, gy, d);
+    d = __builtin_fmaf(gz, gz, d);
+    return d;
+}
+__host__ __device__ __forceinline__ double fpf_err(double M, double L, bool wrap) {
+    return wrap ? 1.01 * 0x1p-24 * (L + 4.0 * M) : 2.02 * 0x1p-24 * M;
+}
+__host__ __device__ __forceinline__ float fpf_upper(double w, double E) {
+    const double b = (1.0 + 0x1p-23) * sqrt(w) + 1.75 * E;
+    const double v = b * b * (1.0 + 0x1p-22);
+    return v >= 3.0e38 ? __builtin_huge_valf() : (float)v * (1.0f + 0x1p-22f);
+}
+__host__ __device__ __forceinline__ float fpf_lower(double lo, double E) {
+    const double b = (1.0 - 0x1p-23) * sqrt(lo) - 1.75 * E;
+    if (!(b > 0.0)) return 0.0f;
+    return (float)(b * b * (1.0 - 0x1p-22)) * (1.0f - 0x1p-22f);
+}
+
 // Float32 squared distance of a leaf, pre-shifted coordinates (scaled by
 // 2^(2 sh)); compared with wrap_filter_bound_pre it never rejects a leaf whose
 // exact squared distance is within the bound.
@@ -591,6 +623,24 @@ __device__ __forceinline__ void packet_walk(const DTree<T> &t, int bucket, bool 
             qmz = pz & mask;
         }
     }
+    // Double coordinates: the float32 leaf pre-filter (fpf_*), with E from
+    // the largest coordinate magnitude of the root box and the query.  Not in
+    // multi-pass walks (FLOOR): there its registers would push the kernel to
+    // one wave per SIMD.
+    constexpr bool FPF = std::is_same<T, double>::value && MK != MK_ANY && !FLOOR;
+    [[maybe_unused]] float fqx = 0.f, fqy = 0.f, fqz = 0.f, fL = 0.f;
+    [[maybe_unused]] double fE = 0.0;
+    if constexpr (FPF) {
+        fqx = (float)px;
+        fqy = (float)py;
+        fqz = (float)pz;
+        const double mr = fmax(fmax(fmax(fabs(t.mnx[0]), fabs(t.mxx[0])),
+                                    fmax(fabs(t.mny[0]), fabs(t.mxy[0]))),
+                               fmax(fabs(t.mnz[0]), fabs(t.mxz[0])));
+        const double mq = fmax(fabs(px), fmax(fabs(py), fabs(pz)));
+        fE = fpf_err(fmax(mr, mq), (double)t.m.box, MK == MK_WRAP);
+        fL = (float)t.m.box;
+    }
     // Node test.  Integer periodic metric: a float32 squared distance that is
     // never above the exact one by more than wrap_filter_bound's margin (gaps
     // from pre-shifted coordinates, the inside test on the full values as in
@@ -650,6 +700,22 @@ __device__ __forceinline__ void packet_walk(const DTree<T> &t, int bucket, bool 
             };
             for (int j = 0; j < c0; ++j) clo = near(j) ? (clo | (1u << j)) : clo;
             for (int j = 32; j < cnt; ++j) chi = near(j) ? (chi | (1u << (j - 32))) : chi;
+        } else if constexpr (FPF) {
+            // float32 pre-filter of double coordinates (fpf_*): never rejects
+            // a leaf within w; the insertion below tests every passing leaf
+            // exactly.
+            const float fbx = (float)bx, fby = (float)by, fbz = (float)bz;
+            const float wf = fpf_upper((double)w, fE);
+            [[maybe_unused]] const float lf = FLOOR ? fpf_lower((double)lo_d, fE) : 0.0f;
+            constexpr bool WR = MK == MK_WRAP;
+            auto near = [&](int j) -> bool {
+                const float d = fpf_sq(fpf_gap(fqx, rl(fbx, j), fL, WR), fpf_gap(fqy, rl(fby, j), fL, WR),
+                                       fpf_gap(fqz, rl(fbz, j), fL, WR));
+                if constexpr (FLOOR) return d <= wf && d >= lf;
+                else return d <= wf;
+            };
+            for (int j = 0; j < c0; ++j) clo |= near(j) ? (1u << j) : 0u;
+            for (int j = 32; j < cnt; ++j) chi |= near(j) ? (1u << (j - 32)) : 0u;
         } else {
             auto near = [&](int j) -> bool {
                 const D d = leaf_d2_k<T, MK>(t.m, px, py, pz, rl(bx, j), rl(by, j), rl(bz, j));

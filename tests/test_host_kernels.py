@@ -63,3 +63,19 @@ def test_float_prefilter_never_rejects_a_leaf_within_the_bound(tmp_path):
     assert out.returncode == 0, out.stdout
     assert "false rejections: 0," in out.stdout
     assert "gap mismatches: 0," in out.stdout and "box false rejections: 0\n" in out.stdout
+
+
+@pytest.mark.skipif(not os.path.exists(HIPCC), reason="hipcc not installed")
+def test_double_coordinate_float_filter_never_rejects(tmp_path):
+    """The packet walk's float32 pre-filter for double coordinates (fpf_*):
+    for periodic and open metrics, box sizes 1 to 12345, gaps down to 1e-7 of
+    the box and pairs at the wrap, every leaf with exact d2 <= w passes and no
+    leaf with d2 >= lo falls below the floor."""
+    src = os.path.join(ROOT, "tests", "host", "fp_filter_check.cpp")
+    exe = str(tmp_path / "fp_filter_check")
+    subprocess.check_call([HIPCC, "-O2", "-std=c++17", "-ffp-contract=off", "--cuda-host-only",
+                           "-x", "hip", src, "-o", exe], stderr=subprocess.DEVNULL)
+    out = subprocess.run([exe], capture_output=True, text=True, timeout=300)
+    print(out.stdout)
+    assert out.returncode == 0, out.stdout
+    assert "false rejections: 0, false floor rejections: 0," in out.stdout
