@@ -1165,17 +1165,18 @@ static int range_csr_t(hipStream_t s, const ft_tree &tree, const T *qx, const T 
     int32_t *counts = nullptr, *pool = nullptr;
     uint32_t *order2 = nullptr;
     FT_HIP(hipMallocAsync((void **)&counts, (size_t)nq * 4, s));
+    // Without room for the pool: a counting walk now and a filling walk later.
     if (hipMallocAsync((void **)&pool, (size_t)nq * CSR_POOL * 4, s) != hipSuccess) {
-        (void)hipFreeAsync(counts, s);
-        return fail(FT_ENOMEM, "CSR range query: pool allocation failed");
+        (void)hipGetLastError();
+        pool = nullptr;
     }
     auto release = [&]() {
         (void)hipFreeAsync(counts, s);
-        (void)hipFreeAsync(pool, s);
+        if (pool) (void)hipFreeAsync(pool, s);
         if (order2) (void)hipFreeAsync(order2, s);
     };
-    int rc = range_walk<T>(s, dt, qx, qy, qz, rmin, rmax, nq, order, pool, counts, CSR_POOL,
-                           nullptr, nullptr, 0);
+    int rc = range_walk<T>(s, dt, qx, qy, qz, rmin, rmax, nq, order, pool, counts,
+                           pool ? CSR_POOL : 0, nullptr, nullptr, 0);
     if (rc == FT_OK) rc = scan_entry(s, counts, nq, row_ptr);
     int64_t total = 0;
     if (rc == FT_OK && hipMemcpyAsync(&total, row_ptr + nq, 8, hipMemcpyDeviceToHost, s) != hipSuccess)
@@ -1189,6 +1190,12 @@ static int range_csr_t(hipStream_t s, const ft_tree &tree, const T *qx, const T 
     if (rc == FT_OK) {
         cols = cols_for(total, ctx);
         if (!cols) rc = fail(FT_ENOMEM, "CSR range query: column buffer");
+    }
+    if (rc == FT_OK && !pool) {
+        rc = range_walk<T>(s, dt, qx, qy, qz, rmin, rmax, nq, order, nullptr, nullptr, 0, row_ptr,
+                           cols, col_cap);
+        release();
+        return rc;
     }
     if (rc == FT_OK) {
         hipLaunchKernelGGL(csr_copy_kernel, dim3(std::min<unsigned>(blocks_for(nq, 4), 16384u)),
