@@ -129,9 +129,10 @@ typedef struct ft_query_opts {
                          /* [3] queries re-run exactly [4] heap insertions              */
                          /* packet walks, per wave: [5] insertion rounds [6] rounds     */
                          /* that ran the insertion chain                                */
-    int32_t window;      /* FT_ALGO_PACKET self queries over all leaves: leaves scanned  */
-                         /* on each side of a packet's own run before the walk (0 =     */
-                         /* auto, negative = none)                                       */
+    int32_t window;      /* FT_ALGO_PACKET self queries: leaves scanned on each side of  */
+                         /* a packet's own run (subsets: the run around its middle      */
+                         /* lane's leaf, widened by up to 4x for sparse subsets) before */
+                         /* the walk (0 = auto: 128, negative = none)                   */
     int32_t reserved2;
 } ft_query_opts;
 
