@@ -145,7 +145,10 @@ __device__ inline int grab_item(uint32_t *ctr, const uint32_t *bnd) {
 
 // Packet tiers.  HEAP: LDS heap of capacity M, else the register top-k;
 // MULTI: the register top-k in passes of M (a separate instantiation, so the
-// single-pass tiers keep their register budget).
+// single-pass tiers keep their register budget).  Each packet first scans
+// BATCH_WINDOW leaves on each side of a run near its queries.
+constexpr int BATCH_WINDOW = 128;
+
 template <typename T, int M, bool HEAP, int MK, bool MULTI = false>
 __global__ __launch_bounds__(64) void batch_packet_kernel(BatchView<T> b,
                                                           const uint2 *__restrict__ items,
@@ -183,6 +186,12 @@ __global__ __launch_bounds__(64) void batch_packet_kernel(BatchView<T> b,
             py = b.qy[qi];
             pz = b.qz[qi];
         }
+        // window pre-scan around the leaf a greedy descent reaches for the
+        // packet's middle query (results do not depend on it)
+        const int src = (j * 64 + 32 < q.num) ? 32 : 0;
+        const int r0 = run_start(descend_rank<T, MK>(t, rl(px, src), rl(py, src), rl(pz, src)), t.n);
+        const int wlo = r0 - BATCH_WINDOW > 0 ? r0 - BATCH_WINDOW : 0;
+        const int whi = r0 + 64 + BATCH_WINDOW < t.n ? r0 + 64 + BATCH_WINDOW : t.n;
         bool redo = false;
         static_assert(!MULTI || !HEAP, "passes need the register tier");
         if constexpr (MULTI) {
@@ -194,8 +203,8 @@ __global__ __launch_bounds__(64) void batch_packet_kernel(BatchView<T> b,
                 if constexpr (PACKED) h.init(kk, ~0ull, 32 - __clz(t.n));
                 else h.init(kk, dist_max<D>());
                 // one walk instance for every pass (the first with floor 0)
-                packet_walk<T, H, MK, true>(t, q.bucket, active, px, py, pz, -1, h, stk, c, 0, 0,
-                                            0, lo);
+                packet_walk<T, H, MK, true>(t, q.bucket, active, px, py, pz, -1, h, stk, c, wlo,
+                                            whi, r0, lo);
                 if (active) {
                     if constexpr (PACKED) redo = redo || h.needs_exact(kk);
                     if (!redo)
@@ -209,7 +218,7 @@ __global__ __launch_bounds__(64) void batch_packet_kernel(BatchView<T> b,
             if constexpr (HEAP) h.key = heap_mem;
             if constexpr (PACKED) h.init(k, ~0ull, 32 - __clz(t.n));
             else h.init(k, dist_max<D>());
-            packet_walk<T, H, MK>(t, q.bucket, active, px, py, pz, -1, h, stk, c);
+            packet_walk<T, H, MK>(t, q.bucket, active, px, py, pz, -1, h, stk, c, wlo, whi, r0);
             if (active) {
                 if constexpr (PACKED) redo = h.needs_exact(k);
                 if (!redo)

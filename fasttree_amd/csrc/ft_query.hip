@@ -336,16 +336,16 @@ __global__ __launch_bounds__(64) void packet_knn_kernel(
     if (!a.excl) self = -1;
     // Window pre-scan.  Self queries over all leaves: the packet is leaf ranks
     // [r0, r0 + 64), so its neighbours along the curve are the leaves on
-    // either side.  Subset queries (Morton-ordered, so a packet's leaves are
-    // neighbours): the run of 64 leaves around the middle lane's rank.
+    // either side.  Subset and external queries (Morton-ordered, so a
+    // packet's queries are neighbours): the run of 64 leaves around the
+    // middle lane's leaf rank, or around the leaf a greedy descent reaches
+    // for the middle lane's point.
     int wlo = 0, whi = 0;
     int r0 = blk * 64;
     if (a.window > 0) {
-        if (qs.leaf_ids) {
-            const int rc = uni(__shfl(rank, (blk * 64 + 32 < a.nq) ? 32 : 0, 64));
-            r0 = rc - 32 < 0 ? 0 : rc - 32;
-            if (r0 > t.n - 64) r0 = t.n - 64 > 0 ? t.n - 64 : 0;
-        }
+        const int src = (blk * 64 + 32 < a.nq) ? 32 : 0;
+        if (qs.qx) r0 = run_start(descend_rank<T, MK>(t, rl(px, src), rl(py, src), rl(pz, src)), t.n);
+        else if (qs.leaf_ids) r0 = run_start(uni(__shfl(rank, src, 64)), t.n);
         wlo = r0 - a.window > 0 ? r0 - a.window : 0;
         whi = r0 + 64 + a.window < t.n ? r0 + 64 + a.window : t.n;
     }
@@ -909,7 +909,7 @@ static int run_canonical(hipStream_t s, const DTree<T> &dt, const QSrc<T> &qs, i
         // clustered points at k = 64: windows of 128 / 256 / 512 / 1024 took
         // 114.7 / 112.7 / 111.9 / 113.8 ms).
         const int spread = qs.leaf_ids ? (int)std::min<int64_t>(4, std::max<int64_t>(1, dt.n / std::max(nq, 1))) : 1;
-        a.window = !qs.qx ? window * spread : 0;
+        a.window = window * spread;
         a.stats = stats;
         // k above 32 with integer coordinates: multi-pass register tier by
         // default, LDS heap tiers on request.

@@ -940,6 +940,31 @@ template <int M> struct TopKSel<uint64_t, M, true> {
     using type = typename std::conditional<(M > 32), TopKHeapLds<M>, TopKPacked<M>>::type;
 };
 
+// Leaf rank reached from the root by a greedy descent for one point: at every
+// node the child whose box is nearer (ties to the left).  The point and every
+// decision are uniform across the wave, so the loads are scalar.  External
+// query packets centre their window pre-scan on it: it is a leaf near the
+// packet's queries along the curve.
+template <typename T, int MK>
+__device__ inline int descend_rank(const DTree<T> &t, T cx, T cy, T cz) {
+    const int off = t.n - 1;
+    int v = 0;
+    while (v < off) {
+        const int l = uni(t.left[v]), r = uni(t.right[v]);
+        const auto dl = node_d2_k<T, MK>(t, cx, cy, cz, l);
+        const auto dr = node_d2_k<T, MK>(t, cx, cy, cz, r);
+        v = uni(dl <= dr ? l : r);
+    }
+    return v - off;
+}
+
+// First leaf rank of the 64-leaf run centred on rank rc, inside [0, n).
+__device__ inline int run_start(int rc, int n) {
+    int r0 = rc - 32 < 0 ? 0 : rc - 32;
+    if (r0 > n - 64) r0 = n - 64 > 0 ? n - 64 : 0;
+    return r0;
+}
+
 // ---------------------------------------------------------------------------
 // Per-lane DFS walks (one query per lane): pop, node d2, prune, leaf insert or
 // both children pushed, the farther first (src/hlbvh.hpp:1545-1628).  The
