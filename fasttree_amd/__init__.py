@@ -85,6 +85,7 @@ EXPORTS = [
     "ft_memcpy_d2h", "ft_memcpy_d2d", "ft_memset", "ft_copy_kernel", "ft_stream_create", "ft_stream_destroy",
     "ft_stream_synchronize", "ft_event_create", "ft_event_destroy", "ft_event_record",
     "ft_event_elapsed_ms", "ft_tree_alloc", "ft_tree_ensure", "ft_tree_free",
+    "ft_tree_serialized_size", "ft_tree_serialize", "ft_tree_deserialize",
     "ft_scratch_create", "ft_scratch_destroy", "ft_scratch_set_profiling", "ft_scratch_stage_ms",
     "ft_scratch_set_radix_bits",
     "ft_compute_bbox", "ft_sfc_encode", "ft_build_tree", "ft_build_bvh", "ft_self_knn_query",
@@ -125,6 +126,9 @@ def lib():
         L.ft_tree_alloc.argtypes = [C.POINTER(_Tree), sz, C.c_int32, C.POINTER(Config), vp]
         L.ft_tree_ensure.argtypes = [C.POINTER(_Tree), sz, C.c_int32, C.POINTER(Config), vp]
         L.ft_tree_free.argtypes = [C.POINTER(_Tree), vp]
+        L.ft_tree_serialized_size.argtypes = [C.POINTER(_Tree), C.POINTER(C.c_size_t)]
+        L.ft_tree_serialize.argtypes = [vp, C.POINTER(_Tree), vp, sz]
+        L.ft_tree_deserialize.argtypes = [vp, C.POINTER(_Tree), vp, sz]
         L.ft_scratch_create.argtypes = [C.POINTER(C.c_void_p)]
         L.ft_scratch_destroy.argtypes = [vp]
         L.ft_scratch_set_profiling.argtypes = [vp, i32]
@@ -344,6 +348,28 @@ class TreeSoA:
     def free(self, stream=None):
         if self.c.block:
             _check(lib().ft_tree_free(C.byref(self.c), _sh(stream)))
+
+    def serialize(self, stream=None) -> bytes:
+        """The tree as bytes: header, the reference's MPI transfer sequence
+        (dims, boxes, children, parents), then ids, ghost flags, original
+        indices, sorted keys and node ranges."""
+        nb = C.c_size_t()
+        _check(lib().ft_tree_serialized_size(C.byref(self.c), C.byref(nb)))
+        buf = np.empty(nb.value, np.uint8)
+        _check(lib().ft_tree_serialize(_sh(stream), C.byref(self.c), buf.ctypes.data,
+                                       buf.nbytes))
+        return buf.tobytes()
+
+    @classmethod
+    def deserialize(cls, data, stream=None) -> "TreeSoA":
+        """A tree on the device from `serialize` output."""
+        buf = np.frombuffer(data, np.uint8)
+        t = cls.__new__(cls)
+        t.c = _Tree()
+        _check(lib().ft_tree_deserialize(_sh(stream), C.byref(t.c), buf.ctypes.data, buf.nbytes))
+        t.config = Config()
+        C.memmove(C.byref(t.config), C.byref(t.c.config), C.sizeof(Config))
+        return t
 
     def __del__(self):
         try:

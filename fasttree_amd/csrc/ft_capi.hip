@@ -308,6 +308,102 @@ int ft_tree_free(ft_tree *tree, ft_stream s) {
     return FT_OK;
 }
 
+// ---- serialised trees (no reference counterpart) --------------------------------
+// Layout: a 64-byte header (magic "FTTREE01", coord_type, reserved, the
+// ft_config), then the reference's transfer sequence (test/test_mpi.cpp:76-100):
+// u64 [num_leaves, num_internal], min_x, max_x, min_y, max_y, min_z, max_z
+// (2n-1 coordinates each), left_child, right_child (n-1 int32), parent (2n-1
+// int32); then id (u32), is_ghost (i8), orig_idx (int32) over 2n-1 nodes,
+// sorted_keys (n u64), range_first, range_last (n-1 int32).  Native byte order.
+namespace {
+constexpr char kTreeMagic[8] = {'F', 'T', 'T', 'R', 'E', 'E', '0', '1'};
+struct TreeHeader {
+    char magic[8];
+    int32_t coord_type;
+    int32_t reserved;
+    ft_config config;
+    uint8_t pad[64 - 16 - sizeof(ft_config)];
+};
+static_assert(sizeof(TreeHeader) == 64, "tree header is 64 bytes");
+
+struct TreePart {
+    const void *ptr;
+    size_t bytes;
+};
+// The arrays of a tree of n leaves in serialised order (pointers into `t`).
+static std::vector<TreePart> tree_parts(const ft_tree &t, size_t n) {
+    const size_t cs = coord_size(t.coord_type);
+    const size_t m = n ? 2 * n - 1 : 0, in = n ? n - 1 : 0;
+    return {{t.min_x, m * cs},      {t.max_x, m * cs},          {t.min_y, m * cs},
+            {t.max_y, m * cs},      {t.min_z, m * cs},          {t.max_z, m * cs},
+            {t.left_child, in * 4}, {t.right_child, in * 4},    {t.parent, m * 4},
+            {t.id, m * 4},          {t.is_ghost, m},            {t.orig_idx, m * 4},
+            {t.sorted_keys, n * 8}, {t.range_first, in * 4},    {t.range_last, in * 4}};
+}
+static size_t tree_bytes(int32_t coord_type, size_t n) {
+    ft_tree t{};
+    t.coord_type = coord_type;
+    size_t b = sizeof(TreeHeader) + 16;
+    for (const TreePart &p : tree_parts(t, n)) b += p.bytes;
+    return b;
+}
+}  // namespace
+
+int ft_tree_serialized_size(const ft_tree *tree, size_t *bytes) {
+    if (!tree || !bytes) return fail(FT_EINVAL, "null argument");
+    *bytes = tree_bytes(tree->coord_type, tree->num_leaves);
+    return FT_OK;
+}
+
+int ft_tree_serialize(ft_stream s, const ft_tree *tree, void *buf, size_t bytes) {
+    if (!tree || !buf) return fail(FT_EINVAL, "null argument");
+    const size_t n = tree->num_leaves;
+    if (bytes < tree_bytes(tree->coord_type, n))
+        return fail(FT_ERANGE, "serialisation buffer is smaller than ft_tree_serialized_size");
+    const hipStream_t hs = as_stream(s);
+    TreeHeader h{};
+    std::memcpy(h.magic, kTreeMagic, 8);
+    h.coord_type = tree->coord_type;
+    h.config = tree->config;
+    const uint64_t dims[2] = {(uint64_t)n, (uint64_t)tree->num_internal};
+    char *o = (char *)buf;
+    FT_HIP(hipMemcpyAsync(o, &h, sizeof h, hipMemcpyDefault, hs));
+    FT_HIP(hipMemcpyAsync(o + sizeof h, dims, 16, hipMemcpyDefault, hs));
+    size_t at = sizeof h + 16;
+    for (const TreePart &p : tree_parts(*tree, n)) {
+        if (p.bytes) FT_HIP(hipMemcpyAsync(o + at, p.ptr, p.bytes, hipMemcpyDefault, hs));
+        at += p.bytes;
+    }
+    FT_HIP(hipStreamSynchronize(hs));  // the header and dims live on this stack frame
+    return FT_OK;
+}
+
+int ft_tree_deserialize(ft_stream s, ft_tree *tree, const void *buf, size_t bytes) {
+    if (!tree || !buf) return fail(FT_EINVAL, "null argument");
+    const hipStream_t hs = as_stream(s);
+    if (bytes < sizeof(TreeHeader) + 16) return fail(FT_ERANGE, "serialised tree is truncated");
+    TreeHeader h{};
+    uint64_t dims[2] = {0, 0};
+    const char *in = (const char *)buf;
+    FT_HIP(hipMemcpyAsync(&h, in, sizeof h, hipMemcpyDefault, hs));
+    FT_HIP(hipMemcpyAsync(dims, in + sizeof h, 16, hipMemcpyDefault, hs));
+    FT_HIP(hipStreamSynchronize(hs));
+    if (std::memcmp(h.magic, kTreeMagic, 8) != 0) return fail(FT_EINVAL, "not a serialised tree");
+    const size_t n = (size_t)dims[0];
+    if (dims[1] != (n ? n - 1 : 0)) return fail(FT_EINVAL, "serialised tree: inconsistent sizes");
+    if (h.coord_type < FT_COORD_F32 || h.coord_type > FT_COORD_U64)
+        return fail(FT_EINVAL, "serialised tree: unknown coordinate type");
+    if (bytes < tree_bytes(h.coord_type, n)) return fail(FT_ERANGE, "serialised tree is truncated");
+    FT_TRY(ft_tree_ensure(tree, n, h.coord_type, &h.config, s));
+    size_t at = sizeof h + 16;
+    for (const TreePart &p : tree_parts(*tree, n)) {
+        if (p.bytes) FT_HIP(hipMemcpyAsync((void *)p.ptr, in + at, p.bytes, hipMemcpyDefault, hs));
+        at += p.bytes;
+    }
+    FT_HIP(hipStreamSynchronize(hs));
+    return FT_OK;
+}
+
 // ---- scratch -------------------------------------------------------------------
 int ft_scratch_create(ft_scratch **scratch) {
     if (!scratch) return fail(FT_EINVAL, "null scratch pointer");

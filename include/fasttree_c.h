@@ -167,6 +167,16 @@ int ft_tree_alloc(ft_tree *tree, size_t n, int32_t coord_type, const ft_config *
 int ft_tree_ensure(ft_tree *tree, size_t n, int32_t coord_type, const ft_config *cfg, ft_stream s);
 /* TreeSoA::free(q) (src/hlbvh.hpp:657-672). */
 int ft_tree_free(ft_tree *tree, ft_stream s);
+/* Serialised trees, for checkpoints and transfers (no reference counterpart; the     */
+/* middle section is the reference's MPI transfer sequence, test/test_mpi.cpp:76-100): */
+/* a 64-byte header (magic "FTTREE01", coord_type, config), u64 [num_leaves,            */
+/* num_internal], min_x..max_z, left_child, right_child, parent, then id, is_ghost,     */
+/* orig_idx, sorted_keys, range_first, range_last.  Buffers may be host or device       */
+/* memory; both calls return when the copies are done.  Deserialising (re)allocates     */
+/* the tree as ft_tree_ensure does.                                                     */
+int ft_tree_serialized_size(const ft_tree *tree, size_t *bytes);
+int ft_tree_serialize(ft_stream s, const ft_tree *tree, void *buf, size_t bytes);
+int ft_tree_deserialize(ft_stream s, ft_tree *tree, const void *buf, size_t bytes);
 /* BvhScratch (src/hlbvh.hpp:720-745). */
 int ft_scratch_create(ft_scratch **scratch);
 int ft_scratch_destroy(ft_scratch *scratch);

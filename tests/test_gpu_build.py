@@ -229,3 +229,45 @@ def test_full_size_build_properties(ft, orc, n):
     for a in ("sorted_keys", "orig_idx", "left_child", "right_child", "parent", "range_first",
               "range_last", *ARRAYS):
         assert np.array_equal(g9[a], g[a]), a
+
+
+@pytest.mark.parametrize("ct", [O.CT_U64, O.CT_F64, O.CT_U32])
+def test_serialized_tree_round_trip(ft, orc, ct):
+    """A serialised tree carries the reference's MPI transfer sequence after its
+    64-byte header (test/test_mpi.cpp:76-100: dims, the six box arrays, left,
+    right, parent) and restores to an identical tree that answers queries
+    identically."""
+    periodic = 1 if ct == O.CT_U64 else 0
+    x, y, z = make_points(("nfw", 30000, 41), ct)
+    tree = gpu_build(ft, ct, x, y, z, O.SFC_PH, 21, periodic)
+    g = tree.to_host()
+    data = tree.serialize()
+    n = len(x)
+    m = 2 * n - 1
+    assert bytes(data[:8]) == b"FTTREE01"
+    assert np.array_equal(np.frombuffer(data, np.uint64, 2, 64), [n, n - 1])
+    dt = np.dtype(O.COORD_DTYPES[ct])
+    off = 80
+    for name in ARRAYS:
+        assert np.array_equal(np.frombuffer(data, dt, m, off), g[name][:m]), name
+        off += m * dt.itemsize
+    for name, cnt in (("left_child", n - 1), ("right_child", n - 1), ("parent", m)):
+        assert np.array_equal(np.frombuffer(data, np.int32, cnt, off), g[name][:cnt]), name
+        off += cnt * 4
+    t2 = ft.TreeSoA.deserialize(data)
+    assert t2.num_leaves == n and t2.coord_type == ct
+    g2 = t2.to_host()
+    for name in g:
+        assert np.array_equal(g2[name], g[name]), name
+    r1, d1 = ft.self_knn_query(tree, 8)
+    r2, d2 = ft.self_knn_query(t2, 8)
+    assert np.array_equal(r1, r2) and np.array_equal(d1, d2)
+    # a damaged header and a truncated buffer are errors
+    import ctypes as C
+    bad = bytearray(data)
+    bad[0:8] = b"NOTATREE"
+    t3 = ft.TreeSoA(1, ct)
+    buf = np.frombuffer(bytes(bad), np.uint8)
+    assert ft.lib().ft_tree_deserialize(None, C.byref(t3.c), buf.ctypes.data, buf.nbytes) == ft.FT_EINVAL
+    short = np.frombuffer(data[:len(data) // 2], np.uint8)
+    assert ft.lib().ft_tree_deserialize(None, C.byref(t3.c), short.ctypes.data, short.nbytes) == ft.FT_ERANGE
