@@ -141,6 +141,24 @@ def main():
     query_ms = float(np.median(queries))
     qps = n / (query_ms * 1e-3)
 
+    # The same step (build + packet self-kNN) captured once in a hipGraph and
+    # replayed: what a caller pays per rebuild of moved particles.
+    with ft.Graph(stream) as graph:
+        ft.build_bvh(p, tree, scratch=scratch, stream=stream.handle)
+        ft.self_knn_query(tree, k, results=res, dists=dst, exclude_self=True,
+                          stream=stream.handle, algorithm=ft.ALGO_PACKET, leaf_bucket=args.bucket,
+                          topk=args.topk, window=args.window)
+    gts = []
+    for _ in range(max(args.steps, 2)):
+        ev[0].record(stream)
+        graph.launch()
+        ev[3].record(stream)
+        stream.synchronize()
+        gts.append(ev[0].elapsed_ms(ev[3]))
+    graph_step_ms = float(np.median(gts))
+    del graph
+    log(f"graph-captured step (build + self-kNN): {graph_step_ms:.3f} ms")
+
     # Per-stage build times (events between stages on the same stream).
     scratch.set_profiling(True)
     ft.build_bvh(p, tree, scratch=scratch, stream=stream.handle)
@@ -303,6 +321,7 @@ def main():
                    "n": n, "k": k, "global_batch": n, "parallelism": "single GPU"},
         "build_ms": round(build_ms, 4),
         "build_stage_ms": {a: round(b, 4) for a, b in stages.items()},
+        "graph_step_ms": round(graph_step_ms, 4),
         "build_hbm_frac": round(build_gbs / PEAK_HBM_GBS, 4),
         "self_knn_ms": round(query_ms, 4),
         "ladder": {a: round(b, 1) for a, b in ladder.items()},

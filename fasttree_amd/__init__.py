@@ -83,7 +83,8 @@ class QueryOpts(C.Structure):
 EXPORTS = [
     "ft_last_error", "ft_version", "ft_device_count", "ft_malloc", "ft_free", "ft_memcpy_h2d",
     "ft_memcpy_d2h", "ft_memcpy_d2d", "ft_memset", "ft_copy_kernel", "ft_stream_create", "ft_stream_destroy",
-    "ft_stream_synchronize", "ft_event_create", "ft_event_destroy", "ft_event_record",
+    "ft_stream_synchronize", "ft_graph_begin", "ft_graph_end", "ft_graph_launch",
+    "ft_graph_destroy", "ft_event_create", "ft_event_destroy", "ft_event_record",
     "ft_event_elapsed_ms", "ft_tree_alloc", "ft_tree_ensure", "ft_tree_free",
     "ft_tree_serialized_size", "ft_tree_serialize", "ft_tree_deserialize",
     "ft_scratch_create", "ft_scratch_destroy", "ft_scratch_set_profiling", "ft_scratch_stage_ms",
@@ -119,6 +120,10 @@ def lib():
         L.ft_stream_create.argtypes = [C.POINTER(C.c_void_p)]
         L.ft_stream_destroy.argtypes = [vp]
         L.ft_stream_synchronize.argtypes = [vp]
+        L.ft_graph_begin.argtypes = [vp]
+        L.ft_graph_end.argtypes = [vp, C.POINTER(C.c_void_p)]
+        L.ft_graph_launch.argtypes = [vp, vp]
+        L.ft_graph_destroy.argtypes = [vp]
         L.ft_event_create.argtypes = [C.POINTER(C.c_void_p)]
         L.ft_event_destroy.argtypes = [vp]
         L.ft_event_record.argtypes = [vp, vp]
@@ -263,6 +268,39 @@ def _sh(stream):
     if stream is None:
         return None
     return stream.handle if isinstance(stream, Stream) else stream
+
+
+class Graph:
+    """The library calls issued on `stream` inside ``with Graph(stream) as g:``,
+    captured as a hipGraph; ``g.launch()`` replays them.  Run the step once
+    eagerly first, so that no buffer grows during the capture."""
+
+    def __init__(self, stream: Stream):
+        self.stream = stream
+        self.exec = None
+
+    def __enter__(self):
+        _check(lib().ft_graph_begin(_sh(self.stream)))
+        return self
+
+    def __exit__(self, et, ev, tb):
+        h = C.c_void_p()
+        rc = lib().ft_graph_end(_sh(self.stream), C.byref(h))
+        if et is None:
+            _check(rc)
+            self.exec = h.value
+        return False
+
+    def launch(self, stream=None):
+        _check(lib().ft_graph_launch(self.exec, _sh(stream if stream is not None else self.stream)))
+
+    def __del__(self):
+        try:
+            if self.exec:
+                lib().ft_graph_destroy(self.exec)
+                self.exec = None
+        except Exception:
+            pass
 
 
 class Event:

@@ -201,6 +201,36 @@ int ft_stream_synchronize(ft_stream s) {
     FT_HIP(hipStreamSynchronize(as_stream(s)));
     return FT_OK;
 }
+// hipGraph capture of the calls issued on a stream (relaxed mode: other
+// threads may keep allocating).  Calls that would allocate, copy to the host
+// or synchronise (growing a tree or scratch, staging host pointers, the CSR
+// range query) end the capture with an error; run the step once eagerly
+// first so that every buffer has its size.
+int ft_graph_begin(ft_stream s) {
+    if (!s) return fail(FT_EINVAL, "graph capture needs a created stream, not the null stream");
+    FT_HIP(hipStreamBeginCapture(as_stream(s), hipStreamCaptureModeRelaxed));
+    return FT_OK;
+}
+int ft_graph_end(ft_stream s, void **exec) {
+    if (!s || !exec) return fail(FT_EINVAL, "null argument");
+    hipGraph_t g = nullptr;
+    FT_HIP(hipStreamEndCapture(as_stream(s), &g));
+    hipGraphExec_t e = nullptr;
+    const hipError_t rc = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (rc != hipSuccess) return fail(FT_EHIP, "hipGraphInstantiate failed");
+    *exec = (void *)e;
+    return FT_OK;
+}
+int ft_graph_launch(void *exec, ft_stream s) {
+    if (!exec) return fail(FT_EINVAL, "null graph");
+    FT_HIP(hipGraphLaunch((hipGraphExec_t)exec, as_stream(s)));
+    return FT_OK;
+}
+int ft_graph_destroy(void *exec) {
+    if (exec) FT_HIP(hipGraphExecDestroy((hipGraphExec_t)exec));
+    return FT_OK;
+}
 int ft_event_create(void **ev) {
     hipEvent_t e;
     FT_HIP(hipEventCreate(&e));

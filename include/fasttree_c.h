@@ -153,6 +153,14 @@ int ft_memset(void *dst, int value, size_t bytes, ft_stream s);
 int ft_copy_kernel(void *dst, const void *src, size_t bytes, ft_stream s);
 int ft_stream_create(ft_stream *s);
 int ft_stream_destroy(ft_stream s);
+/* hipGraph capture of this library's calls on a created stream, e.g. a build_bvh +     */
+/* self_knn_query step replayed on moved particles (no reference counterpart).  Run the  */
+/* step once eagerly first: calls that would allocate, synchronise or stage host         */
+/* pointers during a capture fail it.  ft_graph_end instantiates the captured graph.     */
+int ft_graph_begin(ft_stream s);
+int ft_graph_end(ft_stream s, void **exec);
+int ft_graph_launch(void *exec, ft_stream s);
+int ft_graph_destroy(void *exec);
 int ft_stream_synchronize(ft_stream s);
 int ft_event_create(void **ev);
 int ft_event_destroy(void *ev);

@@ -271,3 +271,46 @@ def test_serialized_tree_round_trip(ft, orc, ct):
     assert ft.lib().ft_tree_deserialize(None, C.byref(t3.c), buf.ctypes.data, buf.nbytes) == ft.FT_EINVAL
     short = np.frombuffer(data[:len(data) // 2], np.uint8)
     assert ft.lib().ft_tree_deserialize(None, C.byref(t3.c), short.ctypes.data, short.nbytes) == ft.FT_ERANGE
+
+
+def test_graph_captured_rebuild_and_query(ft, orc):
+    """A build_bvh + self_knn_query step captured in a hipGraph and replayed on
+    kicked particles (new positions written into the same device arrays,
+    test/benchmark/rebuild_scaling.cpp's setting): every replay equals an eager
+    build and query of the same positions, with unrelated eager work and
+    synchronisations between the replays."""
+    n, k = 100_000, 16
+    x, y, z = make_points(("nfw", n, 51), O.CT_U64)
+    cfg = ft.Config(sfc=O.SFC_PH, bits_per_dim=21, periodic=1)
+    s = ft.Stream()
+    dx, dy, dz = (ft.DeviceArray.from_host(a) for a in (x, y, z))
+    p = ft.particles(dx, dy, dz)
+    tree, sc = ft.TreeSoA(n, O.CT_U64, cfg), ft.BvhScratch()
+    res, dst = ft.DeviceArray(n * k, np.uint64), ft.DeviceArray(n * k, np.uint64)
+
+    def step():
+        ft.build_bvh(p, tree, scratch=sc, stream=s.handle)
+        ft.self_knn_query(tree, k, results=res, dists=dst, stream=s.handle)
+
+    step()  # eager: sizes every buffer
+    s.synchronize()
+    with ft.Graph(s) as g:
+        step()
+    ref_tree, ref_sc = ft.TreeSoA(n, O.CT_U64, cfg), ft.BvhScratch()
+    rng = np.random.default_rng(3)
+    mask = np.uint64((1 << 21) - 1)
+    for rep in range(3):
+        kick = [rng.integers(-40, 41, n).astype(np.int64) for _ in range(3)]
+        new = [((a.astype(np.int64) + d) & int(mask)).astype(np.uint64) for a, d in zip((x, y, z), kick)]
+        for dev, a in zip((dx, dy, dz), new):
+            dev.copy_from(a)
+        g.launch()
+        s.synchronize()
+        # eager reference on separate buffers, and unrelated work in between
+        ft.build_bvh(ft.particles(*(ft.DeviceArray.from_host(a) for a in new)), ref_tree,
+                     scratch=ref_sc)
+        r_ref, d_ref = ft.self_knn_query(ref_tree, k)
+        ft.self_knn_query(ref_tree, 4)
+        assert np.array_equal(res.to_host().reshape(n, k), r_ref), rep
+        assert np.array_equal(dst.to_host().reshape(n, k), d_ref), rep
+        assert np.array_equal(tree.to_host()["parent"], ref_tree.to_host()["parent"]), rep
