@@ -742,8 +742,9 @@ __global__ __launch_bounds__(256) void assemble_top_kernel(int n, const uint64_t
     const int off = n - 1;
     const int lane = threadIdx.x & 63;
     const int waves = gridDim.x * (blockDim.x >> 6);
-    // one wave per chunk, one lane per chunk top
-    for (int c = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); c < chunks; c += waves)
+    // one wave per chunk, one lane per chunk top; XCD x takes a contiguous run
+    // of chunks, so the first climbs between neighbouring chunks stay on it
+    for (int c = xcd_block() * (blockDim.x >> 6) + (threadIdx.x >> 6); c < chunks; c += waves)
     for (int q = lane; q < (int)chunk_tops[c]; q += 64) {
         const int lab0 = tops[c * chunk_size + q];
         if (lab0 < 0 || lab0 > 2 * off) continue;  // never true for a list this build wrote
@@ -987,7 +988,7 @@ static int launch_assemble(hipStream_t s, ft_tree *t, Scratch &S, const LeafSrc<
     FT_CHECK_LAUNCH();
     prof_mark(S, s, 4);
     if (chunks > 1) {
-        hipLaunchKernelGGL(assemble_top_kernel<T>, dim3((chunks + 3) / 4), dim3(256), 0, s, n, keys,
+        hipLaunchKernelGGL(assemble_top_kernel<T>, dim3(xcd_grid((chunks + 3) / 4)), dim3(256), 0, s, n, keys,
                            o, (const int32_t *)S.tops, (const uint32_t *)S.chunk_tops, (int)chunks,
                            C, S.split_words, (const uint32_t *)S.topo_epoch);
         FT_CHECK_LAUNCH();
