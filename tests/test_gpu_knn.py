@@ -159,6 +159,30 @@ def test_edge_sizes_and_sentinels(ft, orc, n):
                 assert np.array_equal(d, dr) and np.array_equal(r, rr)
 
 
+@pytest.mark.parametrize("n", [1, 2, 3, 50, 65, 130, 1000])
+def test_windows_on_small_trees_subset_and_external(ft, orc, n):
+    """The packet walk's window pre-scan on trees smaller than or near one
+    64-leaf run: subset queries (window around the middle lane's rank, widened
+    for sparse subsets) and external queries (window around the leaf a greedy
+    descent reaches), with full, partial and single-query packets."""
+    rng = np.random.default_rng(n)
+    for ct in (O.CT_U64, O.CT_F64):
+        x, y, z = ref_points(O.GEN_UNIFORM, n, 4242 + n, ct)
+        periodic = 1 if ct == O.CT_U64 else 0
+        tree, t = make(ft, ct, O.SFC_PH, 21, periodic, (x, y, z))
+        for m in sorted({1, max(1, n // 3), n, 150}):
+            ids = rng.integers(0, n, m).astype(np.int32)
+            for k in (1, 8, 40):
+                r, d = ft.self_knn_query_subset(tree, ids, k, algorithm=ALGOS["packet"])
+                rb, db = O.knn_bruteforce(t, k, True, leaf_ids=ids)
+                assert np.array_equal(d, db) and np.array_equal(r, rb), (ct, m, k, "subset")
+        q = tuple(a[rng.integers(0, n, 97)].copy() for a in (x, y, z))
+        for k in (1, 8, 40):
+            r, d = ft.knn_query(tree, *q, k)
+            rb, db = O.knn_bruteforce(t, k, queries=q)
+            assert np.array_equal(d, db) and np.array_equal(r, rb), (ct, k, "external")
+
+
 def test_large_sample_against_bruteforce(ft, orc):
     """300k clustered int64 periodic particles, k=32: all queries on the GPU,
     a 3000-query sample checked against brute force."""
