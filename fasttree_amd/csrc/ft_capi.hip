@@ -43,6 +43,7 @@ int range_entry(hipStream_t s, const ft_tree *tree, const void *qx, const void *
                 int32_t *counts, int max_res, const int64_t *row_ptr, int32_t *cols,
                 int64_t col_cap);
 int scan_entry(hipStream_t s, const int32_t *counts, int nq, int64_t *row_ptr);
+int release_stream_bufs(hipStream_t s);
 int range_csr_entry(hipStream_t s, const ft_tree *tree, const void *qx, const void *qy,
                     const void *qz, const void *rmin, const void *rmax, int nq, int64_t *row_ptr,
                     int32_t *(*cols_for)(int64_t total, void *ctx), void *ctx, int64_t col_cap,
@@ -194,6 +195,7 @@ int ft_stream_create(ft_stream *s) {
     return FT_OK;
 }
 int ft_stream_destroy(ft_stream s) {
+    (void)release_stream_bufs(as_stream(s));
     FT_HIP(hipStreamDestroy(as_stream(s)));
     return FT_OK;
 }
@@ -443,9 +445,9 @@ int ft_scratch_create(ft_scratch **scratch) {
 int ft_scratch_destroy(ft_scratch *scratch) {
     if (!scratch) return FT_OK;
     Scratch &S = scratch->s;
-    if (S.block) hipFree(S.block);
+    if (S.block) (void)hipFree(S.block);
     for (auto &e : S.ev)
-        if (e) hipEventDestroy(e);
+        if (e) (void)hipEventDestroy(e);
     delete scratch;
     return FT_OK;
 }

@@ -17,6 +17,8 @@
 #include "ft_traverse.hpp"
 
 #include <algorithm>
+#include <mutex>
+#include <unordered_map>
 #include <vector>
 
 namespace ft {
@@ -753,10 +755,11 @@ template <typename T> static int lds_lanes(int k, int depth, size_t &lds) {
     return lds <= budget ? lanes : 0;
 }
 
-// Re-run buffers: per-slot flags, the compacted slot list, per-tile counts
-// and the list length.  Grown on demand, one set per process (calls on one
-// device are ordered on their streams; concurrent streams need separate
-// processes or the batch API).
+// Per-stream work buffers of the query paths: the re-run buffers (per-slot
+// flags, the compacted slot list, per-tile counts and the list length) and
+// the query-order sort buffers.  Each stream has its own set, grown on
+// demand, so queries on different streams do not share memory; a stream's
+// set is released by ft_stream_destroy.
 struct RerunBuf {
     uint8_t *flags = nullptr;
     int32_t *list = nullptr;
@@ -765,13 +768,53 @@ struct RerunBuf {
     size_t cap = 0;
     void *block = nullptr;
 };
-static RerunBuf g_rerun;
+struct QueryOrderBuf {
+    Scratch S;
+    uint64_t *keys = nullptr, *sorted = nullptr;
+    size_t cap = 0;
+};
+struct StreamBufs {
+    RerunBuf rr;
+    QueryOrderBuf qo;
+};
+static std::mutex g_bufs_mu;
+static std::unordered_map<hipStream_t, StreamBufs *> g_bufs;
 
-static int rerun_ensure(size_t n) {
+static StreamBufs &stream_bufs(hipStream_t s) {
+    std::lock_guard<std::mutex> lk(g_bufs_mu);
+    auto it = g_bufs.find(s);
+    if (it != g_bufs.end()) return *it->second;
+    StreamBufs *b = new StreamBufs();
+    g_bufs.emplace(s, b);
+    return *b;
+}
+
+// Frees the work buffers of stream s (after its pending work).
+int release_stream_bufs(hipStream_t s) {
+    StreamBufs *b = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_bufs_mu);
+        auto it = g_bufs.find(s);
+        if (it == g_bufs.end()) return FT_OK;
+        b = it->second;
+        g_bufs.erase(it);
+    }
+    (void)hipStreamSynchronize(s);
+    if (b->rr.block) (void)hipFree(b->rr.block);
+    if (b->qo.keys) (void)hipFree(b->qo.keys);
+    if (b->qo.S.block) (void)hipFree(b->qo.S.block);
+    for (auto &e : b->qo.S.ev)
+        if (e) (void)hipEventDestroy(e);
+    delete b;
+    return FT_OK;
+}
+
+static int rerun_ensure(hipStream_t s, size_t n) {
+    RerunBuf &g_rerun = stream_bufs(s).rr;
     if (n <= g_rerun.cap && g_rerun.block) return FT_OK;
-    // growing: earlier launches on any stream may still read the old buffers
+    // growing: this stream's earlier launches may still read the old buffers
     if (g_rerun.block) {
-        FT_HIP(hipDeviceSynchronize());
+        FT_HIP(hipStreamSynchronize(s));
         (void)hipFree(g_rerun.block);
     }
     g_rerun.block = nullptr;
@@ -809,6 +852,7 @@ static inline int bit_length(uint64_t v) {
 
 // Flags -> ascending slot list, count on the device.
 static int compact_flags(hipStream_t s, int n) {
+    RerunBuf &g_rerun = stream_bufs(s).rr;
     const unsigned nt = blocks_for((size_t)n, CP_TILE);
     hipLaunchKernelGGL(flag_count_kernel, dim3(nt), dim3(256), 0, s,
                        (const uint8_t *)g_rerun.flags, n, g_rerun.tiles);
@@ -837,6 +881,7 @@ template <typename T, int MK>
 static int launch_exact_listed(hipStream_t s, const DTree<T> &dt, const QSrc<T> &qs, int nq,
                                int k, int excl, const uint32_t *order, uint64_t *res,
                                typename CoordTraits<T>::D *d) {
+    RerunBuf &g_rerun = stream_bufs(s).rr;
     const QueryMap qm{order, g_rerun.list, g_rerun.count, nullptr};
     if (k <= 32) {
         const unsigned g = std::min<unsigned>(blocks_for(nq, LANE_BLOCK), 2048u);
@@ -893,7 +938,8 @@ static int run_canonical(hipStream_t s, const DTree<T> &dt, const QSrc<T> &qs, i
                          const typename CoordTraits<T>::D *seed, uint64_t *res,
                          typename CoordTraits<T>::D *d, uint64_t *stats) {
     constexpr bool INTD = !CoordTraits<T>::fp;
-    FT_TRY(rerun_ensure((size_t)nq));
+    FT_TRY(rerun_ensure(s, (size_t)nq));
+    RerunBuf &g_rerun = stream_bufs(s).rr;
     hipLaunchKernelGGL(fill_u8_kernel, dim3(std::min<unsigned>(blocks_for(nq, 256), 4096u)),
                        dim3(256), 0, s, g_rerun.flags, (size_t)nq);
     const QueryMap qm{order, nullptr, nullptr, g_rerun.flags};
@@ -952,20 +998,13 @@ static int run_canonical(hipStream_t s, const DTree<T> &dt, const QSrc<T> &qs, i
     return launch_exact_listed<T, MK>(s, dt, qs, nq, k, excl, order, res, d);
 }
 
-// Query-order sort buffers (one set per process, like the re-run buffers).
-struct QueryOrderBuf {
-    Scratch S;
-    uint64_t *keys = nullptr, *sorted = nullptr;
-    size_t cap = 0;
-};
-static QueryOrderBuf g_qorder;
-
 template <typename T>
 static int query_order(hipStream_t s, const DTree<T> &dt, const QSrc<T> &qs, int nq,
                        const uint32_t *&order) {
+    QueryOrderBuf &g_qorder = stream_bufs(s).qo;
     if (g_qorder.cap < (size_t)nq) {
         if (g_qorder.keys) {
-            FT_HIP(hipDeviceSynchronize());  // earlier launches may still read them
+            FT_HIP(hipStreamSynchronize(s));  // this stream's launches may still read them
             (void)hipFree(g_qorder.keys);
         }
         g_qorder.keys = nullptr;
@@ -1214,8 +1253,9 @@ static int range_csr_t(hipStream_t s, const ft_tree &tree, const T *qx, const T 
         hipLaunchKernelGGL(csr_copy_kernel, dim3(std::min<unsigned>(blocks_for(nq, 4), 16384u)),
                            dim3(256), 0, s, (const int32_t *)pool, (const int32_t *)counts,
                            (const int64_t *)row_ptr, nq, cols, col_cap);
-        rc = rerun_ensure((size_t)nq);
+        rc = rerun_ensure(s, (size_t)nq);
     }
+    RerunBuf &g_rerun = stream_bufs(s).rr;
     uint32_t n_over = 0;
     if (rc == FT_OK) {
         hipLaunchKernelGGL(csr_overflow_flags_kernel, dim3(std::min<unsigned>(blocks_for(nq, 256), 4096u)),

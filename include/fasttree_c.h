@@ -151,12 +151,17 @@ int ft_memset(void *dst, int value, size_t bytes, ft_stream s);
 /* Device-to-device copy by a plain streaming kernel (16-byte aligned): the       */
 /* known-size reference against which hardware byte counters are scaled.         */
 int ft_copy_kernel(void *dst, const void *src, size_t bytes, ft_stream s);
+/* Queries keep small work buffers per stream (re-run lists, query ordering), so calls  */
+/* on different streams are independent; ft_stream_destroy waits for the stream's work  */
+/* and frees them.                                                                       */
 int ft_stream_create(ft_stream *s);
 int ft_stream_destroy(ft_stream s);
 /* hipGraph capture of this library's calls on a created stream, e.g. a build_bvh +     */
 /* self_knn_query step replayed on moved particles (no reference counterpart).  Run the  */
 /* step once eagerly first: calls that would allocate, synchronise or stage host         */
 /* pointers during a capture fail it.  ft_graph_end instantiates the captured graph.     */
+/* A graph uses the work buffers of the stream it was captured on: destroy the graph     */
+/* before that stream, and do not replay it while that stream runs other queries.        */
 int ft_graph_begin(ft_stream s);
 int ft_graph_end(ft_stream s, void **exec);
 int ft_graph_launch(void *exec, ft_stream s);

@@ -115,6 +115,46 @@ def test_external_knn(ft, orc):
         assert np.array_equal(d, db) and np.array_equal(r, rb), f"k={k}"
 
 
+def test_queries_on_two_streams_run_independently(ft, orc):
+    """Seeded self-kNN (re-run lists) and external kNN (query ordering) issued on
+    two streams with no synchronisation in between: each stream's results equal
+    the brute force of its own tree (every stream has its own work buffers)."""
+    k = 16
+    cases = []
+    for seed, n, nq in ((31, 30000, 9000), (32, 45000, 5000)):
+        pts = nfw_points(n, seed, O.CT_U64)
+        tree, t = make(ft, O.CT_U64, O.SFC_PH, 21, 1, pts)
+        rb, db = O.knn_bruteforce(t, k, True)
+        scale = np.where(np.arange(n) % 3 == 0, 0.5, 1.5)  # a third of the seeds too small
+        sb = np.floor(db[:, -1].astype(np.float64) * scale).astype(np.uint64)
+        rng = np.random.default_rng(seed)
+        q = tuple(rng.integers(0, 1 << 21, nq, dtype=np.uint64) for _ in range(3))
+        qb = O.knn_bruteforce(t, k, queries=q)
+        cases.append(dict(tree=tree, n=n, nq=nq, self_ref=(rb, db), ext_ref=qb,
+                          seed=ft.DeviceArray.from_host(sb),
+                          q=tuple(ft.DeviceArray.from_host(a) for a in q),
+                          stats=ft.DeviceArray(8, np.uint64).zero(),
+                          out=[ft.DeviceArray(m * k, np.uint64) for m in (n, n, nq, nq)],
+                          stream=ft.Stream()))
+    for rep in range(2):  # the second round reuses each stream's grown buffers
+        for c in cases:  # interleaved: both streams' work overlaps
+            ft.self_knn_query(c["tree"], k, results=c["out"][0], dists=c["out"][1],
+                              seed_bound=c["seed"], stats=c["stats"], max_k=256,
+                              stream=c["stream"])
+        for c in cases:
+            ft.knn_query(c["tree"], *c["q"], k, results=c["out"][2], dists=c["out"][3],
+                         stream=c["stream"])
+        for c in cases:
+            c["stream"].synchronize()
+            n, nq = c["n"], c["nq"]
+            r, d, qr, qd = (a.to_host() for a in c["out"])
+            assert np.array_equal(r.reshape(n, k), c["self_ref"][0]), (rep, n)
+            assert np.array_equal(d.reshape(n, k), c["self_ref"][1]), (rep, n)
+            assert np.array_equal(qr.reshape(nq, k), c["ext_ref"][0]), (rep, n)
+            assert np.array_equal(qd.reshape(nq, k), c["ext_ref"][1]), (rep, n)
+            assert c["stats"].to_host()[3] > 0  # re-runs happened on this stream
+
+
 def test_external_knn_reference_merge_tier(ft, orc):
     """knn_query k > 32 in reference mode: SortedMergeHeap tie order."""
     pts = ref_points(O.GEN_UNIFORM, 1000, 42, O.CT_U64)
