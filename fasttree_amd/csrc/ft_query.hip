@@ -683,16 +683,34 @@ __global__ __launch_bounds__(256) void scan_block_kernel(const int32_t *__restri
     if (threadIdx.x == 255) block_sums[blockIdx.x] = run;
 }
 
-__global__ __launch_bounds__(64) void scan_totals_kernel(int64_t *block_sums, int nb,
-                                                         int64_t *out, int nq) {
-    if (threadIdx.x != 0) return;
-    int64_t run = 0;
-    for (int b = 0; b < nb; ++b) {
+// Block totals -> exclusive offsets in place, and the grand total into
+// out[nq]: one 1024-thread block, each thread taking a contiguous run of
+// ceil(nb / 1024) totals (a single lane walking 20k totals took 2 ms).
+__global__ __launch_bounds__(1024) void scan_totals_kernel(int64_t *block_sums, int nb,
+                                                           int64_t *out, int nq) {
+    __shared__ int64_t wsum[16];
+    const int per = (nb + 1023) / 1024;
+    const int b0 = (int)threadIdx.x * per;
+    const int b1 = b0 + per < nb ? b0 + per : nb;
+    int64_t s = 0;
+    for (int b = b0; b < b1; ++b) s += block_sums[b];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int64_t x = s;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        int64_t y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    int64_t run = x - s;
+    for (int q = 0; q < w; ++q) run += wsum[q];
+    for (int b = b0; b < b1; ++b) {
         const int64_t v = block_sums[b];
         block_sums[b] = run;
         run += v;
     }
-    out[nq] = run;
+    if (threadIdx.x == 1023) out[nq] = run;
 }
 
 __global__ __launch_bounds__(256) void scan_add_kernel(int64_t *out, int nq,
@@ -1165,21 +1183,44 @@ int range_entry(hipStream_t s, const ft_tree *tree, const void *qx, const void *
 // ---------------------------------------------------------------------------
 constexpr int CSR_POOL = 128;
 
-// One wave per row (grid-stride over rows): rows of at most CSR_POOL hits.
+// Rows of at most CSR_POOL hits from the pool into cols.  One wave per group
+// of 64 rows (grid-stride over groups): the lanes load the group's counts and
+// offsets in one pass, then the wave copies four rows at a time, their pool
+// loads issued before their stores (one row at a time, each waiting on its
+// count and offset, ran at 1.9 TB/s).
 __global__ __launch_bounds__(256) void csr_copy_kernel(const int32_t *__restrict__ pool,
                                                        const int32_t *__restrict__ counts,
                                                        const int64_t *__restrict__ row_ptr, int nq,
                                                        int32_t *__restrict__ cols,
                                                        int64_t col_cap) {
     const int lane = threadIdx.x & 63;
-    const int w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    const int nw = gridDim.x * (blockDim.x >> 6);
-    for (int q = w; q < nq; q += nw) {
-        const int c = counts[q];
-        if (c > CSR_POOL) continue;
-        const int64_t base = row_ptr[q];
-        for (int j = lane; j < c; j += 64)
-            if (base + j < col_cap) cols[base + j] = pool[(size_t)q * CSR_POOL + j];
+    const int64_t w = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int64_t nw = (int64_t)gridDim.x * (blockDim.x >> 6);
+    for (int64_t g = w; g * 64 < nq; g += nw) {
+        const int64_t q0 = g * 64;
+        const int64_t ql = q0 + lane;
+        // rows past nq count 0 hits, so nothing below touches them
+        const int cl = ql < nq ? counts[ql] : 0;
+        const int64_t bl = ql < nq ? row_ptr[ql] : 0;
+        for (int r = 0; r < 64; r += 4) {
+            int32_t v0[4], v1[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int c = __shfl(cl, r + u, 64);
+                const size_t off = (size_t)(q0 + r + u) * CSR_POOL;
+                const bool fits = c <= CSR_POOL;
+                v0[u] = (fits && lane < c) ? pool[off + lane] : 0;
+                v1[u] = (fits && lane + 64 < c) ? pool[off + lane + 64] : 0;
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int c = __shfl(cl, r + u, 64);
+                const int64_t base = __shfl(bl, r + u, 64);
+                if (c > CSR_POOL) continue;
+                if (lane < c && base + lane < col_cap) cols[base + lane] = v0[u];
+                if (lane + 64 < c && base + lane + 64 < col_cap) cols[base + lane + 64] = v1[u];
+            }
+        }
     }
 }
 
@@ -1250,7 +1291,7 @@ static int range_csr_t(hipStream_t s, const ft_tree &tree, const T *qx, const T 
         return rc;
     }
     if (rc == FT_OK) {
-        hipLaunchKernelGGL(csr_copy_kernel, dim3(std::min<unsigned>(blocks_for(nq, 4), 16384u)),
+        hipLaunchKernelGGL(csr_copy_kernel, dim3(std::min<unsigned>(blocks_for((nq + 63) / 64, 4), 16384u)),
                            dim3(256), 0, s, (const int32_t *)pool, (const int32_t *)counts,
                            (const int64_t *)row_ptr, nq, cols, col_cap);
         rc = rerun_ensure(s, (size_t)nq);
@@ -1302,7 +1343,7 @@ int scan_entry(hipStream_t s, const int32_t *counts, int nq, int64_t *row_ptr) {
     FT_HIP(hipMallocAsync((void **)&sums, (size_t)(nb > 0 ? nb : 1) * 8, s));
     if (nq > 0) {
         hipLaunchKernelGGL(scan_block_kernel, dim3(nb), dim3(256), 0, s, counts, nq, row_ptr, sums);
-        hipLaunchKernelGGL(scan_totals_kernel, dim3(1), dim3(64), 0, s, sums, nb, row_ptr, nq);
+        hipLaunchKernelGGL(scan_totals_kernel, dim3(1), dim3(1024), 0, s, sums, nb, row_ptr, nq);
         hipLaunchKernelGGL(scan_add_kernel, dim3(blocks_for(nq, 256)), dim3(256), 0, s, row_ptr,
                            nq, (const int64_t *)sums);
     } else {
