@@ -82,8 +82,8 @@ class QueryOpts(C.Structure):
 
 EXPORTS = [
     "ft_last_error", "ft_version", "ft_device_count", "ft_malloc", "ft_free", "ft_memcpy_h2d",
-    "ft_memcpy_d2h", "ft_memcpy_d2d", "ft_memset", "ft_copy_kernel", "ft_stream_create", "ft_stream_destroy",
-    "ft_stream_synchronize", "ft_graph_begin", "ft_graph_end", "ft_graph_launch",
+    "ft_memcpy_d2h", "ft_memcpy_d2d", "ft_memset", "ft_copy_kernel", "ft_stream_create",
+    "ft_stream_destroy", "ft_stream_release", "ft_stream_synchronize", "ft_graph_begin", "ft_graph_end", "ft_graph_launch",
     "ft_graph_destroy", "ft_event_create", "ft_event_destroy", "ft_event_record",
     "ft_event_elapsed_ms", "ft_tree_alloc", "ft_tree_ensure", "ft_tree_free",
     "ft_tree_serialized_size", "ft_tree_serialize", "ft_tree_deserialize",
@@ -119,6 +119,7 @@ def lib():
         L.ft_copy_kernel.argtypes = [vp, vp, sz, vp]
         L.ft_stream_create.argtypes = [C.POINTER(C.c_void_p)]
         L.ft_stream_destroy.argtypes = [vp]
+        L.ft_stream_release.argtypes = [vp]
         L.ft_stream_synchronize.argtypes = [vp]
         L.ft_graph_begin.argtypes = [vp]
         L.ft_graph_end.argtypes = [vp, C.POINTER(C.c_void_p)]

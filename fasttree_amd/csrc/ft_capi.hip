@@ -194,6 +194,7 @@ int ft_stream_create(ft_stream *s) {
     *s = (ft_stream)h;
     return FT_OK;
 }
+int ft_stream_release(ft_stream s) { return release_stream_bufs(as_stream(s)); }
 int ft_stream_destroy(ft_stream s) {
     (void)release_stream_bufs(as_stream(s));
     FT_HIP(hipStreamDestroy(as_stream(s)));

@@ -153,9 +153,11 @@ int ft_memset(void *dst, int value, size_t bytes, ft_stream s);
 int ft_copy_kernel(void *dst, const void *src, size_t bytes, ft_stream s);
 /* Queries keep small work buffers per stream (re-run lists, query ordering), so calls  */
 /* on different streams are independent; ft_stream_destroy waits for the stream's work  */
-/* and frees them.                                                                       */
+/* and frees them.  ft_stream_release does only the latter: call it before destroying a */
+/* stream that was not made by ft_stream_create (no counterpart in the reference).      */
 int ft_stream_create(ft_stream *s);
 int ft_stream_destroy(ft_stream s);
+int ft_stream_release(ft_stream s);
 /* hipGraph capture of this library's calls on a created stream, e.g. a build_bvh +     */
 /* self_knn_query step replayed on moved particles (no reference counterpart).  Run the  */
 /* step once eagerly first: calls that would allocate, synchronise or stage host         */

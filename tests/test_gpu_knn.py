@@ -153,6 +153,16 @@ def test_queries_on_two_streams_run_independently(ft, orc):
             assert np.array_equal(qr.reshape(nq, k), c["ext_ref"][0]), (rep, n)
             assert np.array_equal(qd.reshape(nq, k), c["ext_ref"][1]), (rep, n)
             assert c["stats"].to_host()[3] > 0  # re-runs happened on this stream
+    # after ft_stream_release the next query on the stream makes its buffers again
+    c = cases[0]
+    ft._check(ft.lib().ft_stream_release(c["stream"].handle))
+    ft.self_knn_query(c["tree"], k, results=c["out"][0], dists=c["out"][1], seed_bound=c["seed"],
+                      max_k=256, stream=c["stream"])
+    ft.knn_query(c["tree"], *c["q"], k, results=c["out"][2], dists=c["out"][3],
+                 stream=c["stream"])
+    c["stream"].synchronize()
+    assert np.array_equal(c["out"][1].to_host().reshape(c["n"], k), c["self_ref"][1])
+    assert np.array_equal(c["out"][2].to_host().reshape(c["nq"], k), c["ext_ref"][0])
 
 
 def test_external_knn_reference_merge_tier(ft, orc):
