@@ -44,6 +44,8 @@ int range_entry(hipStream_t s, const ft_tree *tree, const void *qx, const void *
                 int64_t col_cap);
 int scan_entry(hipStream_t s, const int32_t *counts, int nq, int64_t *row_ptr);
 int release_stream_bufs(hipStream_t s);
+void graph_attach(hipStream_t s, void *exec);
+void graph_detach(void *exec);
 int range_csr_entry(hipStream_t s, const ft_tree *tree, const void *qx, const void *qy,
                     const void *qz, const void *rmin, const void *rmax, int nq, int64_t *row_ptr,
                     int32_t *(*cols_for)(int64_t total, void *ctx), void *ctx, int64_t col_cap,
@@ -64,24 +66,39 @@ bool is_device_accessible(const void *p) {
            a.type == hipMemoryTypeHost || a.type == hipMemoryTypeUnified;
 }
 
+// Releases what st still owns and clears it.
+static void stage_reset(Staged &st) {
+    if (st.owned) (void)hipFreeAsync(st.dev, st.stream);
+    st.dev = nullptr;
+    st.host_src = nullptr;
+    st.host_dst = nullptr;
+    st.bytes = 0;
+    st.owned = false;
+    st.stream = nullptr;
+}
+
 int stage_in(const void *p, size_t bytes, hipStream_t s, Staged &st) {
-    st = Staged{};
+    stage_reset(st);
     if (!p || bytes == 0 || is_device_accessible(p)) {
         st.dev = const_cast<void *>(p);
         return FT_OK;
     }
     void *d = nullptr;
     if (hipMallocAsync(&d, bytes, s) != hipSuccess) return fail(FT_ENOMEM, "staging allocation failed");
-    FT_HIP(hipMemcpyAsync(d, p, bytes, hipMemcpyHostToDevice, s));
+    if (hipMemcpyAsync(d, p, bytes, hipMemcpyHostToDevice, s) != hipSuccess) {
+        (void)hipFreeAsync(d, s);
+        return fail(FT_EHIP, "staging copy to the device failed");
+    }
     st.dev = d;
     st.host_src = p;
     st.bytes = bytes;
     st.owned = true;
+    st.stream = s;
     return FT_OK;
 }
 
 int stage_out(void *p, size_t bytes, hipStream_t s, Staged &st) {
-    st = Staged{};
+    stage_reset(st);
     if (!p || bytes == 0 || is_device_accessible(p)) {
         st.dev = p;
         return FT_OK;
@@ -92,6 +109,7 @@ int stage_out(void *p, size_t bytes, hipStream_t s, Staged &st) {
     st.host_dst = p;
     st.bytes = bytes;
     st.owned = true;
+    st.stream = s;
     return FT_OK;
 }
 
@@ -222,6 +240,7 @@ int ft_graph_end(ft_stream s, void **exec) {
     const hipError_t rc = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     if (rc != hipSuccess) return fail(FT_EHIP, "hipGraphInstantiate failed");
+    graph_attach(as_stream(s), (void *)e);  // the stream's work buffers outlive the graph
     *exec = (void *)e;
     return FT_OK;
 }
@@ -231,7 +250,10 @@ int ft_graph_launch(void *exec, ft_stream s) {
     return FT_OK;
 }
 int ft_graph_destroy(void *exec) {
-    if (exec) FT_HIP(hipGraphExecDestroy((hipGraphExec_t)exec));
+    if (!exec) return FT_OK;
+    const hipError_t rc = hipGraphExecDestroy((hipGraphExec_t)exec);
+    graph_detach(exec);
+    FT_HIP(rc);
     return FT_OK;
 }
 int ft_event_create(void **ev) {
@@ -382,6 +404,48 @@ static size_t tree_bytes(int32_t coord_type, size_t n) {
 }
 }  // namespace
 
+// Checks, on the host, every index a traversal follows in a serialised tree
+// of n leaves (buffer `in`, host or device memory): children in [0, 2n-1),
+// every node but the root the child of exactly one internal node whose label
+// its parent entry names (so the nodes reachable from the root form a tree and
+// every walk ends), and leaf ranges 0 <= first <= last < n.
+static int check_topology(const char *in, int32_t coord_type, size_t n, hipStream_t hs) {
+    if (n < 2) return FT_OK;
+    ft_tree t{};
+    t.coord_type = coord_type;
+    const std::vector<TreePart> parts = tree_parts(t, n);
+    std::vector<size_t> off(parts.size());
+    size_t at = sizeof(TreeHeader) + 16;
+    for (size_t i = 0; i < parts.size(); ++i) {
+        off[i] = at;
+        at += parts[i].bytes;
+    }
+    const size_t in_n = n - 1, m = 2 * n - 1;
+    std::vector<int32_t> left(in_n), right(in_n), parent(m), rf(in_n), rl(in_n);
+    auto get = [&](std::vector<int32_t> &v, int part) -> int {
+        FT_HIP(hipMemcpyAsync(v.data(), in + off[part], v.size() * 4, hipMemcpyDefault, hs));
+        return FT_OK;
+    };
+    FT_TRY(get(left, 6));
+    FT_TRY(get(right, 7));
+    FT_TRY(get(parent, 8));
+    FT_TRY(get(rf, 13));
+    FT_TRY(get(rl, 14));
+    FT_HIP(hipStreamSynchronize(hs));
+    std::vector<uint8_t> seen(m, 0);
+    for (size_t i = 0; i < in_n; ++i) {
+        for (const int32_t c : {left[i], right[i]}) {
+            if (c <= 0 || (size_t)c >= m || seen[c] || parent[c] != (int32_t)i)
+                return fail(FT_EINVAL, "serialised tree: corrupt child or parent index");
+            seen[c] = 1;
+        }
+        if (rf[i] < 0 || rf[i] > rl[i] || (size_t)rl[i] >= n)
+            return fail(FT_EINVAL, "serialised tree: corrupt leaf range");
+    }
+    if (parent[0] != -1) return fail(FT_EINVAL, "serialised tree: corrupt root");
+    return FT_OK;
+}
+
 int ft_tree_serialized_size(const ft_tree *tree, size_t *bytes) {
     if (!tree || !bytes) return fail(FT_EINVAL, "null argument");
     *bytes = tree_bytes(tree->coord_type, tree->num_leaves);
@@ -427,6 +491,9 @@ int ft_tree_deserialize(ft_stream s, ft_tree *tree, const void *buf, size_t byte
     if (h.coord_type < FT_COORD_F32 || h.coord_type > FT_COORD_U64)
         return fail(FT_EINVAL, "serialised tree: unknown coordinate type");
     if (bytes < tree_bytes(h.coord_type, n)) return fail(FT_ERANGE, "serialised tree is truncated");
+    if (n > (size_t)(INT_MAX / 2)) return fail(FT_EINVAL, "serialised tree: too many leaves");
+    FT_TRY(valid_config(&h.config, h.coord_type));
+    FT_TRY(check_topology(in, h.coord_type, n, hs));
     FT_TRY(ft_tree_ensure(tree, n, h.coord_type, &h.config, s));
     size_t at = sizeof h + 16;
     for (const TreePart &p : tree_parts(*tree, n)) {

@@ -254,12 +254,22 @@ template <typename T> inline DTree<T> make_dtree(const ft_tree &t) {
 // ---------------------------------------------------------------------------
 bool is_device_accessible(const void *p);
 
+// A staged buffer frees its device copy (stream-ordered on the stream it was
+// staged on) when it goes out of scope without unstage, so an entry point
+// that returns early on an error leaks nothing.
 struct Staged {
     void *dev = nullptr;
     const void *host_src = nullptr;
     void *host_dst = nullptr;
     size_t bytes = 0;
     bool owned = false;
+    hipStream_t stream = nullptr;
+    Staged() = default;
+    Staged(const Staged &) = delete;
+    Staged &operator=(const Staged &) = delete;
+    ~Staged() {
+        if (owned) (void)hipFreeAsync(dev, stream);
+    }
 };
 
 // Returns a device-readable pointer for p (copying host data in), or p itself.

@@ -777,7 +777,13 @@ template <typename T> static int lds_lanes(int k, int depth, size_t &lds) {
 // flags, the compacted slot list, per-tile counts and the list length) and
 // the query-order sort buffers.  Each stream has its own set, grown on
 // demand, so queries on different streams do not share memory; a stream's
-// set is released by ft_stream_destroy.
+// set is released by ft_stream_destroy / ft_stream_release.
+//
+// A graph captured on a stream holds the raw addresses of that stream's
+// buffers.  While any such graph lives, a buffer that grows, or a set that is
+// released, is not freed: its block joins the set's retired list, freed when
+// the stream's last graph is destroyed, so a replay never touches freed
+// memory.
 struct RerunBuf {
     uint8_t *flags = nullptr;
     int32_t *list = nullptr;
@@ -794,9 +800,13 @@ struct QueryOrderBuf {
 struct StreamBufs {
     RerunBuf rr;
     QueryOrderBuf qo;
+    int graphs = 0;               // live graphs captured on the stream
+    bool released = false;        // out of the map, kept for its graphs
+    std::vector<void *> retired;  // blocks replaced while graphs > 0
 };
 static std::mutex g_bufs_mu;
 static std::unordered_map<hipStream_t, StreamBufs *> g_bufs;
+static std::unordered_map<void *, StreamBufs *> g_graph_bufs;
 
 static StreamBufs &stream_bufs(hipStream_t s) {
     std::lock_guard<std::mutex> lk(g_bufs_mu);
@@ -807,7 +817,35 @@ static StreamBufs &stream_bufs(hipStream_t s) {
     return *b;
 }
 
-// Frees the work buffers of stream s (after its pending work).
+// Frees block p of stream s's set b after s's pending work, or keeps it on
+// the retired list while a graph captured on s may still use it.
+static int drop_block(StreamBufs &b, hipStream_t s, void *p) {
+    if (!p) return FT_OK;
+    {
+        std::lock_guard<std::mutex> lk(g_bufs_mu);
+        if (b.graphs > 0) {
+            b.retired.push_back(p);
+            return FT_OK;
+        }
+    }
+    FT_HIP(hipStreamSynchronize(s));  // this stream's launches may still use it
+    (void)hipFree(p);
+    return FT_OK;
+}
+
+static void free_set(StreamBufs *b) {
+    if (b->rr.block) (void)hipFree(b->rr.block);
+    if (b->qo.keys) (void)hipFree(b->qo.keys);
+    if (b->qo.S.block) (void)hipFree(b->qo.S.block);
+    for (void *p : b->retired) (void)hipFree(p);
+    for (auto &e : b->qo.S.ev)
+        if (e) (void)hipEventDestroy(e);
+    delete b;
+}
+
+// Frees the work buffers of stream s (after its pending work).  A set that a
+// live graph uses leaves the map (the stream's next query makes a new one)
+// and is freed with that graph.
 int release_stream_bufs(hipStream_t s) {
     StreamBufs *b = nullptr;
     {
@@ -816,26 +854,54 @@ int release_stream_bufs(hipStream_t s) {
         if (it == g_bufs.end()) return FT_OK;
         b = it->second;
         g_bufs.erase(it);
+        if (b->graphs > 0) {
+            b->released = true;
+            return FT_OK;
+        }
     }
     (void)hipStreamSynchronize(s);
-    if (b->rr.block) (void)hipFree(b->rr.block);
-    if (b->qo.keys) (void)hipFree(b->qo.keys);
-    if (b->qo.S.block) (void)hipFree(b->qo.S.block);
-    for (auto &e : b->qo.S.ev)
-        if (e) (void)hipEventDestroy(e);
-    delete b;
+    free_set(b);
     return FT_OK;
 }
 
-static int rerun_ensure(hipStream_t s, size_t n) {
-    RerunBuf &g_rerun = stream_bufs(s).rr;
-    if (n <= g_rerun.cap && g_rerun.block) return FT_OK;
-    // growing: this stream's earlier launches may still read the old buffers
-    if (g_rerun.block) {
-        FT_HIP(hipStreamSynchronize(s));
-        (void)hipFree(g_rerun.block);
+// A graph exec captured on stream s now holds s's buffers.
+void graph_attach(hipStream_t s, void *exec) {
+    StreamBufs &b = stream_bufs(s);
+    std::lock_guard<std::mutex> lk(g_bufs_mu);
+    ++b.graphs;
+    g_graph_bufs[exec] = &b;
+}
+
+// The graph exec is destroyed: when it was its stream's last, the retired
+// blocks (and a released set) are freed once the device is idle, since the
+// graph may have been launched on any stream.
+void graph_detach(void *exec) {
+    StreamBufs *b = nullptr;
+    std::vector<void *> old;
+    bool drop_set = false;
+    {
+        std::lock_guard<std::mutex> lk(g_bufs_mu);
+        auto it = g_graph_bufs.find(exec);
+        if (it == g_graph_bufs.end()) return;
+        b = it->second;
+        g_graph_bufs.erase(it);
+        if (--b->graphs > 0) return;
+        old.swap(b->retired);
+        drop_set = b->released;
     }
+    if (old.empty() && !drop_set) return;
+    (void)hipDeviceSynchronize();
+    for (void *p : old) (void)hipFree(p);
+    if (drop_set) free_set(b);
+}
+
+static int rerun_ensure(hipStream_t s, size_t n) {
+    StreamBufs &sb = stream_bufs(s);
+    RerunBuf &g_rerun = sb.rr;
+    if (n <= g_rerun.cap && g_rerun.block) return FT_OK;
+    FT_TRY(drop_block(sb, s, g_rerun.block));
     g_rerun.block = nullptr;
+    g_rerun.cap = 0;
     const size_t cap = n + n / 2 + 1024;
     const size_t tiles = cap / CP_TILE + 2;
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
@@ -1019,18 +1085,24 @@ static int run_canonical(hipStream_t s, const DTree<T> &dt, const QSrc<T> &qs, i
 template <typename T>
 static int query_order(hipStream_t s, const DTree<T> &dt, const QSrc<T> &qs, int nq,
                        const uint32_t *&order) {
-    QueryOrderBuf &g_qorder = stream_bufs(s).qo;
+    StreamBufs &sb = stream_bufs(s);
+    QueryOrderBuf &g_qorder = sb.qo;
     if (g_qorder.cap < (size_t)nq) {
-        if (g_qorder.keys) {
-            FT_HIP(hipStreamSynchronize(s));  // this stream's launches may still read them
-            (void)hipFree(g_qorder.keys);
-        }
+        FT_TRY(drop_block(sb, s, g_qorder.keys));
         g_qorder.keys = nullptr;
+        g_qorder.cap = 0;
         const size_t cap = (size_t)nq + (size_t)nq / 2 + 1024;
         if (hipMalloc(&g_qorder.keys, 2 * cap * 8) != hipSuccess)
             return fail(FT_ENOMEM, "query order allocation failed");
         g_qorder.sorted = g_qorder.keys + cap;
         g_qorder.cap = cap;
+    }
+    if ((size_t)nq > g_qorder.S.cap_n && g_qorder.S.block) {
+        // the sort scratch grows: hand its block over here, so that the
+        // sort's own growth allocates a new one and frees nothing
+        FT_TRY(drop_block(sb, s, g_qorder.S.block));
+        g_qorder.S.block = nullptr;
+        g_qorder.S.cap_n = 0;
     }
     hipLaunchKernelGGL((query_key_kernel<T>), dim3(blocks_for(nq, 256)), dim3(256), 0, s, dt, qs,
                        nq, g_qorder.keys);

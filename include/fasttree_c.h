@@ -162,8 +162,12 @@ int ft_stream_release(ft_stream s);
 /* self_knn_query step replayed on moved particles (no reference counterpart).  Run the  */
 /* step once eagerly first: calls that would allocate, synchronise or stage host         */
 /* pointers during a capture fail it.  ft_graph_end instantiates the captured graph.     */
-/* A graph uses the work buffers of the stream it was captured on: destroy the graph     */
-/* before that stream, and do not replay it while that stream runs other queries.        */
+/* A graph uses the work buffers of the stream it was captured on.  They stay allocated  */
+/* until the graph is destroyed, also when a later, larger query on that stream grows    */
+/* them or ft_stream_release / ft_stream_destroy drops them, so a replay never reads     */
+/* freed memory.  Do not replay a graph while its stream runs other queries (they share  */
+/* those buffers).  Trees and build scratch are the caller's: a graph that captured a    */
+/* build or query must not be replayed after ft_tree_ensure or a larger build grew them. */
 int ft_graph_begin(ft_stream s);
 int ft_graph_end(ft_stream s, void **exec);
 int ft_graph_launch(void *exec, ft_stream s);

@@ -234,26 +234,46 @@ def test_full_size_build_properties(ft, orc, n):
 @pytest.mark.parametrize("ct", [O.CT_U64, O.CT_F64, O.CT_U32])
 def test_serialized_tree_round_trip(ft, orc, ct):
     """A serialised tree carries the reference's MPI transfer sequence after its
-    64-byte header (test/test_mpi.cpp:76-100: dims, the six box arrays, left,
-    right, parent) and restores to an identical tree that answers queries
-    identically."""
+    64-byte header (test/test_mpi.cpp:76-100: the u64 dims [num_leaves,
+    num_internal], then min_x, max_x, min_y, max_y, min_z, max_z, left_child,
+    right_child, parent), then id, is_ghost, orig_idx, the sorted keys and the
+    leaf ranges.  Every byte range is checked against the arrays of the tree the
+    oracle builds from the same points, so a field-order or size mistake shared
+    by serialise and deserialise fails.  The buffer restores to an identical tree
+    that answers queries identically; damaged, truncated and topologically
+    corrupt buffers are errors, not trees."""
+    import ctypes as C
     periodic = 1 if ct == O.CT_U64 else 0
     x, y, z = make_points(("nfw", 30000, 41), ct)
     tree = gpu_build(ft, ct, x, y, z, O.SFC_PH, 21, periodic)
+    t = O.build_bvh(ct, x, y, z, sfc=O.SFC_PH, bits=21, periodic=periodic)
     g = tree.to_host()
     data = tree.serialize()
     n = len(x)
     m = 2 * n - 1
     assert bytes(data[:8]) == b"FTTREE01"
-    assert np.array_equal(np.frombuffer(data, np.uint64, 2, 64), [n, n - 1])
     dt = np.dtype(O.COORD_DTYPES[ct])
-    off = 80
-    for name in ARRAYS:
-        assert np.array_equal(np.frombuffer(data, dt, m, off), g[name][:m]), name
-        off += m * dt.itemsize
-    for name, cnt in (("left_child", n - 1), ("right_child", n - 1), ("parent", m)):
-        assert np.array_equal(np.frombuffer(data, np.int32, cnt, off), g[name][:cnt]), name
-        off += cnt * 4
+    # the header's coordinate type and configuration (ft_config after 16 bytes)
+    assert np.frombuffer(data, np.int32, 1, 8)[0] == ct
+    off = 64
+    assert np.array_equal(np.frombuffer(data, np.uint64, 2, off), [n, n - 1])
+    off += 16
+    seq = [(a, dt, m, t.arrays[a]) for a in ARRAYS]
+    seq += [("left_child", np.int32, n - 1, t.left), ("right_child", np.int32, n - 1, t.right),
+            ("parent", np.int32, m, t.parent)]
+    seq += [("id", np.uint32, m, None), ("is_ghost", np.int8, m, None),
+            ("orig_idx", np.int32, m, None), ("sorted_keys", np.uint64, n, t.sorted_keys),
+            ("range_first", np.int32, n - 1, t.range_first),
+            ("range_last", np.int32, n - 1, t.range_last)]
+    oracle_leaf = {"id": t.id, "is_ghost": t.ghost, "orig_idx": t.orig}
+    for name, adt, cnt, ref in seq:
+        part = np.frombuffer(data, adt, cnt, off)
+        if ref is not None:
+            assert np.array_equal(part.view(np.uint8), np.ascontiguousarray(ref[:cnt], adt).view(np.uint8)), name
+        else:  # only leaves carry id, is_ghost and orig_idx
+            assert np.array_equal(part[n - 1:m], oracle_leaf[name][n - 1:m].astype(adt)), name
+        off += cnt * np.dtype(adt).itemsize
+    assert off == len(data)
     t2 = ft.TreeSoA.deserialize(data)
     assert t2.num_leaves == n and t2.coord_type == ct
     g2 = t2.to_host()
@@ -263,14 +283,30 @@ def test_serialized_tree_round_trip(ft, orc, ct):
     r2, d2 = ft.self_knn_query(t2, 8)
     assert np.array_equal(r1, r2) and np.array_equal(d1, d2)
     # a damaged header and a truncated buffer are errors
-    import ctypes as C
     bad = bytearray(data)
     bad[0:8] = b"NOTATREE"
     t3 = ft.TreeSoA(1, ct)
-    buf = np.frombuffer(bytes(bad), np.uint8)
-    assert ft.lib().ft_tree_deserialize(None, C.byref(t3.c), buf.ctypes.data, buf.nbytes) == ft.FT_EINVAL
-    short = np.frombuffer(data[:len(data) // 2], np.uint8)
-    assert ft.lib().ft_tree_deserialize(None, C.byref(t3.c), short.ctypes.data, short.nbytes) == ft.FT_ERANGE
+
+    def load(b):
+        buf = np.frombuffer(bytes(b), np.uint8)
+        return ft.lib().ft_tree_deserialize(None, C.byref(t3.c), buf.ctypes.data, buf.nbytes)
+
+    assert load(bad) == ft.FT_EINVAL
+    assert load(data[:len(data) // 2]) == ft.FT_ERANGE
+    # corrupt topology: a child out of range, a child listed twice (a cycle a
+    # walk would never leave), a wrong parent, a leaf range past the leaves
+    o_left = 80 + 6 * m * dt.itemsize
+    o_right = o_left + 4 * (n - 1)
+    o_parent = o_right + 4 * (n - 1)
+    o_rf = o_parent + 4 * m + 4 * m + m + 4 * m + 8 * n
+    o_rl = o_rf + 4 * (n - 1)
+    for at, val in ((o_left + 4 * 5, 2 * n - 1), (o_left + 4 * 7, -3), (o_right + 4 * 9, 0),
+                    (o_right + 4 * 11, int(t.left[11])), (o_parent + 4 * int(t.left[3]), 4),
+                    (o_rf + 4 * 2, -1), (o_rl + 4 * 6, n), (o_parent, 0)):
+        b = bytearray(data)
+        b[at:at + 4] = np.int32(val).tobytes()
+        assert load(b) == ft.FT_EINVAL, (at, val)
+    assert load(data) == ft.FT_OK
 
 
 def test_graph_captured_rebuild_and_query(ft, orc):

@@ -165,6 +165,66 @@ def test_queries_on_two_streams_run_independently(ft, orc):
     assert np.array_equal(c["out"][2].to_host().reshape(c["nq"], k), c["ext_ref"][0])
 
 
+def test_graph_replay_after_stream_buffers_grow(ft, orc):
+    """A seeded self-kNN (re-run buffers) + external kNN (query-order buffers)
+    step captured on a stream at small sizes; then larger eager queries on the
+    same stream grow both buffer sets, and ft_stream_release drops them.  Every
+    replay still equals the brute force: the graph keeps the blocks it was
+    captured with until it is destroyed."""
+    k = 16
+    pts = nfw_points(30000, 41, O.CT_U64)
+    tree, t = make(ft, O.CT_U64, O.SFC_PH, 21, 1, pts)
+    n = 30000
+    rb, db = O.knn_bruteforce(t, k, True)
+    scale = np.where(np.arange(n) % 3 == 0, 0.5, 1.5)  # a third of the seeds too small
+    seed = ft.DeviceArray.from_host(np.floor(db[:, -1].astype(np.float64) * scale).astype(np.uint64))
+    rng = np.random.default_rng(41)
+    nq = 2000
+    q = tuple(rng.integers(0, 1 << 21, nq, dtype=np.uint64) for _ in range(3))
+    qr_b, qd_b = O.knn_bruteforce(t, k, queries=q)
+    dq = tuple(ft.DeviceArray.from_host(a) for a in q)
+    out = [ft.DeviceArray(m * k, np.uint64) for m in (n, n, nq, nq)]
+    s = ft.Stream()
+
+    def step():
+        ft.self_knn_query(tree, k, results=out[0], dists=out[1], seed_bound=seed, max_k=256,
+                          stream=s)
+        ft.knn_query(tree, *dq, k, results=out[2], dists=out[3], stream=s)
+
+    def check(tag):
+        for a in out:
+            a.zero()
+        g.launch()
+        s.synchronize()
+        r, d, qr, qd = (a.to_host() for a in out)
+        assert np.array_equal(r.reshape(n, k), rb) and np.array_equal(d.reshape(n, k), db), tag
+        assert np.array_equal(qr.reshape(nq, k), qr_b) and np.array_equal(qd.reshape(nq, k), qd_b), tag
+
+    step()  # eager: sizes the stream's buffers
+    s.synchronize()
+    with ft.Graph(s) as g:
+        step()
+    check("first replay")
+    # larger eager queries on the same stream: both buffer sets grow
+    big = nfw_points(120000, 42, O.CT_U64)
+    tree2, t2 = make(ft, O.CT_U64, O.SFC_PH, 21, 1, big)
+    r2, d2 = ft.self_knn_query(tree2, k, stream=s)
+    bq = tuple(rng.integers(0, 1 << 21, 90000, dtype=np.uint64) for _ in range(3))
+    ft.knn_query(tree2, *(ft.DeviceArray.from_host(a) for a in bq), k, stream=s)
+    s.synchronize()
+    check("replay after growth")
+    ft._check(ft.lib().ft_stream_release(s.handle))
+    check("replay after ft_stream_release")
+    sample = np.arange(0, 120000, 997)
+    rb2, db2 = O.knn_bruteforce(t2, k, True, leaf_ids=sample.astype(np.int32))
+    assert np.array_equal(d2[sample], db2) and np.array_equal(r2[sample], rb2)
+    del g  # frees the retired blocks; the stream keeps working
+    step()
+    s.synchronize()
+    assert np.array_equal(out[1].to_host().reshape(n, k), db)
+    assert np.array_equal(out[2].to_host().reshape(nq, k), qr_b)
+
+
 def test_external_knn_reference_merge_tier(ft, orc):
     """knn_query k > 32 in reference mode: SortedMergeHeap tie order."""
     pts = ref_points(O.GEN_UNIFORM, 1000, 42, O.CT_U64)
