@@ -89,7 +89,8 @@ EXPORTS = [
     "ft_tree_serialized_size", "ft_tree_serialize", "ft_tree_deserialize",
     "ft_scratch_create", "ft_scratch_destroy", "ft_scratch_set_profiling", "ft_scratch_stage_ms",
     "ft_scratch_set_radix_bits",
-    "ft_compute_bbox", "ft_sfc_encode", "ft_build_tree", "ft_build_bvh", "ft_self_knn_query",
+    "ft_compute_bbox", "ft_sfc_encode", "ft_build_tree", "ft_fill_particle_coords",
+    "ft_build_bvh", "ft_self_knn_query",
     "ft_self_knn_query_subset", "ft_knn_query", "ft_range_query", "ft_range_query_csr",
     "ft_mlp_num_params", "ft_mlp_create", "ft_mlp_destroy", "ft_mlp_set", "ft_mlp_get",
     "ft_knn_features", "ft_mlp_forward", "ft_mlp_grad", "ft_mlp_train_step",
@@ -144,6 +145,8 @@ def lib():
         L.ft_sfc_encode.argtypes = [vp, C.c_int32, C.POINTER(Config), vp, vp, vp, sz, vp, vp]
         L.ft_build_tree.argtypes = [vp, C.POINTER(_Tree), vp, vp, vp, vp, vp, vp, vp]
         L.ft_build_bvh.argtypes = [vp, C.POINTER(_Tree), vp, vp, vp, vp, vp, vp, sz, vp, vp]
+        L.ft_fill_particle_coords.argtypes = [vp, C.c_int32, vp, vp, vp, sz, C.c_int32, i32, vp, vp,
+                                              vp, vp]
         L.ft_self_knn_query.argtypes = [vp, C.POINTER(_Tree), i32, vp, vp, i32,
                                         C.POINTER(QueryOpts)]
         L.ft_self_knn_query_subset.argtypes = [vp, C.POINTER(_Tree), vp, i32, i32, vp, vp, i32,
@@ -484,6 +487,36 @@ class BvhScratch:
 # ---------------------------------------------------------------------------
 # Build
 # ---------------------------------------------------------------------------
+def fill_particle_coords(pos_x, pos_y, pos_z, coord_type: int, bits_per_dim: int = 21,
+                         stream=None):
+    """fill_particle_coords (test/benchmark/BenchmarkUtils.hpp:164-205): snapshot
+    positions (float32 as the HDF5 read gives them, or float64; numpy arrays or
+    DeviceArrays) to tree coordinates of `coord_type`.  Integer types: one cube
+    over all three axes, padded by 1.0001, then float_to_int_rep with
+    `bits_per_dim` bits.  Returns (x, y, z, box_min, box_size), the arrays in the
+    inputs' kind (host or device)."""
+    dev = isinstance(pos_x, DeviceArray)
+    if dev:
+        in_dt = pos_x.dtype
+        n = pos_x.n
+    else:
+        in_dt = np.dtype(np.float64 if np.asarray(pos_x).dtype == np.float64 else np.float32)
+        pos_x, pos_y, pos_z = (np.ascontiguousarray(a, in_dt) for a in (pos_x, pos_y, pos_z))
+        n = len(pos_x)
+    in_type = COORD_F64 if in_dt == np.float64 else COORD_F32
+    dt = COORD_DTYPES[coord_type]
+    if dev:
+        outs = [DeviceArray(n, dt) for _ in range(3)]
+    else:
+        outs = [np.zeros(n, dt) for _ in range(3)]
+    box = np.zeros(2, np.float64)
+    _check(lib().ft_fill_particle_coords(_sh(stream), in_type, _ptr(pos_x), _ptr(pos_y),
+                                         _ptr(pos_z), int(n), int(coord_type), int(bits_per_dim),
+                                         _ptr(outs[0]), _ptr(outs[1]), _ptr(outs[2]),
+                                         box.ctypes.data))
+    return outs[0], outs[1], outs[2], float(box[0]), float(box[1])
+
+
 def compute_bbox(pos_x, pos_y, pos_z, n=None, coord_type=None, stream=None):
     """compute_bbox (src/hlbvh.hpp:462-578) -> (min_x, max_x, min_y, max_y, min_z, max_z)."""
     ct = coord_type if coord_type is not None else coord_type_of(pos_x.dtype)

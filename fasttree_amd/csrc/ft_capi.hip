@@ -45,6 +45,9 @@ int range_entry(hipStream_t s, const ft_tree *tree, const void *qx, const void *
 int scan_entry(hipStream_t s, const int32_t *counts, int nq, int64_t *row_ptr);
 int release_stream_bufs(hipStream_t s);
 void graph_attach(hipStream_t s, void *exec);
+size_t snapshot_work_doubles();
+int snapshot_entry(hipStream_t s, int in_type, const void *px, const void *py, const void *pz,
+                   size_t n, int coord_type, int bits, void *x, void *y, void *z, double *work);
 void graph_detach(void *exec);
 int range_csr_entry(hipStream_t s, const ft_tree *tree, const void *qx, const void *qy,
                     const void *qz, const void *rmin, const void *rmax, int nq, int64_t *row_ptr,
@@ -631,6 +634,56 @@ int ft_build_tree(ft_stream s, ft_tree *tree, const uint64_t *sorted_keys, const
                                                (const int8_t *)g.dev, (const int32_t *)o.dev));
     }
     for (Staged *st : {&k, &x, &y, &z, &i, &g, &o}) unstage(*st, hs, false);
+    return rc;
+}
+
+int ft_fill_particle_coords(ft_stream s, int32_t in_type, const void *px, const void *py,
+                            const void *pz, size_t n, int32_t coord_type, int bits_per_dim,
+                            void *x, void *y, void *z, double *box_out) {
+    if (in_type != FT_COORD_F32 && in_type != FT_COORD_F64)
+        return fail(FT_EINVAL, "snapshot positions must be float or double");
+    if (coord_type < FT_COORD_F32 || coord_type > FT_COORD_U64)
+        return fail(FT_EINVAL, "unknown coordinate type");
+    const bool integer = coord_type == FT_COORD_U32 || coord_type == FT_COORD_U64;
+    const int max_bits = coord_type == FT_COORD_U32 ? 32 : 52;
+    if (integer && (bits_per_dim < 1 || bits_per_dim > max_bits))
+        return fail(FT_EINVAL, "bits_per_dim must be in [1, 32] for uint32 and [1, 52] for "
+                               "uint64 coordinates");
+    const hipStream_t hs = as_stream(s);
+    const double unit_box[2] = {0.0, 1.0};
+    if ((n == 0 || !integer) && box_out) {  // the reference's box for these cases
+        if (!is_device_accessible(box_out)) {
+            box_out[0] = unit_box[0];
+            box_out[1] = unit_box[1];
+        } else {
+            FT_HIP(hipMemcpyAsync(box_out, unit_box, 16, hipMemcpyDefault, hs));
+            FT_HIP(hipStreamSynchronize(hs));  // unit_box lives on this stack frame
+        }
+    }
+    if (n == 0) return FT_OK;
+    if (!px || !py || !pz || !x || !y || !z) return fail(FT_EINVAL, "null coordinate array");
+    const size_t ib = n * (in_type == FT_COORD_F32 ? 4 : 8), ob = n * coord_size(coord_type);
+    Staged sx, sy, sz, ox, oy, oz;
+    FT_TRY(stage_in(px, ib, hs, sx));
+    FT_TRY(stage_in(py, ib, hs, sy));
+    FT_TRY(stage_in(pz, ib, hs, sz));
+    FT_TRY(stage_out(x, ob, hs, ox));
+    FT_TRY(stage_out(y, ob, hs, oy));
+    FT_TRY(stage_out(z, ob, hs, oz));
+    double *work = nullptr;
+    if (hipMallocAsync((void **)&work, snapshot_work_doubles() * 8, hs) != hipSuccess)
+        return fail(FT_ENOMEM, "fill_particle_coords: work buffer allocation failed");
+    int rc = snapshot_entry(hs, in_type, sx.dev, sy.dev, sz.dev, n, coord_type, bits_per_dim,
+                            ox.dev, oy.dev, oz.dev, work);
+    if (rc == FT_OK && integer && box_out) {
+        if (hipMemcpyAsync(box_out, work, 16, hipMemcpyDefault, hs) != hipSuccess ||
+            (!is_device_accessible(box_out) && hipStreamSynchronize(hs) != hipSuccess))
+            rc = fail(FT_EHIP, "fill_particle_coords: copying the box failed");
+    }
+    (void)hipFreeAsync(work, hs);
+    if (rc == FT_OK) rc = unstage(ox, hs, true);
+    if (rc == FT_OK) rc = unstage(oy, hs, true);
+    if (rc == FT_OK) rc = unstage(oz, hs, true);
     return rc;
 }
 

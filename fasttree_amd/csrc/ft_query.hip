@@ -935,8 +935,7 @@ static inline int bit_length(uint64_t v) {
 }
 
 // Flags -> ascending slot list, count on the device.
-static int compact_flags(hipStream_t s, int n) {
-    RerunBuf &g_rerun = stream_bufs(s).rr;
+static int compact_flags(hipStream_t s, const RerunBuf &g_rerun, int n) {
     const unsigned nt = blocks_for((size_t)n, CP_TILE);
     hipLaunchKernelGGL(flag_count_kernel, dim3(nt), dim3(256), 0, s,
                        (const uint8_t *)g_rerun.flags, n, g_rerun.tiles);
@@ -963,9 +962,8 @@ static int compact_flags(hipStream_t s, int n) {
 // unseeded, lexicographic top-k (registers for k <= 32, LDS heap above).
 template <typename T, int MK>
 static int launch_exact_listed(hipStream_t s, const DTree<T> &dt, const QSrc<T> &qs, int nq,
-                               int k, int excl, const uint32_t *order, uint64_t *res,
-                               typename CoordTraits<T>::D *d) {
-    RerunBuf &g_rerun = stream_bufs(s).rr;
+                               int k, int excl, const uint32_t *order, const RerunBuf &g_rerun,
+                               uint64_t *res, typename CoordTraits<T>::D *d) {
     const QueryMap qm{order, g_rerun.list, g_rerun.count, nullptr};
     if (k <= 32) {
         const unsigned g = std::min<unsigned>(blocks_for(nq, LANE_BLOCK), 2048u);
@@ -1020,10 +1018,13 @@ static int run_canonical(hipStream_t s, const DTree<T> &dt, const QSrc<T> &qs, i
                          int excl, int algo, int bucket, int topk, int window,
                          const uint32_t *order,
                          const typename CoordTraits<T>::D *seed, uint64_t *res,
-                         typename CoordTraits<T>::D *d, uint64_t *stats) {
+                         typename CoordTraits<T>::D *d, uint64_t *stats, const RerunBuf *rr) {
     constexpr bool INTD = !CoordTraits<T>::fp;
-    FT_TRY(rerun_ensure(s, (size_t)nq));
-    RerunBuf &g_rerun = stream_bufs(s).rr;
+    if (!rr) {  // the stream's own re-run buffers, grown to nq
+        FT_TRY(rerun_ensure(s, (size_t)nq));
+        rr = &stream_bufs(s).rr;
+    }
+    const RerunBuf &g_rerun = *rr;
     hipLaunchKernelGGL(fill_u8_kernel, dim3(std::min<unsigned>(blocks_for(nq, 256), 4096u)),
                        dim3(256), 0, s, g_rerun.flags, (size_t)nq);
     const QueryMap qm{order, nullptr, nullptr, g_rerun.flags};
@@ -1073,13 +1074,13 @@ static int run_canonical(hipStream_t s, const DTree<T> &dt, const QSrc<T> &qs, i
                                qs, nq, k, excl, lanes, qm, res, d, stats);
     }
     FT_CHECK_LAUNCH();
-    FT_TRY(compact_flags(s, nq));
+    FT_TRY(compact_flags(s, g_rerun, nq));
     if (stats) {
         hipLaunchKernelGGL(add_count_kernel, dim3(1), dim3(1), 0, s, stats,
                            (const uint32_t *)g_rerun.count);
         FT_CHECK_LAUNCH();
     }
-    return launch_exact_listed<T, MK>(s, dt, qs, nq, k, excl, order, res, d);
+    return launch_exact_listed<T, MK>(s, dt, qs, nq, k, excl, order, g_rerun, res, d);
 }
 
 template <typename T>
@@ -1116,7 +1117,7 @@ static int query_order(hipStream_t s, const DTree<T> &dt, const QSrc<T> &qs, int
 template <typename T>
 static int knn_dispatch(hipStream_t s, const ft_tree *tree, const QSrc<T> &qs, int nq, int k,
                         int excl, uint64_t *res, void *dists_v, const ft_query_opts *opts,
-                        bool external) {
+                        bool external, const RerunBuf *rr = nullptr) {
     using D = typename CoordTraits<T>::D;
     D *dists = (D *)dists_v;
     const DTree<T> dt = make_dtree<T>(*tree);
@@ -1171,10 +1172,10 @@ static int knn_dispatch(hipStream_t s, const ft_tree *tree, const QSrc<T> &qs, i
     switch (metric_kind<T>(dt.m)) {
     case MK_WRAP:
         return run_canonical<T, MK_WRAP>(s, dt, qs, nq, k, excl, algo, bucket, o.topk, window,
-                                         order, seed, res, dists, stats);
+                                         order, seed, res, dists, stats, rr);
     case MK_OPEN:
         return run_canonical<T, MK_OPEN>(s, dt, qs, nq, k, excl, algo, bucket, o.topk, window,
-                                         order, seed, res, dists, stats);
+                                         order, seed, res, dists, stats, rr);
     default:
         return fail(FT_EINVAL, "unsupported metric: periodic integer coordinates need B <= 31");
     }
@@ -1373,7 +1374,7 @@ static int range_csr_t(hipStream_t s, const ft_tree &tree, const T *qx, const T 
     if (rc == FT_OK) {
         hipLaunchKernelGGL(csr_overflow_flags_kernel, dim3(std::min<unsigned>(blocks_for(nq, 256), 4096u)),
                            dim3(256), 0, s, order, (const int32_t *)counts, nq, g_rerun.flags);
-        rc = compact_flags(s, nq);
+        rc = compact_flags(s, g_rerun, nq);
     }
     if (rc == FT_OK &&
         (hipMemcpyAsync(&n_over, g_rerun.count, 4, hipMemcpyDeviceToHost, s) != hipSuccess ||

@@ -259,6 +259,35 @@ inline void range_query(hipStream_t q, const TreeSoA &tree, const coord_t *qx, c
                                  results, result_counts, max_results_per_query));
 }
 
+// The snapshot conversion of the reference's benchmark utilities
+// (test/benchmark/BenchmarkUtils.hpp:82-87, 164-205), run on the GPU (null
+// stream): ParticleData holds what the HDF5 read of PartType0 gives; the HDF5
+// read itself is not part of this library.
+namespace bench_utils {
+struct ParticleData {
+    std::vector<float> pos_x, pos_y, pos_z;
+    std::vector<float> vel_x, vel_y, vel_z;
+    std::vector<float> masses;
+    std::size_t count = 0;
+};
+
+template <typename T>
+inline void fill_particle_coords(const ParticleData &data, particles<T> &p, double &out_box_min,
+                                 double &out_box_size) {
+    const std::size_t n = data.count;
+    p.pos_x.resize(n);
+    p.pos_y.resize(n);
+    p.pos_z.resize(n);
+    double box[2] = {0.0, 1.0};
+    detail::check(ft_fill_particle_coords(nullptr, FT_COORD_F32, data.pos_x.data(),
+                                          data.pos_y.data(), data.pos_z.data(), n,
+                                          detail::coord_code<T>(), FASTTREE_BITS, p.pos_x.data(),
+                                          p.pos_y.data(), p.pos_z.data(), box));
+    out_box_min = box[0];
+    out_box_size = box[1];
+}
+}  // namespace bench_utils
+
 static_assert(sizeof(std::size_t) == sizeof(std::uint64_t), "results are 64-bit indices");
 
 }  // namespace fasttree

@@ -223,6 +223,19 @@ int ft_sfc_encode(ft_stream s, int32_t coord_type, const ft_config *cfg, const v
 int ft_build_tree(ft_stream s, ft_tree *tree, const uint64_t *sorted_keys, const void *sx,
                   const void *sy, const void *sz, const uint32_t *sid, const int8_t *sghost,
                   const int32_t *sorig);
+/* fill_particle_coords (test/benchmark/BenchmarkUtils.hpp:164-205): snapshot positions   */
+/* (in_type FT_COORD_F32, as the HDF5 PartType0/Coordinates read gives them, or          */
+/* FT_COORD_F64) to tree coordinates x, y, z of coord_type, n of each.  Integer           */
+/* coord_type: one cube over all three axes, box_min = the smallest coordinate,           */
+/* box_size = (largest - box_min) * 1.0001 (1.0 when all are equal), then each value is   */
+/* float_to_int_rep(v, box_min, 1 / box_size) (src/sfc.peano_hilbert.hpp:495-503) with    */
+/* bits_per_dim bits ([1, 32] for uint32, [1, 52] for uint64).  FP coord_type: a plain    */
+/* cast, box (0, 1).  box_out (nullable, host or device): double[2] = {box_min,           */
+/* box_size}.  Host arrays are staged.  Positions must be finite; the sign of a zero      */
+/* box_min may differ from the reference's, which keeps the first of equal minima.        */
+int ft_fill_particle_coords(ft_stream s, int32_t in_type, const void *px, const void *py,
+                            const void *pz, size_t n, int32_t coord_type, int bits_per_dim,
+                            void *x, void *y, void *z, double *box_out);
 /* build_bvh (src/hlbvh.hpp:2165-2302).  id / is_ghost may be NULL (iota / zeros).  bbox6 */
 /* (host coord_t[6]) and keys (n unsorted SFC keys) are optional precomputed inputs.     */
 /* scratch may be NULL (a temporary one is used, as the reference's 3-arg overload).     */

@@ -214,6 +214,24 @@ def knn_reference(t: Tree, k, exclude_self=True, queries=None, leaf_ids=None, ki
     return res.reshape(nq, k), d.reshape(nq, k)
 
 
+def knn_reference_grouped(t: Tree, k, exclude_self=True, leaf_ids=None, width=8,
+                          visits=False):
+    """The reference's grouped self-kNN as a CPU runs it: `width` lanes in
+    lockstep (native vector width: 8 for 64-bit, 16 for 32-bit coordinates)."""
+    nq = t.n if leaf_ids is None else len(leaf_ids)
+    lid = None if leaf_ids is None else np.ascontiguousarray(leaf_ids, np.int32)
+    res = np.zeros(nq * k, np.uint64)
+    d = np.zeros(nq * k, dist_dtype(t.ct))
+    cnt = np.zeros(nq, np.uint64)
+    s = t.cstruct()
+    rc = lib().or_knn_reference_grouped(C.byref(s), _p(lid), nq, k, int(bool(exclude_self)),
+                                        width, _p(res), _p(d), _p(cnt))
+    assert rc == 0
+    if visits:
+        return res.reshape(nq, k), d.reshape(nq, k), cnt
+    return res.reshape(nq, k), d.reshape(nq, k)
+
+
 def range_reference(t: Tree, qx, qy, qz, rmin, rmax, max_res):
     dt = COORD_DTYPES[t.ct]
     qx, qy, qz, rmin, rmax = (np.ascontiguousarray(a, dt) for a in (qx, qy, qz, rmin, rmax))
@@ -272,3 +290,19 @@ def gen_reference(kind, n, seed, ct, bits=21, box_min=0.0, box_size=100.0):
     lib().or_gen_reference(kind, C.c_size_t(n), C.c_uint32(seed), C.c_double(box_min),
                            C.c_double(box_size), ct, bits, _p(x), _p(y), _p(z))
     return x, y, z
+
+
+def fill_particle_coords(px, py, pz, ct, bits=21):
+    """fill_particle_coords (test/benchmark/BenchmarkUtils.hpp:164-205): float32 or
+    float64 positions -> coordinates of type ct; returns (x, y, z, box_min, box_size)."""
+    in_dt = np.float64 if np.asarray(px).dtype == np.float64 else np.float32
+    px, py, pz = (np.ascontiguousarray(a, in_dt) for a in (px, py, pz))
+    n = len(px)
+    dt = COORD_DTYPES[ct]
+    x, y, z = np.zeros(n, dt), np.zeros(n, dt), np.zeros(n, dt)
+    box = np.zeros(2, np.float64)
+    rc = lib().or_fill_particle_coords(CT_F64 if in_dt == np.float64 else CT_F32, _p(px), _p(py),
+                                       _p(pz), C.c_size_t(n), ct, bits, _p(x), _p(y), _p(z),
+                                       _p(box))
+    assert rc == 0
+    return x, y, z, float(box[0]), float(box[1])

@@ -540,6 +540,11 @@ template <typename D> struct MaxHeap {
     std::vector<int> idx;
     int count = 0;
     explicit MaxHeap(int cap) : dist(cap, std::numeric_limits<D>::max()), idx(cap, -1) {}
+    void reset() {
+        count = 0;
+        std::fill(dist.begin(), dist.end(), std::numeric_limits<D>::max());
+        std::fill(idx.begin(), idx.end(), -1);
+    }
     bool should_prune(D d, int k) const { return count >= k && d >= dist[0]; }
     void swp(int a, int b) {
         std::swap(dist[a], dist[b]);
@@ -732,6 +737,113 @@ static void reference_knn_t(const or_tree *t, const T *qx, const T *qy, const T 
     }
 }
 
+// The reference's grouped self-kNN (src/hlbvh.hpp:1520-1647 with the register
+// heap, :1690-1818 with the local-memory heap; the same heap algorithm) as its
+// CPU backend runs it: W consecutive query slots in lockstep, one step per
+// loop trip.  Each lane pops its own stack and tests its own node; the group
+// skips the step only when no active lane still needs its node
+// (any_of_group); otherwise every lane in the loop processes its node -- a
+// leaf is pushed only by an active lane that did not prune it, but an
+// internal node is expanded even by a lane that pruned it.  Lanes past nq
+// shadow leaf rank 0 and never count for the group.  Lanes whose stack
+// emptied have left the loop.  W = the device's native vector width
+// (:1828-1839): 8 for 64-bit and 16 for 32-bit coordinates on a CPU.
+template <typename T>
+static void reference_knn_grouped_t(const or_tree *t, const int32_t *leaf_ids, int nq, int k,
+                                    int excl, int W, uint64_t *res, void *dists_v,
+                                    uint64_t *counters) {
+    using D = typename DistOf<T>::type;
+    TV<T> tv(t);
+    D *dists = (D *)dists_v;
+    const int n = tv.n;
+    const int off = n - 1;
+    const int groups = (nq + W - 1) / W;
+#pragma omp parallel
+    {
+        std::vector<MaxHeap<D>> heaps;
+        heaps.reserve(W);
+        for (int l = 0; l < W; ++l) heaps.emplace_back(std::max(k, 1));
+        std::vector<int> stack((size_t)W * 64), sp(W), node(W), self_node(W);
+        std::vector<uint8_t> active(W), pruned(W), in_loop(W);
+        std::vector<T> px(W), py(W), pz(W);
+        std::vector<D> d2(W);
+        std::vector<uint64_t> visits(W);
+#pragma omp for schedule(dynamic, 4)
+        for (int g = 0; g < groups; ++g) {
+            for (int l = 0; l < W; ++l) {
+                const int q = g * W + l;
+                active[l] = q < nq;
+                const int r = active[l] ? (leaf_ids ? leaf_ids[q] : q) : 0;
+                self_node[l] = r + off;
+                px[l] = tv.mn[0][r + off];
+                py[l] = tv.mn[1][r + off];
+                pz[l] = tv.mn[2][r + off];
+                heaps[l].reset();
+                sp[l] = 0;
+                stack[(size_t)l * 64 + sp[l]++] = 0;
+                visits[l] = 0;
+            }
+            for (;;) {
+                bool any_in = false, anyone = false;
+                for (int l = 0; l < W; ++l) {
+                    in_loop[l] = sp[l] > 0;
+                    if (!in_loop[l]) continue;
+                    any_in = true;
+                    node[l] = stack[(size_t)l * 64 + --sp[l]];
+                    visits[l]++;
+                    d2[l] = tv.d2(px[l], py[l], pz[l], node[l]);
+                    pruned[l] = active[l] && heaps[l].should_prune(d2[l], k);
+                    anyone |= active[l] && !pruned[l];
+                }
+                if (!any_in) break;
+                if (!anyone) continue;
+                for (int l = 0; l < W; ++l) {
+                    if (!in_loop[l]) continue;
+                    const int nd = node[l];
+                    if (nd >= n - 1 && n > 1) {
+                        if (active[l] && !pruned[l] && !(excl && nd == self_node[l]))
+                            heaps[l].push(d2[l], (int)tv.ret_index(nd), k);
+                    } else if (n == 1) {
+                        if (active[l] && !pruned[l] && !excl)
+                            heaps[l].push(d2[l], (int)tv.ret_index(0), k);
+                    } else {
+                        const int lc = tv.left[nd], rc = tv.right[nd];
+                        const D ld = tv.d2(px[l], py[l], pz[l], lc);
+                        const D rd = tv.d2(px[l], py[l], pz[l], rc);
+                        int *st = &stack[(size_t)l * 64];
+                        if (sp[l] < 64 - 2) {
+                            if (ld <= rd) {
+                                st[sp[l]++] = rc;
+                                st[sp[l]++] = lc;
+                            } else {
+                                st[sp[l]++] = lc;
+                                st[sp[l]++] = rc;
+                            }
+                        }
+                    }
+                }
+            }
+            for (int l = 0; l < W; ++l) {
+                const int q = g * W + l;
+                if (!active[l]) continue;
+                MaxHeap<D> &h = heaps[l];
+                h.sort_ascending();
+                const size_t o = (size_t)q * k;
+                for (int i = 0; i < k; ++i) {
+                    if (i < h.count && h.idx[i] >= 0) {
+                        res[o + i] = (uint64_t)h.idx[i];
+                        dists[o + i] = h.dist[i];
+                    } else {
+                        res[o + i] = ~0ull;
+                        dists[o + i] = sentinel_dist<D>();
+                    }
+                }
+                if (counters) counters[q] = visits[l];
+            }
+        }
+    }
+}
+
 // range_query (src/hlbvh.hpp:2046-2119): DFS pushing right then left, so leaves
 // come out in ascending rank; accept rm2 <= d2 <= RM2; store while count < max;
 // count = all hits.  Radii: FP (double)r squared; integer (r >> _DIST_SHIFT)^2.
@@ -786,14 +898,65 @@ static void reference_range_t(const or_tree *t, const T *qx, const T *qy, const 
 // Reference test generators (test/test_self_knn.cpp:271-368, test/main.cpp:84-101)
 // with phys_pos_to_int = float_to_int_rep(phys, min, 1/size) for integer coords
 // (src/sfc.peano_hilbert.hpp:505-513) and a plain cast for FP coords.
+// float_to_int_rep(val, min_val, inv_dx) (src/sfc.peano_hilbert.hpp:495-503):
+// nx = sycl::clamp((val - min_val) * inv_dx, 0, 0x1.fffffffffffffp+0), i.e.
+// fmin(fmax(., 0), upper), then quantize_coord.
+static uint64_t float_to_int_rep(double val, double min_val, double inv_dx, int B) {
+    const double upper = 0x1.fffffffffffffp+0;
+    double nx = std::fmin(std::fmax((val - min_val) * inv_dx, 0.0), upper);
+    return ph_quantise(nx, B);
+}
+
 template <typename T> static T phys_to_coord(double phys, double bmin, double bsize, int B) {
     if constexpr (is_fp_v<T>) {
         return (T)phys;
     } else {
-        const double upper = 0x1.fffffffffffffp+0;
         double inv = 1.0 / bsize;
-        double nx = std::fmin(std::fmax((phys - bmin) * inv, 0.0), upper);
-        return (T)ph_quantise(nx, B);
+        return (T)float_to_int_rep(phys, bmin, inv, B);
+    }
+}
+
+// fill_particle_coords (test/benchmark/BenchmarkUtils.hpp:164-205).  Integer
+// coordinates: per-axis std::min / std::max over the positions promoted to
+// double, box_min = min of the three minima, box_size = (max of the maxima -
+// box_min) * 1.0001, or 1.0 when that difference is not positive; every value
+// through float_to_int_rep(v, box_min, 1 / box_size).  FP coordinates: a
+// static_cast, box (0, 1).
+template <typename F, typename T>
+static void fill_coords_t(const F *px, const F *py, const F *pz, size_t n, int B, T *x, T *y,
+                          T *z, double *box) {
+    box[0] = 0.0;
+    box[1] = 1.0;
+    if constexpr (is_fp_v<T>) {
+        for (size_t i = 0; i < n; ++i) {
+            x[i] = static_cast<T>(px[i]);
+            y[i] = static_cast<T>(py[i]);
+            z[i] = static_cast<T>(pz[i]);
+        }
+    } else {
+        if (n == 0) return;
+        double min_x = px[0], max_x = px[0];
+        double min_y = py[0], max_y = py[0];
+        double min_z = pz[0], max_z = pz[0];
+        for (size_t i = 1; i < n; ++i) {
+            min_x = std::min(min_x, static_cast<double>(px[i]));
+            max_x = std::max(max_x, static_cast<double>(px[i]));
+            min_y = std::min(min_y, static_cast<double>(py[i]));
+            max_y = std::max(max_y, static_cast<double>(py[i]));
+            min_z = std::min(min_z, static_cast<double>(pz[i]));
+            max_z = std::max(max_z, static_cast<double>(pz[i]));
+        }
+        const double box_min = std::min({min_x, min_y, min_z});
+        const double max_val = std::max({max_x, max_y, max_z});
+        const double box_size = (max_val > box_min) ? (max_val - box_min) * 1.0001 : 1.0;
+        const double inv_dx = 1.0 / box_size;
+        for (size_t i = 0; i < n; ++i) {
+            x[i] = (T)float_to_int_rep(px[i], box_min, inv_dx, B);
+            y[i] = (T)float_to_int_rep(py[i], box_min, inv_dx, B);
+            z[i] = (T)float_to_int_rep(pz[i], box_min, inv_dx, B);
+        }
+        box[0] = box_min;
+        box[1] = box_size;
     }
 }
 
@@ -979,6 +1142,15 @@ int or_knn_reference(const or_tree *t, const void *qx, const void *qy, const voi
     return 0;
 }
 
+int or_knn_reference_grouped(const or_tree *t, const int32_t *leaf_ids, int nq, int k,
+                             int exclude_self, int width, uint64_t *res, void *dists,
+                             uint64_t *visits) {
+    if (width < 1) return 1;
+    OR_DISPATCH(t->ct, reference_knn_grouped_t<T>(t, leaf_ids, nq, k, exclude_self, width, res,
+                                                  dists, visits));
+    return 0;
+}
+
 int or_range_reference(const or_tree *t, const void *qx, const void *qy, const void *qz,
                        const void *rmin, const void *rmax, int nq, int32_t *res, int32_t *counts,
                        int max_res, int64_t *row_ptr, int32_t *cols) {
@@ -1055,6 +1227,20 @@ int or_heap_u64(int kind, int k, int cap, const uint64_t *d, const int32_t *idx,
 int or_gen_reference(int kind, size_t n, uint32_t seed, double bmin, double bsize, int ct, int B,
                      void *x, void *y, void *z) {
     OR_DISPATCH(ct, gen_reference_t<T>(kind, n, seed, bmin, bsize, B, (T *)x, (T *)y, (T *)z));
+    return 0;
+}
+
+int or_fill_particle_coords(int in_type, const void *px, const void *py, const void *pz,
+                            size_t n, int ct, int B, void *x, void *y, void *z, double *box) {
+    if (in_type == CT_F32) {
+        using F = float;
+        OR_DISPATCH(ct, fill_coords_t<F, T>((const F *)px, (const F *)py, (const F *)pz, n, B,
+                                            (T *)x, (T *)y, (T *)z, box));
+    } else {
+        using F = double;
+        OR_DISPATCH(ct, fill_coords_t<F, T>((const F *)px, (const F *)py, (const F *)pz, n, B,
+                                            (T *)x, (T *)y, (T *)z, box));
+    }
     return 0;
 }
 

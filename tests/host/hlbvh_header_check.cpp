@@ -37,6 +37,15 @@ int main() {
     void (*kq)(hipStream_t, const TreeSoA &, const coord_t *, const coord_t *, const coord_t *,
                int, int, std::size_t *, dist_t *) = &knn_query<256>;
     (void)sk; (void)ss; (void)kq; (void)p;
+    // An empty snapshot converts to nothing with the reference's box (0, 1).
+    bench_utils::ParticleData empty;
+    particles<coord_t> pe;
+    double box_min = -1.0, box_size = -1.0;
+    bench_utils::fill_particle_coords(empty, pe, box_min, box_size);
+    if (box_min != 0.0 || box_size != 1.0 || !pe.pos_x.empty()) {
+        std::printf("empty snapshot: box (%g, %g)\n", box_min, box_size);
+        ++failures;
+    }
     std::printf("failures: %d\n", failures);
     return failures ? 1 : 0;
 }

@@ -225,6 +225,28 @@ def test_graph_replay_after_stream_buffers_grow(ft, orc):
     assert np.array_equal(out[2].to_host().reshape(nq, k), qr_b)
 
 
+def test_reference_mode_tie_picks_by_hand(ft, orc):
+    """Reference mode on the GPU against literal arrays derived by hand from the
+    reference source (the derivations are in tests/test_oracle.py): the
+    per-query walk with the max-heap tier (8 points on a line, query x = 7,
+    k = 4: [4, 3, 2, 5], canonical [3, 4, 2, 5]) and the SortedMergeHeap tier
+    of knn_query k > 32 (34 points, query x = 50, k = 33)."""
+    import test_oracle as TO
+    x, y, z = TO._line_tree(range(0, 16, 2))
+    tree, _ = make(ft, O.CT_U64, O.SFC_MORTON, 21, 0, (x, y, z))
+    q = (np.array([7], np.uint64), np.zeros(1, np.uint64), np.zeros(1, np.uint64))
+    r, d = ft.knn_query(tree, *q, 4, mode=ft.MODE_REFERENCE)
+    assert list(r[0]) == [4, 3, 2, 5] and list(d[0]) == [1, 1, 9, 9]
+    r, d = ft.knn_query(tree, *q, 4)
+    assert list(r[0]) == [3, 4, 2, 5] and list(d[0]) == [1, 1, 9, 9]
+    pts, q, _ = TO.merge_tier_case()
+    tree, _ = make(ft, O.CT_U64, O.SFC_MORTON, 21, 0, pts)
+    r, d = ft.knn_query(tree, *q, 33, mode=ft.MODE_REFERENCE, max_k=256)
+    assert list(r[0]) == TO.MERGE_TIER_IDX
+    r, d = ft.knn_query(tree, *q, 33)  # canonical: ties by index
+    assert np.array_equal(np.lexsort((r[0], d[0])), np.arange(33))
+
+
 def test_external_knn_reference_merge_tier(ft, orc):
     """knn_query k > 32 in reference mode: SortedMergeHeap tie order."""
     pts = ref_points(O.GEN_UNIFORM, 1000, 42, O.CT_U64)

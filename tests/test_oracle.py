@@ -343,6 +343,141 @@ def test_sorted_merge_heap_kat(orc):
     assert np.array_equal(od[:48], exp)
 
 
+# ------------------------------------------ reference tie picks, by hand -------
+# Expected arrays below are derived by hand from the reference source (sift and
+# heapsort rules src/maxheap.hpp:94-134, the bitonic network :193-215 and
+# merge_batch :263-323, the DFS child order src/hlbvh.hpp:1618-1626), in the
+# comments; the oracle restatement must reproduce them.
+S = np.iinfo(np.uint64).max
+
+
+def test_max_heap_tie_picks_by_hand(orc):
+    """k = 4, pushes (5,10) (5,11) (3,12) (5,13) (2,14) (5,15) with the prune check.
+    sift_up stops on <=, so the first four stay in arrival order: [5/10, 5/11,
+    3/12, 5/13].  (2,14) replaces the root; sift_down takes the larger child,
+    the left one on a tie: 2/14 swaps with 5/11 (slot 1), then with 5/13 (slot
+    3): [5/11, 5/13, 3/12, 2/14].  (5,15) is pruned (5 >= root 5).  Heapsort:
+    swap root and last, sift down over the rest, three times: [2/14, 3/12, 5/13,
+    5/11].  The (d, index) order would keep 10 and 11 instead."""
+    od, oi, cnt = O.heap_u64(0, 4, 4, [5, 5, 3, 5, 2, 5], [10, 11, 12, 13, 14, 15])
+    assert cnt == 4
+    assert list(od) == [2, 3, 5, 5] and list(oi) == [14, 12, 13, 11]
+
+
+def test_sorted_merge_tie_order_by_hand(orc):
+    """merge_batch with one candidate per batch, k = 2, cap = 64: batch 1 stages
+    (7,2) (the network keeps a lone entry at slot 0 of every ascending block),
+    count = min(32, 2) = 2.  Batch 2 appends (7,1) at slot 2; in the 64-wide
+    sort, k = 2 / j = 1 is descending at slots 2-3, so (7,1) moves to slot 3,
+    then k = 4 / j = 2 ascending moves it to slot 1 (a sentinel sat there); the
+    tie (7,2)/(7,1) at slots 0-1 is only ever compared ascending, so it stays:
+    [(7,2), (7,1)], arrival order, where (d, index) order is [1, 2].  One batch
+    of lanes [(5,3), (5,1), sentinels], k = 2, cap = 32: slots 0 and 1 are
+    compared ascending at every stage, so lane order survives: [3, 1]."""
+    sent = [S] * 31
+    d = [7] + sent + [7] + sent
+    i = [2] + [-1] * 31 + [1] + [-1] * 31
+    od, oi, cnt = O.heap_u64(1, 2, 64, d, i)
+    assert cnt == 2 and list(od[:2]) == [7, 7] and list(oi[:2]) == [2, 1]
+    od, oi, cnt = O.heap_u64(1, 2, 32, [5, 5] + [S] * 30, [3, 1] + [-1] * 30)
+    assert cnt == 2 and list(oi[:2]) == [3, 1]
+
+
+def _line_tree(xs):
+    x = np.array(xs, np.uint64)
+    zero = np.zeros(len(xs), np.uint64)
+    return x, zero, zero.copy()
+
+
+def test_reference_dfs_tie_picks_by_hand(orc):
+    """Eight points at x = 0, 2, ..., 14 (y = z = 0, Morton keys: a perfect
+    tree; leaf rank r at x = 2r), query x = 7, k = 4, knn_query's per-query walk.
+    DFS (farther child pushed first, left on a tie): root -> [0,6] -> [4,6] ->
+    leaf r3 (d2 1), r2 (9) -> [0,2] -> r1 (25), r0 (49) -> [8,14] -> [8,10] ->
+    r4 (1, replaces the root 49), r5 (9, replaces the root 25) -> [12,14]
+    pruned.  Heap [9/5, 1/4, 9/2, 1/3]; heapsort gives [1/4, 1/3, 9/2, 9/5]:
+    indices [4, 3, 2, 5], where the (d2, index) order is [3, 4, 2, 5]."""
+    x, y, z = _line_tree(range(0, 16, 2))
+    t = O.build_bvh(O.CT_U64, x, y, z, sfc=O.SFC_MORTON, bits=21, periodic=0)
+    q = (np.array([7], np.uint64), np.zeros(1, np.uint64), np.zeros(1, np.uint64))
+    r, d = O.knn_reference(t, 4, queries=q, max_k=32)
+    assert list(r[0]) == [4, 3, 2, 5] and list(d[0]) == [1, 1, 9, 9]
+    rb, db = O.knn_bruteforce(t, 4, queries=q)
+    assert list(rb[0]) == [3, 4, 2, 5] and list(db[0]) == [1, 1, 9, 9]
+
+
+MERGE_TIER_IDX = [17, 18, 16, 19, 20, 15, 21, 14, 22, 13, 23, 12, 24, 11, 10, 25, 26, 9, 27,
+                  8, 28, 7, 6, 29, 30, 5, 4, 31, 32, 3, 33, 2, 1]
+
+
+def merge_tier_case():
+    """34 points on a line: ranks 0-16 at x = 2r, ranks 17-33 at x = 64 + 2j
+    (the root splits at x bit 6); query x = 50, k = 33 (the SortedMergeHeap
+    tier of knn_query_large_k, cap 256).  The right subtree is nearer (gap 14
+    against 18), and inside each group the side facing the query comes first,
+    so leaves arrive as ranks 17, 18, ..., 33 then 16, 15, ..., 1; rank 0
+    (d2 2500) meets a full list whose worst is 2304 and is pruned."""
+    x, y, z = _line_tree([2 * r for r in range(17)] + [64 + 2 * j for j in range(17)])
+    q = (np.array([50], np.uint64), np.zeros(1, np.uint64), np.zeros(1, np.uint64))
+    arrivals = [((14 + 2 * j) ** 2, 17 + j) for j in range(17)] + \
+               [((18 + 2 * i) ** 2, 16 - i) for i in range(16)]
+    return (x, y, z), q, arrivals
+
+
+def _bitonic_textbook(d, ix, lo, n, asc):
+    """Batcher's recursive bitonic sort (the textbook form of the iterative
+    network of src/maxheap.hpp:193-215: the same comparators in the same
+    directions, swap when (d[i] > d[i + h]) == ascending)."""
+    if n <= 1:
+        return
+    h = n // 2
+    _bitonic_textbook(d, ix, lo, h, True)
+    _bitonic_textbook(d, ix, lo + h, h, False)
+    _bitonic_merge(d, ix, lo, n, asc)
+
+
+def _bitonic_merge(d, ix, lo, n, asc):
+    if n <= 1:
+        return
+    h = n // 2
+    for i in range(lo, lo + h):
+        if (d[i] > d[i + h]) == asc:
+            d[i], d[i + h] = d[i + h], d[i]
+            ix[i], ix[i + h] = ix[i + h], ix[i]
+    _bitonic_merge(d, ix, lo, h, asc)
+    _bitonic_merge(d, ix, lo + h, h, asc)
+
+
+def test_reference_merge_tier_tie_order(orc):
+    """The merge tier's final order for the hand-derived arrival sequence,
+    computed with the textbook network (one candidate + 31 sentinels staged and
+    sorted, appended at `count`, padded to a power of two, sorted, count
+    clamped to k): ties come out in a network-defined order (e.g. d2 324:
+    ranks [16, 19], though 19 arrived first; d2 900: [10, 25]).  The oracle's
+    restatement of knn_query_large_k gives the same literal array."""
+    pts, q, arrivals = merge_tier_case()
+    cap, k = 256, 33
+    d, ix, count = [int(S)] * cap, [-1] * cap, 0
+    for dd, ii in arrivals:
+        sd, si = [dd] + [int(S)] * 31, [ii] + [-1] * 31
+        _bitonic_textbook(sd, si, 0, 32, True)
+        for lane in range(32):
+            if count + lane < cap:
+                d[count + lane], ix[count + lane] = sd[lane], si[lane]
+        nc = min(count + 32, cap)
+        p2 = 1
+        while p2 < nc:
+            p2 <<= 1
+        for lane in range(nc, min(p2, cap)):
+            d[lane], ix[lane] = int(S), -1
+        _bitonic_textbook(d, ix, 0, p2, True)
+        count = min(nc, k)
+    assert ix[:k] == MERGE_TIER_IDX
+    t = O.build_bvh(O.CT_U64, *pts, sfc=O.SFC_MORTON, bits=21, periodic=0)
+    r, dr = O.knn_reference(t, k, queries=q, kind=1, max_k=256)
+    assert list(r[0]) == MERGE_TIER_IDX and list(dr[0]) == d[:k]
+
+
 # ---------------------------------------------------------------- generators ---
 def _libstdcxx_uniform(seed, n, a, b):
     """std::uniform_real_distribution<double>(a, b) over std::mt19937(seed) as
@@ -372,6 +507,60 @@ def test_reference_fixed_point_conversion(orc):
         bits = int(np.array([bb]).view(np.uint64)[0])
         exp.append((bits & ((1 << 52) - 1)) >> 31)
     assert [int(v) for v in x] == exp
+
+
+# ---------------------------------------- snapshot positions -> coordinates ----
+def test_fill_particle_coords_kat(orc):
+    """fill_particle_coords (test/benchmark/BenchmarkUtils.hpp:164-205) on a hand
+    case: one cube over all axes, min 0, max 100, size 100 * 1.0001, so a value v
+    maps to floor(v / 100.01 * 2^21), computed here in exact rationals: 0, 50,
+    100 -> 0, 1048471, 2096942; y = 10 -> 209694, z = 20 -> 419388."""
+    px = np.array([0.0, 50.0, 100.0], np.float32)
+    py = np.full(3, 10.0, np.float32)
+    pz = np.full(3, 20.0, np.float32)
+    for ct in (O.CT_U64, O.CT_U32):
+        x, y, z, bmin, bsize = O.fill_particle_coords(px, py, pz, ct, 21)
+        assert list(x) == [0, 1048471, 2096942]
+        assert list(y) == [209694] * 3 and list(z) == [419388] * 3
+        assert bmin == 0.0 and bsize == 100.0 * 1.0001
+
+
+def test_fill_particle_coords_invariants(orc):
+    """The largest coordinate always maps to floor(2^B / 1.0001) (2096942 at
+    B = 21, 1023 at B = 10) and the smallest to 0, whichever axis holds them; the
+    box is shared by the axes (moving y's minimum shifts every x); equal
+    positions give box (v, 1.0) and zeros; the map is monotone; FP coordinate
+    types are a plain cast with box (0, 1); empty input gives box (0, 1)."""
+    rng = np.random.default_rng(7)
+    for bits, top in ((21, 2096942), (10, 1023)):
+        px = rng.uniform(-3.0, 40.0, 5000).astype(np.float32)
+        py = rng.uniform(-7.5, 12.0, 5000).astype(np.float32)
+        pz = rng.uniform(0.0, 55.25, 5000).astype(np.float32)
+        x, y, z, bmin, bsize = O.fill_particle_coords(px, py, pz, O.CT_U64, bits)
+        allv = np.concatenate([px, py, pz]).astype(np.float64)
+        allc = np.concatenate([x, y, z])
+        assert bmin == allv.min() and allc[allv.argmax()] == top and allc[allv.argmin()] == 0
+        o = np.argsort(allv, kind="stable")
+        assert np.all(np.diff(allc[o].astype(np.int64)) >= 0)
+        # floor(nx * 2^B) is exact for nx away from the cell edges (nx is
+        # within 2^-52 of an edge only for a handful of values)
+        nx = (allv - bmin) / bsize
+        lo = np.floor(nx * 2.0 ** bits).astype(np.int64)
+        d = allc.astype(np.int64) - lo
+        assert np.all((d == 0) | (d == 1)) and np.count_nonzero(d) <= 3
+        # a lower y minimum moves the shared box, so every x changes
+        py2 = py.copy()
+        py2[17] = -20.0
+        x2, _, _, bmin2, _ = O.fill_particle_coords(px, py2, pz, O.CT_U64, bits)
+        assert bmin2 == -20.0 and np.all(x2 >= x) and np.count_nonzero(x2 != x) > 4900
+    same = np.full(4, 3.5, np.float32)
+    x, y, z, bmin, bsize = O.fill_particle_coords(same, same, same, O.CT_U64, 21)
+    assert bmin == 3.5 and bsize == 1.0 and not x.any() and not y.any() and not z.any()
+    f = rng.uniform(0, 1, 10).astype(np.float32)
+    x, y, z, bmin, bsize = O.fill_particle_coords(f, f, f, O.CT_F64, 21)
+    assert np.array_equal(x, f.astype(np.float64)) and (bmin, bsize) == (0.0, 1.0)
+    e = np.zeros(0, np.float32)
+    assert O.fill_particle_coords(e, e, e, O.CT_U64, 21)[3:] == (0.0, 1.0)
 
 
 # ---------------------------------------------------------------- kNN ----------
@@ -434,6 +623,28 @@ def test_reference_traversal_vs_bruteforce(orc, kind, seed, ct):
             assert np.array_equal(db[:, :k], dr)
             m = untied(db, k)
             assert np.array_equal(rb[:, :k][m], rr[m])
+
+
+@pytest.mark.parametrize("kind,seed", [(O.GEN_UNIFORM, 42), (O.GEN_CLUSTERED, 12345),
+                                       (O.GEN_DEGENERATE, 999)])
+def test_reference_grouped_traversal(orc, kind, seed):
+    """The grouped lockstep walk (8 and 16 lanes; the CPU baseline's form of the
+    reference kernel) returns the per-query walk's arrays exactly -- a lane pushes
+    a leaf only when its own test passes -- on trees too shallow for the stack
+    limit, and makes at least as many visits (a lane expands nodes it pruned
+    while another lane needs its step); with a subset whose size is not a
+    multiple of the width, the shadow lanes change nothing."""
+    x, y, z = O.gen_reference(kind, 1000, seed, O.CT_U64, bits=21)
+    t = O.build_bvh(O.CT_U64, x, y, z, sfc=O.SFC_PH, bits=21)
+    ids = np.arange(3, 1000, 7, dtype=np.int32)
+    for k in (1, 16, 32, 64):
+        rr, dr, cnt = O.knn_reference(t, k, True, leaf_ids=ids, max_k=256, counters=True)
+        rb, db = O.knn_bruteforce(t, k + 1, True, leaf_ids=ids)
+        assert np.array_equal(db[:, :k], dr)
+        for w in (8, 16):
+            rg, dg, vis = O.knn_reference_grouped(t, k, True, leaf_ids=ids, width=w, visits=True)
+            assert np.array_equal(rg, rr) and np.array_equal(dg, dr), (k, w)
+            assert np.all(vis >= cnt[:, 0]) and vis.sum() > cnt[:, 0].sum()
 
 
 def test_reference_sorted_merge_vs_bruteforce(orc):
