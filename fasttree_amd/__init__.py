@@ -91,7 +91,7 @@ EXPORTS = [
     "ft_scratch_set_radix_bits",
     "ft_compute_bbox", "ft_sfc_encode", "ft_build_tree", "ft_fill_particle_coords",
     "ft_build_bvh", "ft_self_knn_query",
-    "ft_self_knn_query_subset", "ft_knn_query", "ft_range_query", "ft_range_query_csr",
+    "ft_self_knn_query_subset", "ft_self_knn_query_noalloc", "ft_knn_query", "ft_range_query", "ft_range_query_csr",
     "ft_mlp_num_params", "ft_mlp_create", "ft_mlp_destroy", "ft_mlp_set", "ft_mlp_get",
     "ft_knn_features", "ft_mlp_forward", "ft_mlp_grad", "ft_mlp_train_step",
     "ft_mlp_last_loss", "ft_mlp_seed", "ft_batch_create", "ft_batch_destroy",
@@ -149,6 +149,8 @@ def lib():
                                               vp, vp]
         L.ft_self_knn_query.argtypes = [vp, C.POINTER(_Tree), i32, vp, vp, i32,
                                         C.POINTER(QueryOpts)]
+        L.ft_self_knn_query_noalloc.argtypes = [vp, C.POINTER(_Tree), i32, vp, vp, i32,
+                                                C.POINTER(QueryOpts)]
         L.ft_self_knn_query_subset.argtypes = [vp, C.POINTER(_Tree), vp, i32, i32, vp, vp, i32,
                                                C.POINTER(QueryOpts)]
         L.ft_knn_query.argtypes = [vp, C.POINTER(_Tree), vp, vp, vp, i32, i32, vp, vp,

@@ -45,6 +45,9 @@ int range_entry(hipStream_t s, const ft_tree *tree, const void *qx, const void *
 int scan_entry(hipStream_t s, const int32_t *counts, int nq, int64_t *row_ptr);
 int release_stream_bufs(hipStream_t s);
 void graph_attach(hipStream_t s, void *exec);
+int knn_noalloc_entry(hipStream_t s, const ft_tree *tree, int k, int excl, uint64_t *res,
+                      void *dists, const ft_query_opts *opts);
+size_t noalloc_max_queries();
 size_t snapshot_work_doubles();
 int snapshot_entry(hipStream_t s, int in_type, const void *px, const void *py, const void *pz,
                    size_t n, int coord_type, int bits, void *x, void *y, void *z, double *work);
@@ -778,6 +781,28 @@ int ft_self_knn_query(ft_stream s, const ft_tree *tree, int k, uint64_t *results
     if (k == 0) return FT_OK;
     return knn_common(s, tree, nullptr, nullptr, nullptr, nullptr, (int)n, k, results, dists,
                       exclude_self ? 1 : 0, opts);
+}
+
+// Same results as ft_self_knn_query, with no allocation on any path: device
+// (or pinned host) buffers only, since staging host memory would allocate.
+int ft_self_knn_query_noalloc(ft_stream s, const ft_tree *tree, int k, uint64_t *results,
+                              void *dists, int exclude_self, const ft_query_opts *opts) {
+    if (!tree) return fail(FT_EINVAL, "null tree");
+    const size_t n = tree->num_leaves;
+    if (n == 0) return FT_OK;
+    FT_TRY(check_k(k, (opts && opts->max_k > 0) ? opts->max_k : 32, "self_knn_query"));
+    if (k == 0) return FT_OK;
+    if (n > noalloc_max_queries())
+        return fail(FT_ERANGE, "ft_self_knn_query_noalloc: more than 16777216 queries (its "
+                               "fixed scratch size); use ft_self_knn_query");
+    if (!results || !dists) return fail(FT_EINVAL, "null results or dists");
+    const void *dev_only[4] = {results, dists, opts ? (const void *)opts->seed_bound : nullptr,
+                               opts ? (const void *)opts->stats : nullptr};
+    for (const void *p : dev_only)
+        if (!is_device_accessible(p))
+            return fail(FT_EINVAL, "ft_self_knn_query_noalloc: results, dists, seed_bound and "
+                                   "stats must be device memory (host buffers need staging)");
+    return knn_noalloc_entry(as_stream(s), tree, k, exclude_self ? 1 : 0, results, dists, opts);
 }
 
 int ft_self_knn_query_subset(ft_stream s, const ft_tree *tree, const int32_t *leaf_ids,

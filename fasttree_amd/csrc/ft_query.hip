@@ -1192,6 +1192,68 @@ int knn_entry(hipStream_t s, const ft_tree *tree, const void *qx, const void *qy
     return FT_OK;
 }
 
+// ---- non-allocating self-kNN (no reference counterpart) ----------------------
+// For callers that cannot allocate (code under graph capture, serving loops):
+// the canonical pipeline's re-run buffers (per-slot flags, the compacted slot
+// list, per-tile counts, the list length) live in one file-scope device array
+// instead of per-stream allocations.  Every call on a device shares that
+// device's copy, which keeps the last call's contents; the flags are cleared
+// by a kernel at the start of each call and the other fields are written
+// before they are read.  Calls that overlap on one device race.
+constexpr size_t NA_MAX_QUERIES = (size_t)1 << 24;
+constexpr size_t na_al(size_t x) { return (x + 255) & ~(size_t)255; }
+constexpr size_t NA_OFF_LIST = na_al(NA_MAX_QUERIES);
+constexpr size_t NA_OFF_TILES = NA_OFF_LIST + na_al(NA_MAX_QUERIES * 4);
+constexpr size_t NA_OFF_COUNT = NA_OFF_TILES + na_al((NA_MAX_QUERIES / CP_TILE + 2) * 4);
+constexpr size_t NA_BYTES = NA_OFF_COUNT + 256;  // 80 MiB + 16.5 KiB
+static __device__ __align__(256) uint8_t scratch[NA_BYTES];
+
+constexpr int NA_MAX_DEVICES = 64;
+static std::mutex g_na_mu;
+static void *g_na_ptr[NA_MAX_DEVICES];
+
+// The device address of `scratch` on the current device, looked up once per
+// device (each device holds its own copy of the array at its own address).
+static int noalloc_scratch(void **out) {
+    int dev = 0;
+    FT_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NA_MAX_DEVICES) return fail(FT_EINVAL, "device index out of range");
+    std::lock_guard<std::mutex> lk(g_na_mu);
+    if (!g_na_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(FT_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_na_ptr[dev] = p;
+    }
+    *out = g_na_ptr[dev];
+    return FT_OK;
+}
+
+size_t noalloc_max_queries() { return NA_MAX_QUERIES; }
+
+int knn_noalloc_entry(hipStream_t s, const ft_tree *tree, int k, int excl, uint64_t *res,
+                      void *dists, const ft_query_opts *opts) {
+    const size_t nq = tree->num_leaves;
+    if (nq > NA_MAX_QUERIES)
+        return fail(FT_ERANGE, "ft_self_knn_query_noalloc: more than 16777216 queries (its "
+                               "fixed scratch size); use ft_self_knn_query");
+    void *p = nullptr;
+    FT_TRY(noalloc_scratch(&p));
+    char *c = (char *)p;
+    RerunBuf rr;
+    rr.flags = (uint8_t *)c;
+    rr.list = (int32_t *)(c + NA_OFF_LIST);
+    rr.tiles = (uint32_t *)(c + NA_OFF_TILES);
+    rr.count = (uint32_t *)(c + NA_OFF_COUNT);
+    rr.cap = NA_MAX_QUERIES;
+    FT_DISPATCH_COORD(tree->coord_type, {
+        const QSrc<T> qs{nullptr, nullptr, nullptr, nullptr};
+        return knn_dispatch<T>(s, tree, qs, (int)nq, k, excl, res, dists, opts, false, &rr);
+    });
+    return FT_OK;
+}
+
 // Range query launch: queries in the order `order` (slot -> query), packet
 // walk with the metric kind fixed at compile time.
 template <typename T>

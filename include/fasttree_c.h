@@ -249,6 +249,16 @@ int ft_build_bvh(ft_stream s, ft_tree *tree, ft_scratch *scratch, const void *x,
 /* -DBL_MAX (FP) or 0 (integer), as type_identity_max (src/hlbvh.hpp:59-69).             */
 int ft_self_knn_query(ft_stream s, const ft_tree *tree, int k, uint64_t *results, void *dists,
                       int exclude_self, const ft_query_opts *opts);
+/* Non-allocating self_knn_query: no counterpart in the reference.  For callers that     */
+/* cannot allocate device memory (code under hipGraph capture, tight serving loops,       */
+/* harnesses that forbid hipMalloc).  Same arguments and results as ft_self_knn_query;    */
+/* the re-run buffers live in a fixed device array of 80 MiB per GPU, held until the      */
+/* process ends, so at most 16777216 queries (num_leaves; FT_ERANGE above).  results,     */
+/* dists, seed_bound and stats must be device (or pinned host) memory (FT_EINVAL          */
+/* otherwise).  Every call on one GPU shares that array: calls that overlap on one GPU    */
+/* (on different streams or threads) race.                                                */
+int ft_self_knn_query_noalloc(ft_stream s, const ft_tree *tree, int k, uint64_t *results,
+                              void *dists, int exclude_self, const ft_query_opts *opts);
 /* self_knn_query_subset (src/hlbvh.hpp:1956-1983): leaf ranks, results in query order. */
 int ft_self_knn_query_subset(ft_stream s, const ft_tree *tree, const int32_t *leaf_ids,
                              int num_ids, int k, uint64_t *results, void *dists,
