@@ -448,72 +448,82 @@ template <typename D, bool REF> struct LdsHeap {
 
 
 // ---------------------------------------------------------------------------
-// Packed top-k for integer distances: key = min(d2, SAT) << ib | index, one
-// 64-bit compare orders by (d2, index).  ib = bit length of n, so every index
-// is below 2^ib - 1 and an all-ones index marks a sentinel.  A key whose d2
-// part saturated cannot be ordered exactly; the traversal reports such queries
-// (and queries left with sentinels) for an exact re-run.
+// Packed top-k for integer distances: key = BIAS | min(d2, SAT) << ib | index,
+// one 64-bit compare orders by (d2, index).  ib = bit length of n, so every
+// index is below 2^ib - 1 and an all-ones index marks a sentinel.  A key whose
+// d2 part saturated cannot be ordered exactly; the traversal reports such
+// queries (and queries left with sentinels) for an exact re-run.
+//
+// SAT = 2^(61 - ib) - 1 keeps every unbiased key below 2^61, and BIAS = 2^62
+// lifts it into [2^62, 2^62 + 2^61): read as an IEEE double, such a key is a
+// positive, normal, finite number (exponent field 0x400 to 0x5FF), and positive
+// finite doubles order exactly as their bit patterns do.  So the insertion
+// chain can use the native v_max_f64 / v_min_f64 (one instruction each for a
+// whole 64-bit key) instead of a 64-bit compare and four 32-bit selects.
+// Dead slots (below M - k) hold 0 (+0.0), below every key.
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ double f64_max(double a, double b) {
+    double r;
+    asm("v_max_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+__device__ __forceinline__ double f64_min(double a, double b) {
+    double r;
+    asm("v_min_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+__device__ __forceinline__ double key_f64(uint64_t k) { return __longlong_as_double((long long)k); }
+__device__ __forceinline__ uint64_t key_u64(double k) { return (uint64_t)__double_as_longlong(k); }
+
 template <int M> struct TopKPacked {
     static constexpr bool kPacked = true;
+    static constexpr uint64_t BIAS = 1ull << 62;
+    static constexpr uint64_t NOKEY = BIAS | (1ull << 61);  // above every key, still finite
     uint64_t key[M];
     int ib;
     uint64_t sat, imask;
     __device__ inline void init(int k, uint64_t bound, int ibits) {
         ib = ibits;
-        sat = (1ull << (64 - ib)) - 1;
+        sat = (1ull << (61 - ib)) - 1;
         imask = (1ull << ib) - 1;
-        const uint64_t sent = ((bound < sat ? bound : sat) << ib) | imask;
+        const uint64_t sent = BIAS | ((bound < sat ? bound : sat) << ib) | imask;
 #pragma unroll
         for (int s = 0; s < M; ++s) key[s] = (s >= M - k) ? sent : 0ull;
     }
     __device__ inline uint64_t make(uint64_t d, int32_t i) const {
-        return ((d < sat ? d : sat) << ib) | (uint64_t)(uint32_t)i;
+        return BIAS | ((d < sat ? d : sat) << ib) | (uint64_t)(uint32_t)i;
     }
-    __device__ inline uint64_t worst_d() const { return key[M - 1] >> ib; }
+    __device__ inline uint64_t dist_of(uint64_t kb) const { return (kb & ~BIAS) >> ib; }
+    __device__ inline uint64_t worst_d() const { return dist_of(key[M - 1]); }
     __device__ inline bool beats_worst(uint64_t d, int32_t i) const { return make(d, i) < key[M - 1]; }
-    // Sorted insertion as an in-place shift from the top slot down: slot s
-    // takes slot s-1 when the new key is below both, the new key when it is
-    // below s only.  A lane that takes nothing uses the all-ones key, which is
-    // below no slot, so no predicate is needed; every compare reads an
-    // unshifted slot, so the compares are independent.
-    // Slots s >= L of the shift chain, when no taking lane's key is below
-    // key[L-1] (slots below L then keep their keys): slot s becomes key[s-1]
-    // if c < key[s-1] (which, the slots being sorted, implies c < key[s]),
-    // else c if c < key[s], else stays.  The select on the previous slot's
-    // compare comes first, so it can issue while the new 64-bit compare's
-    // result settles.
-    template <int L> __device__ inline void chain_from(uint64_t c, uint64_t cs) {
-        bool lt_hi = c < key[M - 1];
+    // Sorted insertion, in place from the top slot down: with the slots
+    // sorted, the new content of slot s is the median of key[s-1], c and
+    // key[s], i.e. max(key[s-1], min(c, key[s])) -- key[s-1] when c is below
+    // it, c when c falls between, key[s] when c is not below it.  Slots s >= L
+    // only, when no taking lane's key is below key[L-1] (slots below L then
+    // keep their keys, and slot L becomes min(c, key[L])).  A lane that takes
+    // nothing inserts NOKEY, which changes no slot.
+    template <int L> __device__ inline void chain_from(double c) {
 #pragma unroll
-        for (int s = M - 1; s >= L + 1; --s) {
-            const bool lt_lo = c < key[s - 1];
-            const uint64_t stay = lt_hi ? cs : key[s];
-            key[s] = lt_lo ? key[s - 1] : stay;
-            lt_hi = lt_lo;
-        }
-        key[L] = lt_hi ? cs : key[L];
+        for (int s = M - 1; s >= L + 1; --s)
+            key[s] = key_u64(f64_max(key_f64(key[s - 1]), f64_min(c, key_f64(key[s]))));
+        key[L] = key_u64(f64_min(c, key_f64(key[L])));
     }
     __device__ inline void insert(bool take, uint64_t d, int32_t i) {
-        const uint64_t c = take ? make(d, i) : ~0ull;
-        // cs is c behind an empty asm, so the compiler does not turn
-        // (c < k ? cs : k) into min(c, k), which it would expand with a second
-        // compare per slot.  Not volatile: a volatile asm counts as a possible
-        // memory write, and later uniform loads of tree nodes would then lose
-        // their scalar form.
-        uint64_t cs = c;
-        asm("" : "+v"(cs));
+        const uint64_t c = take ? make(d, i) : NOKEY;
+        const double cd = key_f64(c);
         // New keys mostly land near the top: run only the part of the chain a
         // taking lane can reach (one compare and a wave vote per level).
         if constexpr (M >= 16) {
-            if (!__any(c < key[3 * M / 4 - 1])) { chain_from<3 * M / 4>(c, cs); return; }
-            if (!__any(c < key[M / 2 - 1])) { chain_from<M / 2>(c, cs); return; }
+            if (!__any(c < key[7 * M / 8 - 1])) { chain_from<7 * M / 8>(cd); return; }
+            if (!__any(c < key[3 * M / 4 - 1])) { chain_from<3 * M / 4>(cd); return; }
+            if (!__any(c < key[M / 2 - 1])) { chain_from<M / 2>(cd); return; }
         }
-        chain_from<0>(c, cs);
+        chain_from<0>(cd);
     }
     // Sentinels left in live slots, or a saturated k-th distance.
     __device__ inline bool needs_exact(int k) const {
-        bool bad = (key[M - 1] >> ib) >= sat;
+        bool bad = dist_of(key[M - 1]) >= sat;
 #pragma unroll
         for (int s = 0; s < M; ++s) bad |= (s >= M - k) & ((key[s] & imask) == imask);
         return bad;
@@ -527,7 +537,7 @@ template <int M> struct TopKPacked {
     // Multi-pass queries: a pass takes only keys at or above the floor.
     using Floor = uint64_t;
     __device__ static inline Floor no_floor() { return 0ull; }
-    __device__ inline uint64_t floor_d(Floor lo) const { return lo >> ib; }
+    __device__ inline uint64_t floor_d(Floor lo) const { return dist_of(lo); }
     __device__ inline bool above(Floor lo, uint64_t d, int32_t i) const { return make(d, i) >= lo; }
     // Lower key bound of the next pass of a multi-pass query: above the k-th
     // key, or nothing when a slot is still empty (no further neighbours).
@@ -546,7 +556,7 @@ template <int M> struct TopKPacked {
                 const size_t j = o + (size_t)(s - (M - k));
                 const bool ok = (key[s] & imask) != imask;
                 res[j] = ok ? (key[s] & imask) : ~0ull;
-                dists[j] = ok ? (DOut)(key[s] >> ib) : missing_dist<DOut>();
+                dists[j] = ok ? (DOut)dist_of(key[s]) : missing_dist<DOut>();
             }
         }
     }
@@ -559,7 +569,7 @@ template <int M> struct TopKPacked {
                 const size_t j = o + (size_t)(s - (M - k));
                 const bool ok = (key[s] & imask) != imask;
                 res[j] = ok ? (key[s] & imask) : ~0ull;
-                dists[j] = ok ? (DOut)(key[s] >> ib) : missing_dist<DOut>();
+                dists[j] = ok ? (DOut)dist_of(key[s]) : missing_dist<DOut>();
             }
         }
     }
