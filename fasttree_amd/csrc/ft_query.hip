@@ -386,7 +386,18 @@ __global__ __launch_bounds__(64) void packet_knn_kernel(
         if constexpr (HEAP) h.key = heap_mem;
         if constexpr (PACKED) h.init(k, (SEEDED && active) ? (uint64_t)seed[q] : ~0ull, a.ibits);
         else h.init(k, (SEEDED && active) ? seed[q] : dist_max<D>());
-        packet_walk<T, H, MK>(t, a.bucket, active, px, py, pz, self, h, stk, c, wlo, whi, r0);
+        int flo = 0, fhi = 0;
+        if constexpr (H::kSortFill) {
+            // self queries over all leaves: lane = leaf rank r0 + lane; the
+            // window (>= 32 leaves on each side) covers every filled rank
+            if (!qs.qx && !qs.leaf_ids && a.window >= 32) {
+                __shared__ FillStage<T> fst;
+                c.inserts += (uint64_t)fill_from_run<T, MK>(t, h, k, active, self >= 0, rank, px,
+                                                            py, pz, r0, &fst, flo, fhi);
+            }
+        }
+        packet_walk<T, H, MK>(t, a.bucket, active, px, py, pz, self, h, stk, c, wlo, whi, r0,
+                              H::no_floor(), flo, fhi);
         if (active) {
             bool redo = false;
             if constexpr (PACKED) redo = h.needs_exact(k);

@@ -285,6 +285,7 @@ template <typename T> __device__ inline int32_t ret_index(const DTree<T> &t, int
 // ---------------------------------------------------------------------------
 template <typename D, int M> struct TopK {
     static constexpr bool kPacked = false;
+    static constexpr bool kSortFill = false;
     D d[M];
     int32_t i[M];
     __device__ inline void init(int k, D bound) {
@@ -477,6 +478,7 @@ __device__ __forceinline__ uint64_t key_u64(double k) { return (uint64_t)__doubl
 
 template <int M> struct TopKPacked {
     static constexpr bool kPacked = true;
+    static constexpr bool kSortFill = true;
     static constexpr uint64_t BIAS = 1ull << 62;
     static constexpr uint64_t NOKEY = BIAS | (1ull << 61);  // above every key, still finite
     uint64_t key[M];
@@ -520,6 +522,45 @@ template <int M> struct TopKPacked {
             if (!__any(c < key[M / 2 - 1])) { chain_from<M / 2>(cd); return; }
         }
         chain_from<0>(cd);
+    }
+    // Bitonic sort of the M slots, ascending, in registers (each
+    // compare-exchange is one v_min_f64 and one v_max_f64).
+    __device__ inline void sort_slots() {
+#pragma unroll
+        for (int kk = 2; kk <= M; kk <<= 1) {
+#pragma unroll
+            for (int j = kk >> 1; j > 0; j >>= 1) {
+#pragma unroll
+                for (int i = 0; i < M; ++i) {
+                    const int l = i ^ j;
+                    if (l > i) {
+                        const double a = key_f64(key[i]), b = key_f64(key[l]);
+                        const double lo = f64_min(a, b), hi = f64_max(a, b);
+                        const bool asc = (i & kk) == 0;
+                        key[i] = key_u64(asc ? lo : hi);
+                        key[l] = key_u64(asc ? hi : lo);
+                    }
+                }
+            }
+        }
+    }
+    // Right after init: the k live slots take cand(0..k-1) (keys from make();
+    // a key above the init sentinel, i.e. beyond the bound, becomes the
+    // sentinel), then the slots are sorted.  Returns the number of keys kept.
+    template <typename F> __device__ inline int fill_sorted(int k, F &&cand) {
+        const uint64_t sent = key[M - 1];
+        int kept = 0;
+#pragma unroll
+        for (int s = 0; s < M; ++s) {
+            if (s >= M - k) {
+                const uint64_t kb = cand(s - (M - k));
+                const bool in = kb < sent;
+                key[s] = in ? kb : sent;
+                kept += in ? 1 : 0;
+            }
+        }
+        sort_slots();
+        return kept;
     }
     // Sentinels left in live slots, or a saturated k-th distance.
     __device__ inline bool needs_exact(int k) const {
@@ -590,6 +631,8 @@ template <int M> struct TopKPacked {
 //
 // h must be initialised by the caller (k, bound); self = leaf rank excluded
 // from the lane's results (-1: none); inactive lanes take no candidates.
+// [fill_lo, fill_hi): leaf ranks the caller already put into this lane's h
+// (fill_from_run); the window pre-scan skips them.
 // ===========================================================================
 constexpr int PK_STACK = 128;
 
@@ -603,7 +646,8 @@ __device__ __forceinline__ void packet_walk(const DTree<T> &t, int bucket, bool 
                                             T pz, int self, H &h, int32_t *stk,
                                             PacketCounters &c, int win_lo = 0, int win_hi = 0,
                                             int win_mid = 0,
-                                            typename H::Floor lo = H::no_floor()) {
+                                            typename H::Floor lo = H::no_floor(),
+                                            int fill_lo = 0, int fill_hi = 0) {
     using D = typename CoordTraits<T>::D;
     const int lane = threadIdx.x & 63;
     const int n = t.n;
@@ -739,6 +783,14 @@ __device__ __forceinline__ void packet_walk(const DTree<T> &t, int bucket, bool 
         if (!active) cand = 0;
         const int sr = self - first;
         if (sr >= 0 && sr < cnt) cand &= ~(1ull << sr);
+        if (!skip_window && fill_hi > fill_lo) {  // leaves this lane's fill put in h
+            const int a0 = fill_lo - first > 0 ? fill_lo - first : 0;
+            const int a1 = fill_hi - first < cnt ? fill_hi - first : cnt;
+            if (a1 > a0) {
+                const uint64_t hi_m = a1 >= 64 ? ~0ull : ((1ull << a1) - 1);
+                cand &= ~(hi_m & ~((1ull << a0) - 1));
+            }
+        }
         if (skip_window) {  // leaves of the pre-scanned window are already in h
             const int a0 = win_lo - first > 0 ? win_lo - first : 0;
             const int a1 = win_hi - first < cnt ? win_hi - first : cnt;
@@ -865,6 +917,55 @@ __device__ __forceinline__ void packet_walk(const DTree<T> &t, int bucket, bool 
 }
 
 // ---------------------------------------------------------------------------
+// Curve-neighbour fill of a self-query packet (queries = leaf ranks
+// [r0, r0 + 64)): each lane's k live slots take the exact keys of its k
+// nearest ranks along the curve (k/2 below, the rest above, itself skipped
+// when excluded), sorted in registers -- one bitonic sort instead of k
+// insertion rounds, and a tight bound before the window pre-scan.  Leaves
+// [r0 - 32, r0 + 96) are staged in LDS (st: 128 entries) so each lane reads
+// its candidates without shuffles.  Returns this lane's filled rank range
+// [flo, fhi) for the window pre-scan to skip.  k <= 32.
+// ---------------------------------------------------------------------------
+template <typename T> struct FillStage {
+    T x[128], y[128], z[128];
+    int32_t idx[128];
+};
+
+template <typename T, int MK, typename H>
+__device__ __forceinline__ int fill_from_run(const DTree<T> &t, H &h, int k, bool active,
+                                             bool excl, int rank, T px, T py, T pz, int r0,
+                                             FillStage<T> *st, int &flo, int &fhi) {
+    const int lane = threadIdx.x & 63;
+    const int n = t.n, off = n - 1;
+    const int base = r0 - 32;
+#pragma unroll
+    for (int e = lane; e < 128; e += 64) {
+        const int j = base + e;
+        if (j >= 0 && j < n) {
+            st->x[e] = t.mnx[off + j];
+            st->y[e] = t.mny[off + j];
+            st->z[e] = t.mnz[off + j];
+            st->idx[e] = ret_index(t, off + j);
+        }
+    }
+    __syncthreads();  // the block is this one wave
+    const int half = k >> 1;
+    const int kept = h.fill_sorted(k, [&](int i) -> uint64_t {
+        int o = i - half;
+        if (excl && o >= 0) ++o;
+        const int j = rank + o;
+        if (!active || j < 0 || j >= n) return ~0ull;
+        const int e = j - base;
+        const auto d = leaf_d2_k<T, MK>(t.m, px, py, pz, st->x[e], st->y[e], st->z[e]);
+        return h.make((uint64_t)d, st->idx[e]);
+    });
+    flo = active ? rank - half : 0;
+    fhi = active ? rank + (k - half) + (excl ? 1 : 0) : 0;
+    __syncthreads();
+    return kept;
+}
+
+// ---------------------------------------------------------------------------
 // Packed (d2, index) keys in a max-heap per lane in LDS, layout [slot][64
 // lanes] so a wave's accesses to one slot hit 64 distinct banks: the top-k
 // container for k above the register tiers (k <= KCAP).  Same key form and
@@ -874,6 +975,7 @@ __device__ __forceinline__ void packet_walk(const DTree<T> &t, int bucket, bool 
 // ---------------------------------------------------------------------------
 template <int KCAP> struct TopKHeapLds {
     static constexpr bool kPacked = true;
+    static constexpr bool kSortFill = false;
     using Floor = uint64_t;  // (single pass only)
     __device__ static inline Floor no_floor() { return 0ull; }
     uint64_t *key;  // [KCAP][64], owned by the kernel

@@ -109,8 +109,9 @@ def test_training_learns_and_seeds_are_exact(ft):
     """A short GPU training run on exact k=16 distances of one snapshot lowers
     the loss well below the constant predictor's; seeding the traversal with
     the trained model leaves every index and distance unchanged, and what the
-    network gains is measured on the same tree: the seeded walk performs
-    fewer top-k insertions than the walk without it."""
+    network gains is measured on the same tree: with the seed as each lane's
+    only prior bound, the seeded walk performs fewer top-k insertions than the
+    walk without it."""
     from fasttree_amd import learned as L
     k = 16
     tree = L.snapshot_tree(200_000, 11)
@@ -135,10 +136,12 @@ def test_training_learns_and_seeds_are_exact(ft):
     seeds = ft.DeviceArray(n, np.uint64)
     scratch = ft.DeviceArray(n * 16, np.float32)
     m.seed(tree, k, seeds, scratch)
+    # The network's gain, measured where the seed is each lane's only prior
+    # bound (no window pre-scan or curve-neighbour fill): fewer insertions.
     stats = ft.DeviceArray(8, np.uint64).zero()
-    r1, d1 = ft.self_knn_query(tree, k, seed_bound=seeds, stats=stats)
+    r1, d1 = ft.self_knn_query(tree, k, seed_bound=seeds, stats=stats, window=-1)
     stats0 = ft.DeviceArray(8, np.uint64).zero()
-    r0, d0 = ft.self_knn_query(tree, k, algorithm=ft.ALGO_PACKET, stats=stats0)
+    r0, d0 = ft.self_knn_query(tree, k, algorithm=ft.ALGO_PACKET, stats=stats0, window=-1)
     assert np.array_equal(d1, d0) and np.array_equal(r1, r0)
     s1, s0 = stats.to_host(), stats0.to_host()
     reruns = int(s1[3])
@@ -146,6 +149,11 @@ def test_training_learns_and_seeds_are_exact(ft):
           f"{s1[4] / n:.1f} seeded vs {s0[4] / n:.1f} without the network")
     assert reruns < 0.2 * n
     assert s1[4] < 0.9 * s0[4]
+    # with the default pre-scan and fill the results are the same again
+    stats2 = ft.DeviceArray(8, np.uint64).zero()
+    r2, d2 = ft.self_knn_query(tree, k, seed_bound=seeds, stats=stats2)
+    assert np.array_equal(d2, d0) and np.array_equal(r2, r0)
+    assert int(stats2.to_host()[3]) == reruns
 
 
 def test_fp_model_seeds_subset_and_self_queries_exactly(ft):
