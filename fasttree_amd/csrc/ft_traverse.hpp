@@ -511,8 +511,10 @@ template <int M> struct TopKPacked {
             key[s] = key_u64(f64_max(key_f64(key[s - 1]), f64_min(c, key_f64(key[s]))));
         key[L] = key_u64(f64_min(c, key_f64(key[L])));
     }
-    __device__ inline void insert(bool take, uint64_t d, int32_t i) {
-        const uint64_t c = take ? make(d, i) : NOKEY;
+    __device__ inline bool beats_worst_key(uint64_t kb) const { return kb < key[M - 1]; }
+    __device__ inline void insert(bool take, uint64_t d, int32_t i) { insert_key(take, make(d, i)); }
+    __device__ inline void insert_key(bool take, uint64_t kb) {
+        const uint64_t c = take ? kb : NOKEY;
         const double cd = key_f64(c);
         // New keys mostly land near the top: run only the part of the chain a
         // taking lane can reach (one compare and a wave vote per level).
@@ -806,7 +808,8 @@ __device__ __forceinline__ void packet_walk(const DTree<T> &t, int bucket, bool 
             const bool has = cand != 0;
             const int j = has ? (__ffsll((long long)cand) - 1) : 0;
             cand &= cand - 1;
-            const int32_t ri = shfl_any(bo, j);
+            // index of leaf first + j: its rank, or orig_idx fetched from lane j
+            const int32_t ri = t.orig_indices ? shfl_any(bo, j) : first + j;
             D d;
             if constexpr (PRE) {
                 const uint64_t gx = wrap_gap_pre(qxs, shfl32(ex, j), wsh);
@@ -817,12 +820,23 @@ __device__ __forceinline__ void packet_walk(const DTree<T> &t, int bucket, bool 
                 const T x = shfl_any(bx, j), y = shfl_any(by, j), z = shfl_any(bz, j);
                 d = leaf_d2_k<T, MK>(t.m, px, py, pz, x, y, z);
             }
-            bool take = has && h.beats_worst(d, ri);
-            if constexpr (FLOOR) take = take && h.above(lo, d, ri);
             ++c.rounds;
-            if (__any(take)) {  // a round no lane takes skips the chain
-                h.insert(take, d, ri);
-                ++c.chains;
+            bool take;
+            if constexpr (H::kSortFill) {  // packed keys: made once for test and insertion
+                const uint64_t kb = h.make((uint64_t)d, ri);
+                take = has && h.beats_worst_key(kb);
+                if constexpr (FLOOR) take = take && kb >= lo;
+                if (__any(take)) {  // a round no lane takes skips the chain
+                    h.insert_key(take, kb);
+                    ++c.chains;
+                }
+            } else {
+                take = has && h.beats_worst(d, ri);
+                if constexpr (FLOOR) take = take && h.above(lo, d, ri);
+                if (__any(take)) {
+                    h.insert(take, d, ri);
+                    ++c.chains;
+                }
             }
             c.inserts += take ? 1 : 0;
         };
