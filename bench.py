@@ -237,7 +237,7 @@ def main():
         del res2, dst2, feats, seeds
     if args.sweep_buckets:
         for tk in (ft.TOPK_REGISTERS, ft.TOPK_LDS_HEAP):
-            for bk in (8, 16, 24, 32, 48, 64):
+            for bk in (16, 32, 48, 64):
                 ev[0].record(stream)
                 ft.self_knn_query(tree, k, results=res, dists=dst, stream=stream.handle,
                                   algorithm=ft.ALGO_PACKET, leaf_bucket=bk, topk=tk)
