@@ -244,6 +244,13 @@ def main():
                 ev[1].record(stream)
                 stream.synchronize()
                 log(f"topk {tk} bucket {bk}: {ev[0].elapsed_ms(ev[1]):.2f} ms")
+        for gw in (16, 32, 64):
+            ev[0].record(stream)
+            ft.self_knn_query(tree, k, results=res, dists=dst, stream=stream.handle,
+                              algorithm=ft.ALGO_PACKET, group_width=gw)
+            ev[1].record(stream)
+            stream.synchronize()
+            log(f"group width {gw}: {ev[0].elapsed_ms(ev[1]):.2f} ms")
         for wn in (-1, 32, 64, 128, 256):
             ev[0].record(stream)
             ft.self_knn_query(tree, k, results=res, dists=dst, stream=stream.handle,

@@ -16,7 +16,8 @@
 //   4. re-runs exactly, one lane per query, every query whose packed keys
 //      could not order its results.
 // The autotuner times candidate settings per (tree-size class, k tier) on the
-// bound data -- tier and leaf bucket for the queries; per size class the
+// bound data -- tier, leaf bucket and packet width (queries per wavefront
+// walk: 64, 32 or 16, the reference's sub-group width) for the queries; per size class the
 // radix width of the builds' sort; the branch count of the builds -- and
 // keeps the fastest.  Results are canonical (ascending (d2,
 // index)), so every setting gives the same output.
@@ -46,6 +47,7 @@ struct BatchReq {
     int32_t tree, k;
     int32_t entry_first, num;  // entries [entry_first, entry_first + num): sorted positions too
     int32_t bucket, variant;   // tuned leaf bucket; tier kernel that runs it
+    int32_t width, pad;        // queries per item (packet tiers: 16, 32 or 64)
     int64_t query_first;       // first query point
     int64_t result_first;      // first result slot
 };
@@ -60,6 +62,7 @@ template <typename T> struct BatchView {
     typename CoordTraits<T>::D *dists;
     int32_t *rerun;  // sorted positions to re-run exactly
     uint32_t *rerun_count;
+    uint32_t rerun_cap;  // entries the list holds (appends beyond it are dropped)
     uint64_t *stats;
 };
 
@@ -175,8 +178,9 @@ __global__ __launch_bounds__(64) void batch_packet_kernel(BatchView<T> b,
         const int j = uni((int)w.y);
         const DTree<T> t = b.trees[q.tree];
         const int k = q.k;
-        const int e = j * 64 + lane;
-        const bool active = e < q.num;
+        const int W = q.width;
+        const int e = j * W + lane;
+        const bool active = lane < W && e < q.num;
         int local = 0;
         T px = T(0), py = T(0), pz = T(0);
         if (active) {
@@ -188,7 +192,7 @@ __global__ __launch_bounds__(64) void batch_packet_kernel(BatchView<T> b,
         }
         // window pre-scan around the leaf a greedy descent reaches for the
         // packet's middle query (results do not depend on it)
-        const int src = (j * 64 + 32 < q.num) ? 32 : 0;
+        const int src = (j * W + W / 2 < q.num) ? W / 2 : 0;
         const int r0 = run_start(descend_rank<T, MK>(t, rl(px, src), rl(py, src), rl(pz, src)), t.n);
         const int wlo = r0 - BATCH_WINDOW > 0 ? r0 - BATCH_WINDOW : 0;
         const int whi = r0 + 64 + BATCH_WINDOW < t.n ? r0 + 64 + BATCH_WINDOW : t.n;
@@ -227,7 +231,7 @@ __global__ __launch_bounds__(64) void batch_packet_kernel(BatchView<T> b,
         }
         if (active && redo) {
             const uint32_t s2 = atomicAdd(b.rerun_count, 1u);
-            b.rerun[s2] = q.entry_first + e;
+            if (s2 < b.rerun_cap) b.rerun[s2] = q.entry_first + e;
         }
     }
     if (lane == 0 && b.stats) {
@@ -299,7 +303,7 @@ __global__ __launch_bounds__(64) void batch_exact_kernel(BatchView<T> b, int lan
     int32_t *hi = (int32_t *)(hd + (size_t)kcap * lanes);
     int32_t *stk = hi + (size_t)kcap * lanes;
     const int lane = threadIdx.x;
-    const int nlist = (int)*b.rerun_count;
+    const int nlist = (int)min(*b.rerun_count, b.rerun_cap);
     uint64_t visits = 0, drops = 0, inserts = 0;
     for (int base = blockIdx.x * lanes; base < nlist; base += gridDim.x * lanes) {
         const int slot = base + lane;
@@ -366,7 +370,7 @@ static const int kTierCap[8] = {1, 4, 8, 16, 32, 64, 128, 256};
 constexpr int N_CLASS = 4, N_KTIER = 8;
 
 struct GroupTune {
-    int variant = -1, bucket = 32;
+    int variant = -1, bucket = 32, width = 64;
     double ms_per_item = 0.0;  // measured cost of one item (load balancing)
     bool tuned = false;
 };
@@ -478,6 +482,7 @@ static void default_tuning(ft_batch *b) {
             if (g.tuned) continue;
             g.variant = candidates(t, fp).front();
             g.bucket = 32;
+            g.width = 64;
         }
 }
 
@@ -512,10 +517,12 @@ template <typename T> static int upload_tables(ft_batch *b) {
         q.num = (int32_t)r.num_queries;
         q.bucket = g.bucket;
         q.variant = g.variant;
+        q.width = is_packet(g.variant) ? g.width : b->lane_width;
+        q.pad = 0;
         q.query_first = r.query_first;
         q.result_first = r.result_first;
         entry += r.num_queries;
-        const int pw = is_packet(g.variant) ? 64 : b->lane_width;
+        const int pw = q.width;
         const double cost = g.ms_per_item > 0 ? g.ms_per_item : (double)r.k * std::log2(2.0 + n);
         if (n == 0) continue;  // an empty tree: results untouched, as knn_query
         for (int j = 0; j * pw < r.num_queries; ++j)
@@ -601,6 +608,7 @@ template <typename T> static BatchView<T> view(ft_batch *b) {
     v.dists = (typename CoordTraits<T>::D *)b->dists;
     v.rerun = b->drerun;
     v.rerun_count = b->dctr + V_COUNT * FT_XCDS;
+    v.rerun_cap = (uint32_t)std::max<int64_t>(b->n_entries, 1);
     v.stats = b->stats;
     return v;
 }
@@ -815,12 +823,29 @@ static float elapsed(hipEvent_t a, hipEvent_t c) {
 }
 
 // Autotuner.  Builds once, sorts the queries once, then for every (size class,
-// k tier) group with work times each candidate tier kernel (and leaf bucket for
-// packet tiers) on that group's items alone, keeping the fastest; then, per
+// k tier) group with work times each candidate tier kernel (and, for packet
+// tiers, leaf bucket and then packet width) on that group's items alone,
+// keeping the fastest; then, per
 // size class, that class's builds with 8- and 9-bit sort digits; then the
 // build phase at 1, 2, 4 and 8 branches.
+// One timed setting of a group: tier kernel, leaf bucket, queries per item;
+// the time of the group's items alone and their count.
+struct Setting {
+    int variant, bucket, width;
+    float ms;
+    int items;
+};
+static void apply_setting(GroupTune &g, const Setting &st) {
+    g.variant = st.variant;
+    g.bucket = st.bucket;
+    g.width = st.width;
+    g.ms_per_item = st.ms / std::max(st.items, 1);
+    g.tuned = true;
+}
+
 template <typename T> static int tune_t(ft_batch *b, hipStream_t s) {
     const bool fp = CoordTraits<T>::fp;
+    std::vector<Setting> alts[N_CLASS][N_KTIER];
     default_tuning(b);
     FT_TRY(upload_tables<T>(b));
     FT_TRY(enqueue_builds<T>(b, s, b->branches));
@@ -829,8 +854,8 @@ template <typename T> static int tune_t(ft_batch *b, hipStream_t s) {
     hipEvent_t e0, e1;
     FT_HIP(hipEventCreate(&e0));
     FT_HIP(hipEventCreate(&e1));
-    // Items of every group, under a fixed packet width per tier (64 for
-    // packet tiers, lane_width for the per-lane tier).
+    // Items of every group: packet tiers at 64, 32 or 16 queries per item,
+    // the per-lane tier at lane_width.
     const int nr = (int)b->reqs.size();
     int rc = FT_OK;
     for (int c = 0; c < N_CLASS && rc == FT_OK; ++c) {
@@ -843,20 +868,21 @@ template <typename T> static int tune_t(ft_batch *b, hipStream_t s) {
                     members.push_back(i);
             }
             if (members.empty()) continue;
-            double best = 1e30;
-            int best_v = -1, best_bucket = 32, best_items = 1;
-            for (int v : candidates(kt, fp)) {
-                const int pw = is_packet(v) ? 64 : b->lane_width;
+            // Times variant v at (bucket, width) on the group's items alone;
+            // returns the time of the better of two runs, or a negative value
+            // after an error (rc set).
+            auto time_setting = [&](int v, int bk, int pw, int &n_items) -> float {
                 std::vector<uint2> its;
                 for (int i : members)
                     for (int j = 0; j * pw < b->reqs[i].num_queries; ++j)
                         its.push_back(make_uint2((unsigned)i, (unsigned)j));
+                n_items = (int)its.size();
                 // scratch item list at the end of the item table's region is not
                 // guaranteed large enough: use a temporary device buffer
                 uint2 *tmp = nullptr;
                 if (hipMalloc(&tmp, its.size() * sizeof(uint2)) != hipSuccess) {
                     rc = fail(FT_ENOMEM, "tuner allocation failed");
-                    break;
+                    return -1.0f;
                 }
                 (void)hipMemcpy(tmp, its.data(), its.size() * sizeof(uint2), hipMemcpyHostToDevice);
                 // one chunk holding every item: all XCDs drain it
@@ -864,46 +890,123 @@ template <typename T> static int tune_t(ft_batch *b, hipStream_t s) {
                 tb[0] = 0;
                 uint32_t *tbnd = b->dbnd + V_COUNT * (FT_XCDS + 1);
                 (void)hipMemcpy(tbnd, tb.data(), 4 * tb.size(), hipMemcpyHostToDevice);
-                const std::vector<int> buckets =
-                    is_packet(v) ? std::vector<int>{16, 32, 64} : std::vector<int>{32};
-                for (int bk : buckets) {
-                    // set the bucket of the group's requests on the device
-                    for (int i : members) {
-                        BatchReq q;
-                        (void)hipMemcpy(&q, b->dreqs + i, sizeof q, hipMemcpyDeviceToHost);
-                        q.bucket = bk;
-                        q.variant = v;
-                        (void)hipMemcpy(b->dreqs + i, &q, sizeof q, hipMemcpyHostToDevice);
-                    }
-                    // null-stream copies above, before the timed launches on s
-                    (void)hipDeviceSynchronize();
-                    float t_best = 1e30f;
-                    for (int rep = 0; rep < 2 && rc == FT_OK; ++rep) {
-                        hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(128), 0, s, b->dctr,
-                                           V_COUNT * FT_XCDS + 1);
-                        (void)hipEventRecord(e0, s);
-                        rc = launch_variant<T>(b, s, v, tmp, (int)its.size(), tbnd, b->dctr);
-                        (void)hipEventRecord(e1, s);
-                        (void)hipEventSynchronize(e1);
-                        t_best = std::min(t_best, elapsed(e0, e1));
-                    }
-                    if (rc != FT_OK) break;
-                    if (t_best < best) {
-                        best = t_best;
-                        best_v = v;
-                        best_bucket = bk;
-                        best_items = (int)its.size();
-                    }
+                // set the group's requests on the device
+                for (int i : members) {
+                    BatchReq q;
+                    (void)hipMemcpy(&q, b->dreqs + i, sizeof q, hipMemcpyDeviceToHost);
+                    q.bucket = bk;
+                    q.variant = v;
+                    q.width = pw;
+                    (void)hipMemcpy(b->dreqs + i, &q, sizeof q, hipMemcpyHostToDevice);
+                }
+                // null-stream copies above, before the timed launches on s
+                (void)hipDeviceSynchronize();
+                float t_best = 1e30f;
+                for (int rep = 0; rep < 2 && rc == FT_OK; ++rep) {
+                    hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(128), 0, s, b->dctr,
+                                       V_COUNT * FT_XCDS + 1);
+                    (void)hipEventRecord(e0, s);
+                    rc = launch_variant<T>(b, s, v, tmp, (int)its.size(), tbnd, b->dctr);
+                    (void)hipEventRecord(e1, s);
+                    (void)hipEventSynchronize(e1);
+                    t_best = std::min(t_best, elapsed(e0, e1));
                 }
                 (void)hipFree(tmp);
+                return rc == FT_OK ? t_best : -1.0f;
+            };
+            std::vector<Setting> found;  // every timed setting of the group
+            for (int v : candidates(kt, fp)) {
+                // packet tiers: leaf buckets at 64 queries per packet, then
+                // 32 and 16 queries per packet with the best of those buckets
+                const std::vector<int> buckets =
+                    is_packet(v) ? std::vector<int>{16, 32, 64} : std::vector<int>{32};
+                int v_bucket = buckets.front();
+                float v_best = 1e30f;
+                for (int bk : buckets) {
+                    int ni = 1;
+                    const int pw = is_packet(v) ? 64 : b->lane_width;
+                    const float tm = time_setting(v, bk, pw, ni);
+                    if (rc != FT_OK) break;
+                    found.push_back({v, bk, pw, tm, ni});
+                    if (tm < v_best) {
+                        v_best = tm;
+                        v_bucket = bk;
+                    }
+                }
+                if (rc != FT_OK) break;
+                if (!is_packet(v)) continue;
+                for (int pw : {32, 16}) {
+                    int ni = 1;
+                    const float tm = time_setting(v, v_bucket, pw, ni);
+                    if (rc != FT_OK) break;
+                    found.push_back({v, v_bucket, pw, tm, ni});
+                }
                 if (rc != FT_OK) break;
             }
+            if (rc != FT_OK || found.empty()) break;
+            std::stable_sort(found.begin(), found.end(),
+                             [](const Setting &x, const Setting &y) { return x.ms < y.ms; });
+            // the fastest of each tier kernel, fastest first: the whole-step
+            // stage below tries the runners-up
+            std::vector<Setting> &alt = alts[c][kt];
+            alt.clear();
+            for (const Setting &st : found) {
+                bool seen = false;
+                for (const Setting &a : alt) seen = seen || a.variant == st.variant;
+                if (!seen) alt.push_back(st);
+            }
+            // and the fastest tier at 64 queries per packet, when it won narrower
+            if (alt.front().width != 64 && is_packet(alt.front().variant))
+                for (const Setting &st : found)
+                    if (st.variant == alt.front().variant && st.width == 64) {
+                        alt.insert(alt.begin() + 1, st);
+                        break;
+                    }
+            apply_setting(b->tune[c][kt], alt.front());
+        }
+    }
+    // Whole-step refinement.  Each group was timed alone, on an otherwise idle
+    // device, where a tier can win by parallelism alone (narrow packets) or
+    // lose what it would gain by sharing the device (the per-lane tier leans
+    // on LDS, the packet tiers on registers).  The query phase of the step,
+    // every tier running at once, is timed with the isolated choices, then,
+    // group by group (largest measured cost first), with each of the group's
+    // other tiers; a change stays when it makes the phase faster.
+    if (rc == FT_OK) {
+        auto time_queries = [&]() -> float {
+            float tb = 1e30f;
+            if ((rc = upload_tables<T>(b)) != FT_OK) return tb;
+            for (int rep = 0; rep < 2 && rc == FT_OK; ++rep) {
+                (void)hipEventRecord(e0, s);
+                rc = enqueue_queries<T>(b, s);
+                (void)hipEventRecord(e1, s);
+                (void)hipEventSynchronize(e1);
+                tb = std::min(tb, elapsed(e0, e1));
+            }
+            return tb;
+        };
+        std::vector<std::pair<double, int>> order;
+        for (int c = 0; c < N_CLASS; ++c)
+            for (int kt = 0; kt < N_KTIER; ++kt)
+                if (!alts[c][kt].empty()) order.push_back({-(double)alts[c][kt].front().ms, c * N_KTIER + kt});
+        std::sort(order.begin(), order.end());
+        float t_cur = time_queries();
+        for (const auto &o : order) {
             if (rc != FT_OK) break;
-            GroupTune &g = b->tune[c][kt];
-            g.variant = best_v;
-            g.bucket = best_bucket;
-            g.ms_per_item = best / std::max(best_items, 1);
-            g.tuned = true;
+            const int c = o.second / N_KTIER, kt = o.second % N_KTIER;
+            GroupTune keep = b->tune[c][kt];
+            for (size_t a = 1; a < alts[c][kt].size() && a < 4 && rc == FT_OK; ++a) {
+                const GroupTune prev = b->tune[c][kt];
+                apply_setting(b->tune[c][kt], alts[c][kt][a]);
+                const float t = time_queries();
+                if (t < 0.99f * t_cur) {
+                    t_cur = t;
+                    keep = b->tune[c][kt];
+                } else {
+                    b->tune[c][kt] = prev;
+                }
+            }
+            b->tune[c][kt] = keep;
         }
     }
     // Radix width per size class: that class's builds, in order on s, with
@@ -1164,9 +1267,10 @@ int ft_batch_tuning(ft_batch *b, char *buf, size_t len) {
             char line[256];
             std::snprintf(line, sizeof line,
                           "%s{\"n_class\": \"%s\", \"k_max\": %d, \"tier\": \"%s\", "
-                          "\"bucket\": %d, \"ms_per_item\": %.6f}",
+                          "\"bucket\": %d, \"queries_per_item\": %d, \"ms_per_item\": %.6f}",
                           first ? "" : ", ", cls[c], kTierCap[t], kVariantName[g.variant],
-                          g.bucket, g.ms_per_item);
+                          g.bucket, is_packet(g.variant) ? g.width : b->lane_width,
+                          g.ms_per_item);
             out += line;
             first = false;
         }

@@ -301,6 +301,7 @@ __global__ __launch_bounds__(64) void merge_knn_kernel(DTree<T> t, QSrc<T> qs, i
 // ===========================================================================
 struct PacketArgs {
     int nq, k, excl, bucket, ibits;
+    int width;   // queries per packet (16, 32 or 64): lanes at or above it idle
     int window;  // self queries over all leaves: leaves pre-scanned on each side
     uint64_t *stats;
 };
@@ -308,7 +309,7 @@ struct PacketArgs {
 // MULTI (register tier, packed keys, k > M): ceil(k / M) passes, a separate
 // instantiation so that the single-pass kernels keep their register budget.
 template <typename T, int M, bool HEAP, bool SEEDED, bool PACKED, int MK, bool MULTI = false>
-__global__ __launch_bounds__(64) void packet_knn_kernel(
+__global__ __launch_bounds__(64, (PACKED && !MULTI) ? 3 : 2) void packet_knn_kernel(
     DTree<T> t, QSrc<T> qs, PacketArgs a, QueryMap qm,
     const typename CoordTraits<T>::D *__restrict__ seed, uint64_t *__restrict__ res,
     typename CoordTraits<T>::D *__restrict__ dists) {
@@ -327,9 +328,10 @@ __global__ __launch_bounds__(64) void packet_knn_kernel(
     const int k = a.k;
     PacketCounters c;
     const int blk = xcd_block();
-    if (blk * 64 >= a.nq) return;  // padding block of the XCD-ordered grid
-    const int slot = blk * 64 + lane;
-    const bool active = slot < a.nq;
+    const int W = a.width;
+    if (blk * W >= a.nq) return;  // padding block of the XCD-ordered grid
+    const int slot = blk * W + lane;
+    const bool active = lane < W && slot < a.nq;
     const int q = active ? slot_query(qm, slot) : 0;
     T px = T(0), py = T(0), pz = T(0);
     int self = -1;
@@ -343,9 +345,9 @@ __global__ __launch_bounds__(64) void packet_knn_kernel(
     // middle lane's leaf rank, or around the leaf a greedy descent reaches
     // for the middle lane's point.
     int wlo = 0, whi = 0;
-    int r0 = blk * 64;
+    int r0 = blk * W;
     if (a.window > 0) {
-        const int src = (blk * 64 + 32 < a.nq) ? 32 : 0;
+        const int src = (blk * W + W / 2 < a.nq) ? W / 2 : 0;
         if (qs.qx) r0 = run_start(descend_rank<T, MK>(t, rl(px, src), rl(py, src), rl(pz, src)), t.n);
         else if (qs.leaf_ids) r0 = run_start(uni(__shfl(rank, src, 64)), t.n);
         wlo = r0 - a.window > 0 ? r0 - a.window : 0;
@@ -998,7 +1000,7 @@ template <typename T, int M, bool HEAP, bool SEEDED, bool PACKED, int MK>
 static void launch_pk(hipStream_t s, const DTree<T> &dt, const QSrc<T> &qs, const PacketArgs &a,
                       const QueryMap &qm, const typename CoordTraits<T>::D *seed, uint64_t *res,
                       typename CoordTraits<T>::D *d) {
-    const dim3 g(xcd_grid(blocks_for(a.nq, 64)));
+    const dim3 g(xcd_grid(blocks_for(a.nq, a.width)));
     if constexpr (!HEAP && M == 32) {
         if (a.k > M) {
             hipLaunchKernelGGL((packet_knn_kernel<T, M, HEAP, SEEDED, PACKED, MK, true>), g,
@@ -1026,7 +1028,7 @@ static void launch_packet_m(hipStream_t s, const DTree<T> &dt, const QSrc<T> &qs
 // the flags, and the exact listed re-run.  No host round trip.
 template <typename T, int MK>
 static int run_canonical(hipStream_t s, const DTree<T> &dt, const QSrc<T> &qs, int nq, int k,
-                         int excl, int algo, int bucket, int topk, int window,
+                         int excl, int algo, int bucket, int topk, int window, int width,
                          const uint32_t *order,
                          const typename CoordTraits<T>::D *seed, uint64_t *res,
                          typename CoordTraits<T>::D *d, uint64_t *stats, const RerunBuf *rr) {
@@ -1045,6 +1047,7 @@ static int run_canonical(hipStream_t s, const DTree<T> &dt, const QSrc<T> &qs, i
         a.k = k;
         a.excl = excl;
         a.bucket = bucket;
+        a.width = width;
         a.ibits = bit_length((uint64_t)dt.n);
         // A subset packet's leaves spread over about n / nq times more ranks:
         // its window grows with that spread, up to 4x (a 20 % subset of 20M
@@ -1180,13 +1183,17 @@ static int knn_dispatch(hipStream_t s, const ft_tree *tree, const QSrc<T> &qs, i
     // auto: 128 leaves on each side (at 10M clustered points 64 / 128 / 192 /
     // 256 / 384 took 28.7 / 28.55 / 28.5 / 28.6 / 29.0 ms)
     const int window = o.window < 0 ? 0 : (o.window == 0 ? 128 : std::min(o.window, 1 << 20));
+    // queries per packet: 64 (one per lane) unless 16 or 32 is asked for
+    if (o.group_width != 0 && o.group_width != 16 && o.group_width != 32 && o.group_width != 64)
+        return fail(FT_EINVAL, "group_width must be 0, 16, 32 or 64");
+    const int width = o.group_width ? o.group_width : 64;
     switch (metric_kind<T>(dt.m)) {
     case MK_WRAP:
         return run_canonical<T, MK_WRAP>(s, dt, qs, nq, k, excl, algo, bucket, o.topk, window,
-                                         order, seed, res, dists, stats, rr);
+                                         width, order, seed, res, dists, stats, rr);
     case MK_OPEN:
         return run_canonical<T, MK_OPEN>(s, dt, qs, nq, k, excl, algo, bucket, o.topk, window,
-                                         order, seed, res, dists, stats, rr);
+                                         width, order, seed, res, dists, stats, rr);
     default:
         return fail(FT_EINVAL, "unsupported metric: periodic integer coordinates need B <= 31");
     }

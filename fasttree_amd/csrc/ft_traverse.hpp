@@ -749,13 +749,14 @@ __device__ __forceinline__ void packet_walk(const DTree<T> &t, int bucket, bool 
             ez = (uint32_t)bz << wsh;
             const float wf = wrap_filter_bound_pre((uint64_t)w, wsh);
             [[maybe_unused]] const float lf = wrap_filter_floor_pre((uint64_t)lo_d, wsh);
-            auto near = [&](int j) -> bool {
+            auto near = [&](int j) -> uint32_t {
                 const float d = wrap_sq_pre(qxs, qys, qzs, rl(ex, j), rl(ey, j), rl(ez, j));
-                if constexpr (FLOOR) return d <= wf && d >= lf;
-                else return d <= wf;
+                if constexpr (FLOOR) return (d <= wf && d >= lf) ? 1u : 0u;
+                else return d <= wf ? 1u : 0u;
             };
-            for (int j = 0; j < c0; ++j) clo = near(j) ? (clo | (1u << j)) : clo;
-            for (int j = 32; j < cnt; ++j) chi = near(j) ? (chi | (1u << (j - 32))) : chi;
+            // leaf j's bit shifted in from the top, leaves taken last to first
+            for (int j = c0 - 1; j >= 0; --j) clo = (clo << 1) | near(j);
+            for (int j = cnt - 1; j >= 32; --j) chi = (chi << 1) | near(j);
         } else if constexpr (FPF) {
             // float32 pre-filter of double coordinates (fpf_*): never rejects
             // a leaf within w; the insertion below tests every passing leaf
@@ -764,14 +765,14 @@ __device__ __forceinline__ void packet_walk(const DTree<T> &t, int bucket, bool 
             const float wf = fpf_upper((double)w, fE);
             [[maybe_unused]] const float lf = FLOOR ? fpf_lower((double)lo_d, fE) : 0.0f;
             constexpr bool WR = MK == MK_WRAP;
-            auto near = [&](int j) -> bool {
+            auto near = [&](int j) -> uint32_t {
                 const float d = fpf_sq(fpf_gap(fqx, rl(fbx, j), fL, WR), fpf_gap(fqy, rl(fby, j), fL, WR),
                                        fpf_gap(fqz, rl(fbz, j), fL, WR));
-                if constexpr (FLOOR) return d <= wf && d >= lf;
-                else return d <= wf;
+                if constexpr (FLOOR) return (d <= wf && d >= lf) ? 1u : 0u;
+                else return d <= wf ? 1u : 0u;
             };
-            for (int j = 0; j < c0; ++j) clo |= near(j) ? (1u << j) : 0u;
-            for (int j = 32; j < cnt; ++j) chi |= near(j) ? (1u << (j - 32)) : 0u;
+            for (int j = c0 - 1; j >= 0; --j) clo = (clo << 1) | near(j);
+            for (int j = cnt - 1; j >= 32; --j) chi = (chi << 1) | near(j);
         } else {
             auto near = [&](int j) -> bool {
                 const D d = leaf_d2_k<T, MK>(t.m, px, py, pz, rl(bx, j), rl(by, j), rl(bz, j));

@@ -116,7 +116,9 @@ typedef struct ft_query_opts {
                          /* point's reference default (32 self, 256 subset/knn)          */
     int32_t mode;        /* FT_MODE_*                                                   */
     int32_t algorithm;   /* FT_ALGO_*                                                   */
-    int32_t group_width; /* lanes that share a traversal (FT_ALGO_PACKET: 64): 0 = auto   */
+    int32_t group_width; /* FT_ALGO_PACKET: queries that share one walk (16, 32 or 64; */
+                         /* 0 = 64), the reference's sub-group width W, one wavefront   */
+                         /* per group; other algorithms walk per lane.  Same results.   */
     int32_t leaf_bucket; /* FT_ALGO_PACKET: subtrees with at most this many leaves are    */
                          /* scanned as one contiguous run of leaf slots (1..64, 0 = auto) */
     int32_t topk;        /* FT_TOPK_*: top-k container of the packet traversal            */

@@ -104,6 +104,36 @@ def test_self_knn_subset(ft, orc, algo):
             assert np.array_equal(d, db) and np.array_equal(r, rb)
 
 
+@pytest.mark.parametrize("width", [16, 32])
+def test_packet_group_width(ft, orc, width):
+    """Packets of 16 or 32 queries (group_width; the reference's sub-group
+    width W) return the brute-force rows for self, subset and external queries,
+    k in the register tier, in passes of 32 and with a seed."""
+    pts = nfw_points(20000, 17, O.CT_U64)
+    tree, t = make(ft, O.CT_U64, O.SFC_PH, 21, 1, pts)
+    for k in (1, 16, 32, 64):
+        r, d = ft.self_knn_query(tree, k, group_width=width, max_k=64)
+        rb, db = O.knn_bruteforce(t, k, True)
+        assert np.array_equal(d, db) and np.array_equal(r, rb), k
+    ids = np.random.default_rng(3).choice(20000, 3001, replace=False).astype(np.int32)
+    r, d = ft.self_knn_query_subset(tree, ids, 32, group_width=width)
+    rb, db = O.knn_bruteforce(t, 32, True, leaf_ids=ids)
+    assert np.array_equal(d, db) and np.array_equal(r, rb)
+    rng = np.random.default_rng(5)
+    q = tuple(rng.integers(0, 1 << 21, 999, dtype=np.uint64) for _ in range(3))
+    r, d = ft.knn_query(tree, *q, 8, group_width=width)
+    rb, db = O.knn_bruteforce(t, 8, queries=q)
+    assert np.array_equal(d, db) and np.array_equal(r, rb)
+    _, d32 = O.knn_bruteforce(t, 16, True)
+    seed = ft.DeviceArray.from_host((d32[:, -1] * np.where(np.arange(20000) % 4 == 0, 0.5, 1.5))
+                                    .astype(np.uint64))
+    r, d = ft.self_knn_query(tree, 16, group_width=width, seed_bound=seed)
+    rb, db = O.knn_bruteforce(t, 16, True)
+    assert np.array_equal(d, db) and np.array_equal(r, rb)
+    with pytest.raises(ValueError):
+        ft.self_knn_query(tree, 4, group_width=48)
+
+
 def test_external_knn(ft, orc):
     pts = nfw_points(20000, 7, O.CT_U64)
     tree, t = make(ft, O.CT_U64, O.SFC_PH, 21, 1, pts)
