@@ -260,7 +260,7 @@ def main():
             log(f"window {wn}: {ev[0].elapsed_ms(ev[1]):.2f} ms")
     if args.seed_probe:
         dk = dst.to_host().reshape(n, k)[:, k - 1].astype(np.float64)
-        for f in (1.0, 1.2, 1.5, 2.0, 0.9):
+        for f in (1.0, 1.2, 1.5, 2.0):
             sd = ft.DeviceArray.from_host(np.floor(dk * f).astype(np.uint64))
             st2 = ft.DeviceArray(8, np.uint64).zero()
             ev[0].record(stream)
