@@ -24,7 +24,7 @@ FT_OK, FT_EINVAL, FT_ERANGE, FT_ELDS, FT_ENOMEM, FT_EHIP = 0, 1, 2, 3, 4, 5
 COORD_F32, COORD_F64, COORD_U32, COORD_U64 = 0, 1, 2, 3
 SFC_MORTON, SFC_PEANO_HILBERT = 0, 1
 MODE_CANONICAL, MODE_REFERENCE = 0, 1
-ALGO_AUTO, ALGO_PLAIN, ALGO_GROUPED, ALGO_PACKET = 0, 1, 2, 3
+ALGO_AUTO, ALGO_PLAIN, ALGO_GROUPED, ALGO_PACKET, ALGO_WAVE = 0, 1, 2, 3, 4
 TOPK_AUTO, TOPK_REGISTERS, TOPK_LDS_HEAP = 0, 1, 2
 
 COORD_DTYPES = {COORD_F32: np.float32, COORD_F64: np.float64, COORD_U32: np.uint32,
@@ -96,7 +96,7 @@ EXPORTS = [
     "ft_knn_features", "ft_mlp_forward", "ft_mlp_grad", "ft_mlp_train_step",
     "ft_mlp_last_loss", "ft_mlp_seed", "ft_batch_create", "ft_batch_destroy",
     "ft_batch_set_trees", "ft_batch_set_queries", "ft_batch_set_stats", "ft_batch_tune",
-    "ft_batch_run", "ft_batch_capture", "ft_batch_launch", "ft_batch_tuning",
+    "ft_batch_run", "ft_batch_capture", "ft_batch_launch", "ft_batch_tuning", "ft_batch_set_tier",
 ]
 
 _lib = None

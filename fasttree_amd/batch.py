@@ -43,6 +43,7 @@ def _bind():
     L.ft_batch_capture.argtypes = [vp, vp]
     L.ft_batch_launch.argtypes = [vp, vp]
     L.ft_batch_tuning.argtypes = [vp, C.c_char_p, C.c_size_t]
+    L.ft_batch_set_tier.argtypes = [vp, i32, i32, C.c_char_p, i32, i32]
     _bound = True
 
 
@@ -87,6 +88,13 @@ class Batch:
 
     def set_stats(self, stats: DeviceArray | None):
         _check(lib().ft_batch_set_stats(self.h, _ptr(stats)))
+
+    def set_tier(self, n_class: int, k_tier: int, tier: str, bucket: int = 0,
+                 queries_per_item: int = 0):
+        """Pins one (tree-size class, k tier) group's tier kernel (a name as
+        tuning() reports it), leaf bucket and queries per item; tune() keeps it."""
+        _check(lib().ft_batch_set_tier(self.h, int(n_class), int(k_tier), tier.encode(),
+                                       int(bucket), int(queries_per_item)))
 
     def tune(self, stream=None):
         _check(lib().ft_batch_tune(self.h, _sh(stream)))

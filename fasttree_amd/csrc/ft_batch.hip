@@ -70,13 +70,18 @@ template <typename T> struct BatchView {
 // width M; HP_M: packet walk with an LDS heap of capacity M (integer metrics);
 // LANE: per-lane DFS with LDS heaps, one query per lane (any k, any metric).
 // PASSES: the 32-entry register tier in passes of 32 for k above 32 (integer
-// metrics).
+// metrics).  WAVE_C: one wavefront per query with a sorted list of C entries
+// across its lanes (k up to C; wave_walk).
 enum Variant : int {
-    V_PK1, V_PK4, V_PK8, V_PK16, V_PK32, V_HP32, V_HP64, V_HP128, V_LANE, V_PASSES, V_COUNT
+    V_PK1, V_PK4, V_PK8, V_PK16, V_PK32, V_HP32, V_HP64, V_HP128, V_LANE, V_PASSES,
+    V_WAVE64, V_WAVE128, V_WAVE256, V_COUNT
 };
 static const char *const kVariantName[V_COUNT] = {
     "packet-reg1",   "packet-reg4",    "packet-reg8", "packet-reg16",       "packet-reg32",
-    "packet-heap32", "packet-heap64", "packet-heap128", "lane-heap", "packet-reg32-passes"};
+    "packet-heap32", "packet-heap64", "packet-heap128", "lane-heap", "packet-reg32-passes",
+    "wave-merge64", "wave-merge128", "wave-merge256"};
+static bool is_wave(int v) { return v >= V_WAVE64 && v <= V_WAVE256; }
+static int wave_cap(int v) { return 64 << (v - V_WAVE64); }
 
 // ---------------------------------------------------------------------------
 // Kernels
@@ -293,6 +298,49 @@ __global__ __launch_bounds__(64) void batch_lane_kernel(BatchView<T> b,
     wave_add_stats(b.stats, 4, inserts);
 }
 
+// Wave tier: one wavefront per query, `width` consecutive sorted queries of a
+// request per item, each answered by wave_walk with a list of 64 S entries,
+// its first leaves around the leaf a greedy descent reaches.
+template <typename T, int S, int MK>
+__global__ __launch_bounds__(64) void batch_wave_kernel(BatchView<T> b,
+                                                        const uint2 *__restrict__ items,
+                                                        const uint32_t *__restrict__ bnd,
+                                                        uint32_t *__restrict__ ctr) {
+    const int lane = threadIdx.x;
+    WaveCounters c;
+    while (true) {
+        const int it = grab_item(ctr, bnd);
+        if (it < 0) break;
+        const uint2 w = items[it];
+        const BatchReq q = b.reqs[uni((int)w.x)];
+        const int j = uni((int)w.y);
+        const DTree<T> t = b.trees[q.tree];
+        const int e_end = (j + 1) * q.width < q.num ? (j + 1) * q.width : q.num;
+        for (int e = j * q.width; e < e_end; ++e) {
+            const int local = uni((int)b.order[q.entry_first + e] - q.entry_first);
+            const int64_t qi = q.query_first + local;
+            const T px = b.qx[qi], py = b.qy[qi], pz = b.qz[qi];
+            const int anchor = descend_rank<T, MK>(t, px, py, pz);
+            WaveList<S> L;
+            L.init(q.k);
+            if (wave_walk<T, S, MK>(t, px, py, pz, -1, anchor, L, c)) {
+                if (lane == 0) {
+                    const uint32_t s2 = atomicAdd(b.rerun_count, 1u);
+                    if (s2 < b.rerun_cap) b.rerun[s2] = q.entry_first + e;
+                }
+            } else {
+                L.store(q.k, b.res + q.result_first, b.dists + q.result_first, (size_t)local,
+                        lane);
+            }
+        }
+    }
+    if (lane == 0 && b.stats) {
+        atomicAdd((unsigned long long *)&b.stats[0], (unsigned long long)c.nodes);
+        atomicAdd((unsigned long long *)&b.stats[1], (unsigned long long)c.leaves);
+        atomicAdd((unsigned long long *)&b.stats[4], (unsigned long long)c.inserts);
+    }
+}
+
 // Exact re-run of listed sorted positions: one lane per query, each lane on its
 // own tree (found by binary search over the requests), lexicographic LDS heap.
 template <typename T>
@@ -373,7 +421,9 @@ struct GroupTune {
     int variant = -1, bucket = 32, width = 64;
     double ms_per_item = 0.0;  // measured cost of one item (load balancing)
     bool tuned = false;
+    bool pinned = false;  // set by the caller (ft_batch_set_tier): the tuner keeps it
 };
+constexpr int WAVE_ITEM = 8;  // wave tiers: queries per item
 
 }  // namespace ft
 
@@ -455,8 +505,9 @@ static int ensure_streams(ft_batch *b, int nb) {
     return FT_OK;
 }
 
-// Candidate variants for a group (class, k tier) and metric.
-static std::vector<int> candidates(int ktier, bool fp) {
+// Candidate variants for a group (class, k tier) and metric; kmax = the
+// largest k of the group's requests (tier 7 holds every k above 128).
+static std::vector<int> candidates(int ktier, bool fp, int kmax) {
     std::vector<int> v;
     const int cap = kTierCap[ktier];
     if (cap <= 32) v.push_back(V_PK1 + (ktier <= 4 ? ktier : 4));
@@ -467,10 +518,15 @@ static std::vector<int> candidates(int ktier, bool fp) {
         else if (cap <= 128) v.push_back(V_HP128);
     }
     v.push_back(V_LANE);
+    if (cap > 32 && kmax <= 256) v.push_back(cap <= 64 ? V_WAVE64 : (cap <= 128 ? V_WAVE128 : V_WAVE256));
     return v;
 }
 
-static bool is_packet(int v) { return v != V_LANE; }
+static bool is_packet(int v) { return v != V_LANE && !is_wave(v); }
+// Queries per item of a group's setting.
+static int item_width(int v, int width, int lane_width) {
+    return is_packet(v) ? width : (is_wave(v) ? width : lane_width);
+}
 
 // Default settings: register packets (in passes of 32 above k = 32 for integer
 // metrics), per-lane heaps for larger k with FP metrics; bucket 32.
@@ -479,8 +535,8 @@ static void default_tuning(ft_batch *b) {
     for (int c = 0; c < N_CLASS; ++c)
         for (int t = 0; t < N_KTIER; ++t) {
             GroupTune &g = b->tune[c][t];
-            if (g.tuned) continue;
-            g.variant = candidates(t, fp).front();
+            if (g.tuned || g.pinned) continue;
+            g.variant = candidates(t, fp, kTierCap[t]).front();
             g.bucket = 32;
             g.width = 64;
         }
@@ -516,8 +572,9 @@ template <typename T> static int upload_tables(ft_batch *b) {
         q.entry_first = (int32_t)entry;
         q.num = (int32_t)r.num_queries;
         q.bucket = g.bucket;
-        q.variant = g.variant;
-        q.width = is_packet(g.variant) ? g.width : b->lane_width;
+        // a wave list shorter than this request's k: the per-lane tier
+        q.variant = (is_wave(g.variant) && r.k > wave_cap(g.variant)) ? (int)V_LANE : g.variant;
+        q.width = item_width(q.variant, g.width, b->lane_width);
         q.pad = 0;
         q.query_first = r.query_first;
         q.result_first = r.result_first;
@@ -526,7 +583,7 @@ template <typename T> static int upload_tables(ft_batch *b) {
         const double cost = g.ms_per_item > 0 ? g.ms_per_item : (double)r.k * std::log2(2.0 + n);
         if (n == 0) continue;  // an empty tree: results untouched, as knn_query
         for (int j = 0; j * pw < r.num_queries; ++j)
-            items[g.variant].push_back({cost, make_uint2((unsigned)i, (unsigned)j)});
+            items[q.variant].push_back({cost, make_uint2((unsigned)i, (unsigned)j)});
     }
     b->n_entries = entry;
     std::vector<uint2> flat;
@@ -669,6 +726,15 @@ static int launch_variant_mk(ft_batch *b, hipStream_t s, int v, const uint2 *it,
         auto kf = batch_lane_kernel<T, MK>;
         hipLaunchKernelGGL(kf, dim3(persistent_waves(kf, lds, n_items)), dim3(64), lds, s, bv, it,
                            bnd, ctr, lanes, b->kmax);
+        break;
+    }
+    case V_WAVE64:
+    case V_WAVE128:
+    case V_WAVE256: {
+        auto kf = v == V_WAVE64 ? batch_wave_kernel<T, 1, MK>
+                                : (v == V_WAVE128 ? batch_wave_kernel<T, 2, MK> : batch_wave_kernel<T, 4, MK>);
+        hipLaunchKernelGGL(kf, dim3(persistent_waves(kf, 0, n_items)), dim3(64), 0, s, bv, it, bnd,
+                           ctr);
         break;
     }
     default: return fail(FT_EINVAL, "unknown batch tier");
@@ -868,6 +934,9 @@ template <typename T> static int tune_t(ft_batch *b, hipStream_t s) {
                     members.push_back(i);
             }
             if (members.empty()) continue;
+            int gk = 1;
+            for (int i : members) gk = std::max(gk, (int)b->reqs[i].k);
+            if (b->tune[c][kt].pinned) continue;
             // Times variant v at (bucket, width) on the group's items alone;
             // returns the time of the better of two runs, or a negative value
             // after an error (rc set).
@@ -915,7 +984,7 @@ template <typename T> static int tune_t(ft_batch *b, hipStream_t s) {
                 return rc == FT_OK ? t_best : -1.0f;
             };
             std::vector<Setting> found;  // every timed setting of the group
-            for (int v : candidates(kt, fp)) {
+            for (int v : candidates(kt, fp, gk)) {
                 // packet tiers: leaf buckets at 64 queries per packet, then
                 // 32 and 16 queries per packet with the best of those buckets
                 const std::vector<int> buckets =
@@ -924,7 +993,7 @@ template <typename T> static int tune_t(ft_batch *b, hipStream_t s) {
                 float v_best = 1e30f;
                 for (int bk : buckets) {
                     int ni = 1;
-                    const int pw = is_packet(v) ? 64 : b->lane_width;
+                    const int pw = is_packet(v) ? 64 : (is_wave(v) ? WAVE_ITEM : b->lane_width);
                     const float tm = time_setting(v, bk, pw, ni);
                     if (rc != FT_OK) break;
                     found.push_back({v, bk, pw, tm, ni});
@@ -1223,6 +1292,41 @@ int ft_batch_set_stats(ft_batch *b, uint64_t *stats) {
     return FT_OK;
 }
 
+int ft_batch_set_tier(ft_batch *b, int32_t n_class, int32_t k_tier, const char *tier,
+                      int32_t bucket, int32_t queries_per_item) {
+    if (!b || !tier) return fail(FT_EINVAL, "null batch or tier name");
+    if (n_class < 0 || n_class >= N_CLASS || k_tier < 0 || k_tier >= N_KTIER)
+        return fail(FT_EINVAL, "size class must be 0..3 and k tier 0..7");
+    int v = -1;
+    for (int i = 0; i < V_COUNT; ++i)
+        if (std::strcmp(tier, kVariantName[i]) == 0) v = i;
+    if (v < 0) return fail(FT_EINVAL, std::string("unknown batch tier: ") + tier);
+    const bool fp = b->coord_type == FT_COORD_F32 || b->coord_type == FT_COORD_F64;
+    const std::vector<int> ok = candidates(k_tier, fp, kTierCap[k_tier]);
+    if (std::find(ok.begin(), ok.end(), v) == ok.end())
+        return fail(FT_EINVAL, std::string("tier ") + tier + " does not serve this k tier");
+    if (bucket < 0 || bucket > 64) return fail(FT_EINVAL, "leaf bucket must be 0..64");
+    int w = queries_per_item;
+    if (is_packet(v)) {
+        if (w == 0) w = 64;
+        if (w != 16 && w != 32 && w != 64)
+            return fail(FT_EINVAL, "packet tiers take 16, 32 or 64 queries per item");
+    } else if (is_wave(v)) {
+        if (w == 0) w = WAVE_ITEM;
+        if (w < 1 || w > 64) return fail(FT_EINVAL, "wave tiers take 1..64 queries per item");
+    }
+    GroupTune &g = b->tune[n_class][k_tier];
+    g.variant = v;
+    g.bucket = bucket ? bucket : 32;
+    g.width = w;
+    g.ms_per_item = 0.0;
+    g.tuned = true;
+    g.pinned = true;
+    b->dirty = true;
+    free_graph(b);
+    return FT_OK;
+}
+
 int ft_batch_tune(ft_batch *b, ft_stream s) {
     if (!b) return fail(FT_EINVAL, "null batch");
     FT_BATCH_DISPATCH(b, tune_t<T>(b, as_stream(s)));
@@ -1269,7 +1373,7 @@ int ft_batch_tuning(ft_batch *b, char *buf, size_t len) {
                           "%s{\"n_class\": \"%s\", \"k_max\": %d, \"tier\": \"%s\", "
                           "\"bucket\": %d, \"queries_per_item\": %d, \"ms_per_item\": %.6f}",
                           first ? "" : ", ", cls[c], kTierCap[t], kVariantName[g.variant],
-                          g.bucket, is_packet(g.variant) ? g.width : b->lane_width,
+                          g.bucket, item_width(g.variant, g.width, b->lane_width),
                           g.ms_per_item);
             out += line;
             first = false;

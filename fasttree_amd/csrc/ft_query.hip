@@ -308,6 +308,45 @@ struct PacketArgs {
 
 // MULTI (register tier, packed keys, k > M): ceil(k / M) passes, a separate
 // instantiation so that the single-pass kernels keep their register budget.
+// ===========================================================================
+// Wave-per-query canonical tier (FT_ALGO_WAVE): one wavefront per query, its
+// sorted list of C = 64 S entries spread over the lanes and candidates merged
+// 64 leaves at a time by cross-lane bitonic networks (WaveList, wave_walk) --
+// the canonical counterpart of knn_query_large_k's SortedMergeHeap
+// (src/hlbvh.hpp:1232-1387, src/maxheap.hpp:226-324).  Exact in one walk for
+// k up to 1024.  Four queries per 256-thread block; self and subset queries
+// anchor their first leaves at their own rank, external ones at the leaf a
+// greedy descent reaches.
+// ===========================================================================
+template <typename T, int S, int MK>
+__global__ __launch_bounds__(256) void wave_knn_kernel(DTree<T> t, QSrc<T> qs, int nq, int k,
+                                                       int excl, QueryMap qm,
+                                                       uint64_t *__restrict__ res,
+                                                       typename CoordTraits<T>::D *__restrict__ dists,
+                                                       uint64_t *__restrict__ stats) {
+    const int lane = threadIdx.x & 63;
+    const int slot = uni(xcd_block() * 4 + (int)(threadIdx.x >> 6));
+    WaveCounters c;
+    if (slot < nq) {
+        const int q = uni(slot_query(qm, slot));
+        T px = T(0), py = T(0), pz = T(0);
+        int rank = -1;
+        load_query(t, qs, q, px, py, pz, rank);
+        const int self = excl ? rank : -1;
+        const int anchor = rank >= 0 ? rank : descend_rank<T, MK>(t, px, py, pz);
+        WaveList<S> L;
+        L.init(k);
+        if (wave_walk<T, S, MK>(t, px, py, pz, self, anchor, L, c)) {
+            if (lane == 0) qm.redo[slot] = 1;
+        } else {
+            L.store(k, res, dists, (size_t)q, lane);
+        }
+    }
+    wave_add_stats(stats, 0, lane == 0 ? c.nodes : 0);
+    wave_add_stats(stats, 1, lane == 0 ? c.leaves : 0);
+    wave_add_stats(stats, 4, lane == 0 ? c.inserts : 0);
+}
+
 template <typename T, int M, bool HEAP, bool SEEDED, bool PACKED, int MK, bool MULTI = false>
 __global__ __launch_bounds__(64, (PACKED && !MULTI) ? 3 : 2) void packet_knn_kernel(
     DTree<T> t, QSrc<T> qs, PacketArgs a, QueryMap qm,
@@ -1067,6 +1106,18 @@ static int run_canonical(hipStream_t s, const DTree<T> &dt, const QSrc<T> &qs, i
             else if (k <= 128) launch_packet_m<T, 128, true, MK>(s, dt, qs, a, qm, seed, res, d);
             else return fail(FT_EINVAL, "packet traversal: k above the heap tiers");
         }
+    } else if (algo == FT_ALGO_WAVE) {
+        const dim3 g(xcd_grid(blocks_for(nq, 4))), b(256);
+#define FT_WAVE_LAUNCH(S_)                                                                      \
+    hipLaunchKernelGGL((wave_knn_kernel<T, S_, MK>), g, b, 0, s, dt, qs, nq, k, excl, qm, res, d, \
+                       stats)
+        if (k <= 64) FT_WAVE_LAUNCH(1);
+        else if (k <= 128) FT_WAVE_LAUNCH(2);
+        else if (k <= 256) FT_WAVE_LAUNCH(4);
+        else if (k <= 512) FT_WAVE_LAUNCH(8);
+        else if (k <= WAVE_MAX_K) FT_WAVE_LAUNCH(16);
+        else return fail(FT_EINVAL, "wave traversal: k above 1024");
+#undef FT_WAVE_LAUNCH
     } else if (k <= 32) {
         const dim3 g(xcd_grid(blocks_for(nq, LANE_BLOCK))), b(LANE_BLOCK);
         if (algo == FT_ALGO_PLAIN)
@@ -1178,7 +1229,8 @@ static int knn_dispatch(hipStream_t s, const ft_tree *tree, const QSrc<T> &qs, i
     const D *seed = algo == FT_ALGO_PACKET ? (const D *)o.seed_bound : nullptr;
     const int bucket = o.leaf_bucket > 0 ? std::min(o.leaf_bucket, 64) : 64;
     const uint32_t *order = nullptr;
-    if (algo == FT_ALGO_PACKET && (qs.qx || qs.leaf_ids) && nq > 64)
+    if (algo == FT_ALGO_WAVE && k > WAVE_MAX_K) algo = FT_ALGO_GROUPED;
+    if ((algo == FT_ALGO_PACKET || algo == FT_ALGO_WAVE) && (qs.qx || qs.leaf_ids) && nq > 64)
         FT_TRY(query_order<T>(s, dt, qs, nq, order));
     // auto: 128 leaves on each side (at 10M clustered points 64 / 128 / 192 /
     // 256 / 384 took 28.7 / 28.55 / 28.5 / 28.6 / 29.0 ms)

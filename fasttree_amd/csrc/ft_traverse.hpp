@@ -1201,4 +1201,331 @@ __device__ __forceinline__ bool lane_heap_walk(const DTree<T> &t, int k, bool ac
     return overflow;
 }
 
+
+// ===========================================================================
+// Wave-per-query tier: the canonical counterpart of SortedMergeHeap
+// (src/maxheap.hpp:226-324) in knn_query_large_k (src/hlbvh.hpp:1232-1387).
+// One wavefront answers one query.  Its list of C = 64 S entries
+// (d2, returned index), ascending, is spread over the lanes: entry
+// e = 64 s + lane is slot s of that lane.  The walk is wave-uniform (scalar
+// node loads; a 128-entry stack of (node, d2) held one entry per lane in
+// VGPRs), and a subtree of at most 64 leaves is tested one leaf per lane.
+// Leaves below the current k-th entry join the list:
+//  * a few (at most WaveList::kSparse): one at a time, each by shifting the
+//    entries above its place one position up (DPP wave_shr:1 inside a slot,
+//    a readlane from the slot below into lane 0);
+//  * more: the 64 candidates are bitonic-sorted across the lanes (DPP
+//    quad_perm for lane distances 1 and 2, ds_swizzle for 4 to 16,
+//    ds_bpermute for 32); the list's last 64 entries take the smaller of
+//    themselves and the reversed batch, which leaves the whole list bitonic
+//    and holding the C smallest of both; a bitonic merge (in-lane across
+//    slots for distances of 64 and more, cross-lane below) sorts it again.
+// Keys compare as (distance bits as u64, index as u32): non-negative doubles
+// order as their bit patterns, so one comparison serves both distance types.
+// The list is exact for every k <= C (no saturation, nothing to re-run).
+// ===========================================================================
+__device__ __forceinline__ bool wk_lt(uint64_t ad, uint32_t ai, uint64_t bd, uint32_t bi) {
+    return (ad < bd) | ((ad == bd) & (ai < bi));
+}
+template <typename D> __device__ __forceinline__ uint64_t dist_bits(D d) {
+    if constexpr (std::is_floating_point<D>::value) return (uint64_t)__double_as_longlong(d);
+    else return (uint64_t)d;
+}
+template <typename D> __device__ __forceinline__ D bits_dist(uint64_t b) {
+    if constexpr (std::is_floating_point<D>::value) return __longlong_as_double((long long)b);
+    else return (D)b;
+}
+
+// Lane i receives v from lane i ^ J.
+template <int J> __device__ __forceinline__ uint32_t lane_xor(uint32_t v) {
+    if constexpr (J == 1)
+        return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false);  // quad_perm 1,0,3,2
+    else if constexpr (J == 2)
+        return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, false);  // quad_perm 2,3,0,1
+    else if constexpr (J < 32)
+        return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, 0x1F | (J << 10));  // bit mode, xor J
+    else
+        return (uint32_t)__builtin_amdgcn_ds_bpermute((int)(((threadIdx.x & 63) ^ 32) << 2), (int)v);
+}
+// Lane i receives v from lane i - 1; lane 0 receives fill (DPP wave_shr:1).
+__device__ __forceinline__ uint32_t lane_shr1(uint32_t v, uint32_t fill) {
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)fill, (int)v, 0x138, 0xF, 0xF, false);
+}
+
+// Compare-exchange with lane ^ J: the lane keeps the larger key when want_max.
+template <int J>
+__device__ __forceinline__ void wk_cx(uint64_t &d, uint32_t &i, bool want_max) {
+    const uint64_t pd = ((uint64_t)lane_xor<J>((uint32_t)(d >> 32)) << 32) | lane_xor<J>((uint32_t)d);
+    const uint32_t pi = lane_xor<J>(i);
+    // equal keys (two empty entries) are the same value either way
+    const bool take = wk_lt(pd, pi, d, i) != want_max;
+    d = take ? pd : d;
+    i = take ? pi : i;
+}
+// Bitonic sort of one key per lane, ascending over the 64 lanes.
+template <int KK, int J>
+__device__ __forceinline__ void wsort_stage(uint64_t &d, uint32_t &i, int lane) {
+    wk_cx<J>(d, i, ((lane & J) != 0) != ((lane & KK) != 0));
+    if constexpr (J > 1) wsort_stage<KK, J / 2>(d, i, lane);
+}
+template <int KK> __device__ __forceinline__ void wsort_from(uint64_t &d, uint32_t &i, int lane) {
+    wsort_stage<KK, KK / 2>(d, i, lane);
+    if constexpr (KK < 64) wsort_from<KK * 2>(d, i, lane);
+}
+// Half-cleaners of lane distance J..1 (the cross-lane end of a bitonic merge).
+template <int J> __device__ __forceinline__ void wmerge_lanes(uint64_t &d, uint32_t &i, int lane) {
+    wk_cx<J>(d, i, (lane & J) != 0);
+    if constexpr (J > 1) wmerge_lanes<J / 2>(d, i, lane);
+}
+
+constexpr int WAVE_MAX_K = 1024;
+
+template <int S> struct WaveList {
+    static constexpr int C = 64 * S;
+    // Candidates of one leaf batch inserted one at a time up to this count;
+    // more take the sort-and-merge path.
+    static constexpr int kSparse = S == 1 ? 12 : 16;
+    uint64_t d[S];
+    uint32_t i[S];
+    uint64_t wd;  // the k-th entry (wave-uniform): pruning and admission bound
+    uint32_t wi;
+    int ks, kl;   // its slot and lane
+    __device__ inline void init(int k) {
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            d[s] = ~0ull;
+            i[s] = ~0u;
+        }
+        wd = ~0ull;
+        wi = ~0u;
+        ks = (k - 1) >> 6;
+        kl = (k - 1) & 63;
+    }
+    __device__ inline void refresh() {
+        uint64_t xd = d[0];
+        uint32_t xi = i[0];
+#pragma unroll
+        for (int s = 1; s < S; ++s) {
+            xd = ks == s ? d[s] : xd;
+            xi = ks == s ? i[s] : xi;
+        }
+        wd = rl(xd, kl);
+        wi = rl32(xi, kl);
+    }
+    __device__ inline bool admits(uint64_t cd, uint32_t ci) const { return wk_lt(cd, ci, wd, wi); }
+    // One wave-uniform key into its place; the last entry drops off.  Slots
+    // whose last entry is below the key keep their entries (so do all slots
+    // below them): the shift stops there.
+    __device__ inline void insert_one(uint64_t cd, uint32_t ci) {
+#pragma unroll
+        for (int s = S - 1; s >= 0; --s) {
+            // lane 0 of slot s receives entry 64 s - 1 (slot s - 1, lane 63,
+            // not yet shifted); entry 0 receives (0, 0), at or below every key
+            uint64_t fd = 0;
+            uint32_t fi = 0;
+            if (s > 0) {
+                fd = rl(d[s - 1], 63);
+                fi = rl32(i[s - 1], 63);
+            }
+            if (s < S - 1) {
+                const uint64_t ld = rl(d[s], 63);
+                const uint32_t li = rl32(i[s], 63);
+                if (wk_lt(ld, li, cd, ci)) break;
+            }
+            const uint64_t sd = ((uint64_t)lane_shr1((uint32_t)(d[s] >> 32), (uint32_t)(fd >> 32)) << 32) |
+                                lane_shr1((uint32_t)d[s], (uint32_t)fd);
+            const uint32_t si = lane_shr1(i[s], fi);
+            const bool keep = wk_lt(d[s], i[s], cd, ci);
+            const bool put = wk_lt(sd, si, cd, ci);
+            d[s] = keep ? d[s] : (put ? cd : sd);
+            i[s] = keep ? i[s] : (put ? ci : si);
+        }
+    }
+    // Merge one key per lane (non-candidates hold (~0, ~0)) into the list.
+    __device__ inline void merge_batch(uint64_t cd, uint32_t ci, int lane) {
+        wsort_from<2>(cd, ci, lane);
+        const int src = (63 - lane) << 2;
+        const uint64_t rd =
+            ((uint64_t)(uint32_t)__builtin_amdgcn_ds_bpermute(src, (int)(uint32_t)(cd >> 32)) << 32) |
+            (uint32_t)__builtin_amdgcn_ds_bpermute(src, (int)(uint32_t)cd);
+        const uint32_t ri = (uint32_t)__builtin_amdgcn_ds_bpermute(src, (int)ci);
+        const bool t = wk_lt(rd, ri, d[S - 1], i[S - 1]);
+        d[S - 1] = t ? rd : d[S - 1];
+        i[S - 1] = t ? ri : i[S - 1];
+#pragma unroll
+        for (int js = S / 2; js >= 1; js >>= 1) {
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                if ((s & js) == 0) {
+                    const bool sw = wk_lt(d[s + js], i[s + js], d[s], i[s]);
+                    const uint64_t ad = d[s], bd = d[s + js];
+                    const uint32_t ai = i[s], bi = i[s + js];
+                    d[s] = sw ? bd : ad;
+                    i[s] = sw ? bi : ai;
+                    d[s + js] = sw ? ad : bd;
+                    i[s + js] = sw ? ai : bi;
+                }
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < S; ++s) wmerge_lanes<32>(d[s], i[s], lane);
+    }
+    template <typename DOut>
+    __device__ inline void store(int k, uint64_t *res, DOut *dists, size_t q, int lane) const {
+        const size_t o = q * (size_t)k;
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            const int e = 64 * s + lane;
+            if (e < k) {
+                const bool ok = i[s] != ~0u;
+                res[o + e] = ok ? (uint64_t)i[s] : ~0ull;
+                dists[o + e] = ok ? bits_dist<DOut>(d[s]) : missing_dist<DOut>();
+            }
+        }
+    }
+};
+
+struct WaveCounters {
+    uint64_t nodes = 0, leaves = 0, inserts = 0;
+};
+
+// The wave's walk for one query (px, py, pz): self = leaf rank excluded from
+// the results (-1: none); anchor = a leaf rank near the query (-1: none).
+// The 64 (S + 1) leaves around the anchor are inserted first, so the walk
+// starts with a finite bound; it then skips them (and any subtree inside
+// them).  Returns true if the stack would overflow (never for a tree over
+// distinct (64-bit key, 32-bit index) strings, whose depth is below 97).
+template <typename T, int S, int MK>
+__device__ __forceinline__ bool wave_walk(const DTree<T> &t, T px, T py, T pz, int self,
+                                          int anchor, WaveList<S> &L, WaveCounters &c) {
+    using D = typename CoordTraits<T>::D;
+    const int lane = threadIdx.x & 63;
+    const int n = t.n;
+    const int off = n - 1;
+    int w0 = 0, w1 = 0;
+    // Tests leaves [r0, r0 + cnt) (cnt <= 64), one per lane, and inserts the
+    // ones the list admits.
+    auto scan = [&](int r0, int cnt, bool skip_win) {
+        const int r = r0 + lane;
+        bool ok = lane < cnt && r != self;
+        if (skip_win) ok = ok && (r < w0 || r >= w1);
+        const int leaf = off + (lane < cnt ? r : r0);
+        const D d2 = leaf_d2_k<T, MK>(t.m, px, py, pz, t.mnx[leaf], t.mny[leaf], t.mnz[leaf]);
+        uint64_t cd = dist_bits(d2);
+        uint32_t ci = (uint32_t)ret_index(t, leaf);
+        ok = ok && L.admits(cd, ci);
+        c.leaves += (uint64_t)cnt;
+        const uint64_t m = __ballot(ok);
+        if (!m) return;
+        const int cnt_in = __popcll(m);
+        c.inserts += (uint64_t)cnt_in;
+        if (cnt_in <= WaveList<S>::kSparse) {
+            uint64_t mm = m;
+            while (mm) {
+                const int j = __builtin_ctzll(mm);
+                mm &= mm - 1;
+                const uint64_t xd = rl(cd, j);
+                const uint32_t xi = rl32(ci, j);
+                if (L.admits(xd, xi)) {
+                    L.insert_one(xd, xi);
+                    L.refresh();
+                }
+            }
+        } else {
+            cd = ok ? cd : ~0ull;
+            ci = ok ? ci : ~0u;
+            L.merge_batch(cd, ci, lane);
+            L.refresh();
+        }
+    };
+    const int nwin = 64 * (S + 1);
+    if (anchor >= 0) {
+        w0 = anchor - nwin / 2;
+        if (w0 > n - nwin) w0 = n - nwin;
+        if (w0 < 0) w0 = 0;
+        w1 = w0 + nwin < n ? w0 + nwin : n;
+        for (int r0 = w0; r0 < w1; r0 += 64) scan(r0, w1 - r0 < 64 ? w1 - r0 : 64, false);
+        if (w0 == 0 && w1 == n) return false;
+    }
+    if (n <= 64) {
+        scan(0, n, true);
+        return false;
+    }
+    // stack entry e: lane e & 63 of register (e >> 6)
+    uint32_t sn0 = 0, sn1 = 0, sd0l = 0, sd0h = 0, sd1l = 0, sd1h = 0;
+    int sp = 0;
+    int node = 0;
+    while (true) {
+        ++c.nodes;
+        if (node >= off) {
+            const int r = node - off;
+            if (r != self && (r < w0 || r >= w1)) {
+                const D d2 = leaf_d2_k<T, MK>(t.m, px, py, pz, t.mnx[node], t.mny[node], t.mnz[node]);
+                const uint64_t cd = dist_bits(d2);
+                const uint32_t ci = (uint32_t)ret_index(t, node);
+                ++c.leaves;
+                if (L.admits(cd, ci)) {
+                    ++c.inserts;
+                    L.insert_one(cd, ci);
+                    L.refresh();
+                }
+            }
+        } else {
+            const int rf = uni(t.rfirst[node]), rl_ = uni(t.rlast[node]);
+            if (rf >= w0 && rl_ < w1) {
+                // inside the window: already inserted
+            } else if (rl_ - rf < 64) {
+                scan(rf, rl_ - rf + 1, true);
+            } else {
+                const int l = uni(t.left[node]), r = uni(t.right[node]);
+                const uint64_t dl = dist_bits(node_d2_k<T, MK>(t, px, py, pz, l));
+                const uint64_t dr = dist_bits(node_d2_k<T, MK>(t, px, py, pz, r));
+                const bool lfirst = dl <= dr;
+                const int nn = uni(lfirst ? l : r), fn = uni(lfirst ? r : l);
+                const uint64_t dn = lfirst ? dl : dr, df = lfirst ? dr : dl;
+                if (df <= L.wd) {
+                    if (sp >= 128) return true;
+                    const uint32_t lo = (uint32_t)df, hi = (uint32_t)(df >> 32);
+                    const bool at = lane == (sp & 63);
+                    if (sp < 64) {
+                        sn0 = at ? (uint32_t)fn : sn0;
+                        sd0l = at ? lo : sd0l;
+                        sd0h = at ? hi : sd0h;
+                    } else {
+                        sn1 = at ? (uint32_t)fn : sn1;
+                        sd1l = at ? lo : sd1l;
+                        sd1h = at ? hi : sd1h;
+                    }
+                    ++sp;
+                }
+                if (dn <= L.wd) {
+                    node = nn;
+                    continue;
+                }
+            }
+        }
+        // pop the next entry still within the bound
+        bool found = false;
+        while (sp > 0) {
+            --sp;
+            int v;
+            uint64_t dv;
+            if (sp < 64) {
+                v = (int)rl32(sn0, sp);
+                dv = ((uint64_t)rl32(sd0h, sp) << 32) | rl32(sd0l, sp);
+            } else {
+                v = (int)rl32(sn1, sp - 64);
+                dv = ((uint64_t)rl32(sd1h, sp - 64) << 32) | rl32(sd1l, sp - 64);
+            }
+            if (dv <= L.wd) {
+                node = v;
+                found = true;
+                break;
+            }
+        }
+        if (!found) break;
+    }
+    return false;
+}
+
 }  // namespace ft

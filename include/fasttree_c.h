@@ -58,6 +58,9 @@ extern "C" {
 #define FT_ALGO_PLAIN 1   /* one lane per query, independent DFS (reference knn_query_small_k) */
 #define FT_ALGO_GROUPED 2 /* SFC-grouped lockstep DFS with a wave vote (reference self-kNN)    */
 #define FT_ALGO_PACKET 3  /* one shared traversal per wavefront of SFC-adjacent queries        */
+#define FT_ALGO_WAVE 4    /* one wavefront per query: leaves tested 64 at a time, a sorted      */
+                          /* list spread over the lanes, batches merged by cross-lane bitonic   */
+                          /* networks (canonical SortedMergeHeap); k up to 1024                 */
 
 #define FT_TOPK_AUTO 0      /* = FT_TOPK_REGISTERS                                        */
 #define FT_TOPK_REGISTERS 1 /* sorted (d2, index) entries in registers, up to 32; a larger */
@@ -346,6 +349,15 @@ int ft_batch_set_stats(ft_batch *batch, uint64_t *stats);
 /* class, k tier), the fastest tier kernel and leaf bucket, and the fastest build    */
 /* branch count.  Synchronises the stream.                                            */
 int ft_batch_tune(ft_batch *batch, ft_stream s);
+/* Pins the setting of one (tree-size class, k tier) group, which the autotuner then  */
+/* keeps: size classes 0..3 = trees below 40k, 160k, 640k leaves and larger; k tiers  */
+/* 0..7 = k up to 1, 4, 8, 16, 32, 64, 128 and above.  tier: a tier name as           */
+/* ft_batch_tuning prints it ("packet-reg32", "packet-reg32-passes", "lane-heap",     */
+/* "wave-merge128", ...) that serves the k tier; bucket: packet leaf bucket (1..64,   */
+/* 0 = 32); queries_per_item: packet tiers 16, 32 or 64, wave tiers 1..64 (0 = the    */
+/* default).  No reference counterpart (the reference fixes these at compile time).   */
+int ft_batch_set_tier(ft_batch *batch, int32_t n_class, int32_t k_tier, const char *tier,
+                      int32_t bucket, int32_t queries_per_item);
 /* One step, launched kernel by kernel (no graph).                                    */
 int ft_batch_run(ft_batch *batch, ft_stream s);
 /* Records the step into a hipGraph (allocates nothing while recording).             */

@@ -166,3 +166,87 @@ def test_batch_queries_only_on_prebuilt_trees(ft, orc):
     b.launch()
     check(reqs, expected(ft, trees, reqs, q), res, dst, "prebuilt")
     assert stats.to_host()[0] > 0
+
+
+def _tiers_for(kt, fp):
+    """The tier kernels that serve k tier kt (k up to 1, 4, 8, 16, 32, 64, 128,
+    256), as the batch's tuner offers them."""
+    cap = (1, 4, 8, 16, 32, 64, 128, 256)[kt]
+    t = []
+    if cap <= 32:
+        t.append(f"packet-reg{cap}")
+    elif cap <= 128:
+        t.append("packet-reg32-passes")
+    if not fp and cap <= 128:
+        t.append(f"packet-heap{max(cap, 32)}")
+    t.append("lane-heap")
+    if cap > 32:
+        t.append(f"wave-merge{cap}")
+    return t
+
+
+@pytest.mark.parametrize("ct", [O.CT_U32, O.CT_F64])
+def test_batch_pinned_tiers_every_size_class(ft, orc, ct):
+    """Every tier kernel, pinned (ft_batch_set_tier) for every (size class, k
+    tier) group it serves, at several leaf buckets and queries per item,
+    returns knn_query's results -- on trees in all four size classes (below
+    40k, 160k, 640k leaves and above), which the tuner alone may never run."""
+    from fasttree_amd.batch import Batch, plan_requests
+    fp = ct == O.CT_F64
+    dt = O.COORD_DTYPES[ct]
+    cfg = ft.Config(sfc=ft.SFC_MORTON, bits_per_dim=21)
+    rng = np.random.default_rng(8)
+    sizes = (20000, 100000, 300000, 700000)
+    ks = (1, 3, 8, 13, 32, 50, 128, 200)  # one per k tier
+    trees, parts = [], []
+    for i, n in enumerate(sizes):
+        x, y, z = synth.nfw_clustered(n, 500 + i, box=100.0, n_halos=max(2, n // 2000))
+        pts = tuple(a.astype(dt) if fp else synth.to_fixed(a, 100.0, 21, dt) for a in (x, y, z))
+        trees.append(ft.TreeSoA(n, ct, cfg))
+        parts.append(tuple(ft.DeviceArray.from_host(a) for a in pts))
+    nq, kk, tt = [], [], []
+    for i in range(len(sizes)):
+        for k in ks:
+            nq.append(int(rng.choice([1, 65, 300])))
+            kk.append(k)
+            tt.append(i)
+    reqs, total_q, total_r = plan_requests(nq, kk, tt)
+    hi = 100.0 if fp else float(1 << 21)
+    q = tuple(rng.uniform(0, hi, total_q).astype(dt) for _ in range(3))
+    dq = [ft.DeviceArray.from_host(a) for a in q]
+    res = ft.DeviceArray(total_r, np.uint64)
+    dst = ft.DeviceArray(total_r, ft.dist_dtype(ct))
+    b = Batch(ct)
+    b.set_trees(trees, parts)
+    b.set_queries(reqs, *dq, res, dst)
+    b.run()  # builds the trees
+    exp = expected(ft, trees, reqs, q)
+    tiers = [_tiers_for(kt, fp) for kt in range(8)]
+    for rnd in range(max(len(t) for t in tiers)):
+        for c in range(4):
+            for kt in range(8):
+                name = tiers[kt][(rnd + c) % len(tiers[kt])]
+                if name.startswith("packet"):
+                    b.set_tier(c, kt, name, bucket=(16, 32, 64)[(rnd + kt) % 3],
+                               queries_per_item=(64, 32, 16)[(rnd + c) % 3])
+                elif name.startswith("wave"):
+                    b.set_tier(c, kt, name, queries_per_item=(8, 1, 64)[(rnd + c) % 3])
+                else:
+                    b.set_tier(c, kt, name)
+        res.zero()
+        dst.zero()
+        print(f"[batch] pinned round {rnd}: {b.tuning()['groups'][:3]}", flush=True)
+        b.run()
+        check(reqs, exp, res, dst, f"pinned round {rnd}")
+    with pytest.raises(ValueError):
+        b.set_tier(0, 2, "packet-reg32-passes")  # does not serve k <= 8
+    with pytest.raises(ValueError):
+        b.set_tier(0, 6, "wave-merge128", queries_per_item=65)
+    # the tuner keeps pinned groups
+    b.set_tier(3, 6, "wave-merge128")
+    b.tune()
+    g = [x for x in b.tuning()["groups"] if x["n_class"] == ">=640k" and x["k_max"] == 128]
+    assert g and g[0]["tier"] == "wave-merge128"
+    res.zero()
+    b.launch()
+    check(reqs, exp, res, dst, "tuned with a pinned group")
