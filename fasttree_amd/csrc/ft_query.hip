@@ -1219,13 +1219,21 @@ static int knn_dispatch(hipStream_t s, const ft_tree *tree, const QSrc<T> &qs, i
         FT_CHECK_LAUNCH();
         return FT_OK;
     }
-    // Canonical mode.  Packet tiers: registers for k <= 32; an LDS heap per
-    // lane for integer metrics up to k = 128.  Larger k (or FP metrics above
-    // 32) take the per-lane heap walk.
+    // Canonical mode.  Packet tiers: registers for k <= 32, passes of 32 up
+    // to k = 128.  Larger k, and external queries above k = 64, take the
+    // wave-per-query tier (at 10M clustered points, 2.5M self queries at
+    // k = 256: 94.5 ms against 1,100 ms for the per-lane LDS heaps; 1M
+    // external queries at k = 128: 34.9 against 59.8 ms for the passes, at
+    // k = 64 28.7 against 29.5 ms), above k = 1024 the per-lane heap walk.
     const bool packet_ok = k <= 128;  // above 32: passes of 32
-    if (algo == FT_ALGO_AUTO) algo = packet_ok ? FT_ALGO_PACKET : FT_ALGO_GROUPED;
+    if (algo == FT_ALGO_AUTO) {
+        if (k > WAVE_MAX_K) algo = FT_ALGO_GROUPED;
+        else if (k > 128 || (external && k > 64)) algo = FT_ALGO_WAVE;
+        else algo = FT_ALGO_PACKET;
+    }
     if (o.seed_bound && packet_ok) algo = FT_ALGO_PACKET;
-    if (algo == FT_ALGO_PACKET && !packet_ok) algo = FT_ALGO_GROUPED;
+    if (algo == FT_ALGO_PACKET && !packet_ok)
+        algo = k <= WAVE_MAX_K ? FT_ALGO_WAVE : FT_ALGO_GROUPED;
     const D *seed = algo == FT_ALGO_PACKET ? (const D *)o.seed_bound : nullptr;
     const int bucket = o.leaf_bucket > 0 ? std::min(o.leaf_bucket, 64) : 64;
     const uint32_t *order = nullptr;
