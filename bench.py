@@ -65,6 +65,8 @@ def main():
                     help="run only the batched multi-tree workload (development)")
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the batched multi-tree workload (64 trees, 4M queries)")
+    ap.add_argument("--no-large-k", action="store_true",
+                    help="skip the wave-per-query tier's large-k comparison on the C2 tree")
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the small plain and the SPH mixed-query configurations")
     ap.add_argument("--configs-only", action="store_true",
@@ -284,6 +286,8 @@ def main():
             ladder[name] = n / (ms * 1e-3)
             log(f"{name}: {ms:.1f} ms")
 
+    large_k = None if args.no_large_k else large_k_bench(ft, tree, x, y, z, n, stream, ev)
+
     # Roofline of the dominant kernel (the self-kNN traversal): algorithmic bytes
     # N*(3s + 16k) + (12s + 8)*N with s = 8 bytes per coordinate.
     s_b = 8
@@ -340,10 +344,71 @@ def main():
                       "chains_per_wave": float(st[6]) / ((n + 63) // 64)},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "large_k": large_k,
         "batched": batched,
         "configs": configs,
     }
     print(json.dumps(out), flush=True)
+
+
+def large_k_bench(ft, tree, x, y, z, n, stream, ev):
+    """The wave-per-query tier (one wavefront per query, a sorted list spread
+    over its lanes) against the tier the other algorithms give, on the C2 tree:
+    1M external queries near the particles at k = 128 (against the packet tier
+    in passes of 32) and 2.5M self queries (every fourth leaf) at k = 256
+    (against the per-lane LDS heaps); rows compared on a sample."""
+    rng = np.random.default_rng(3)
+
+    def timed(fn, reps=2):
+        fn()
+        stream.synchronize()
+        ts = []
+        for _ in range(reps):
+            ev[0].record(stream)
+            fn()
+            ev[1].record(stream)
+            stream.synchronize()
+            ts.append(ev[0].elapsed_ms(ev[1]))
+        return min(ts)
+
+    def same_rows(ra, da, rb, db, nq, k):
+        rows = np.linspace(0, nq - 1, 1000).astype(np.int64)
+        return all(np.array_equal(ra.to_host_range(r * k, k), rb.to_host_range(r * k, k)) and
+                   np.array_equal(da.to_host_range(r * k, k), db.to_host_range(r * k, k))
+                   for r in rows)
+
+    out = {}
+    nq, k = 1_000_000, 128
+    sel = rng.integers(0, n, nq)
+    mask = np.uint64((1 << 21) - 1)
+    q = [ft.DeviceArray.from_host((a[sel] + rng.integers(0, 64, nq).astype(np.uint64)) & mask)
+         for a in (x, y, z)]
+    ra, da, rb, db = (ft.DeviceArray(nq * k, np.uint64) for _ in range(4))
+    t_other = timed(lambda: ft.knn_query(tree, *q, k, results=ra, dists=da, stream=stream.handle,
+                                         algorithm=ft.ALGO_PACKET))
+    t_wave = timed(lambda: ft.knn_query(tree, *q, k, results=rb, dists=db, stream=stream.handle,
+                                        algorithm=ft.ALGO_WAVE))
+    out["external_k128"] = {"queries": nq, "wave_ms": round(t_wave, 3),
+                            "packet_passes_ms": round(t_other, 3),
+                            "wave_queries_per_s": round(nq / (t_wave * 1e-3), 1),
+                            "identical_on_1000_sampled_rows": bool(same_rows(ra, da, rb, db, nq, k))}
+    log(f"large k: external k=128 wave {t_wave:.2f} ms, packet passes {t_other:.2f} ms")
+    del ra, da, rb, db, q
+    nq, k = n // 4, 256
+    ids = ft.DeviceArray.from_host(np.arange(nq, dtype=np.int32) * 4)
+    ra, da, rb, db = (ft.DeviceArray(nq * k, np.uint64) for _ in range(4))
+    t_other = timed(lambda: ft.self_knn_query_subset(tree, ids, k, results=ra, dists=da,
+                                                     stream=stream.handle,
+                                                     algorithm=ft.ALGO_GROUPED), reps=1)
+    t_wave = timed(lambda: ft.self_knn_query_subset(tree, ids, k, results=rb, dists=db,
+                                                    stream=stream.handle,
+                                                    algorithm=ft.ALGO_WAVE))
+    out["self_subset_k256"] = {"queries": nq, "wave_ms": round(t_wave, 3),
+                               "lane_heap_ms": round(t_other, 3),
+                               "wave_queries_per_s": round(nq / (t_wave * 1e-3), 1),
+                               "identical_on_1000_sampled_rows": bool(same_rows(ra, da, rb, db, nq, k))}
+    log(f"large k: self subset k=256 wave {t_wave:.2f} ms, per-lane heaps {t_other:.2f} ms")
+    return out
 
 
 TRAFFIC_PROFILE = "profiles/c2_hbm_traffic_counters.md"
