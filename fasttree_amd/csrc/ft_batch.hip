@@ -462,6 +462,13 @@ struct ft_batch {
     int mk = 0;  // metric kind shared by every tree (MK_ANY when they differ)
     ft::Scratch sort_scratch;
     std::vector<ft::Scratch> build_scratch;
+    // segmented build: every tree in one launch of each build kernel
+    int seg_mode = -1;  // -1 untuned (segmented when possible), 0 per-tree builds, 1 segmented
+    bool seg_ok = false;  // the trees share one key configuration (checked at upload)
+    void *seg_block = nullptr;
+    size_t seg_bytes = 0;
+    ft::SegBuild seg{};
+    ft::Scratch seg_scratch;
     std::vector<hipStream_t> side;
     hipEvent_t ev_fork = nullptr;
     std::vector<hipEvent_t> ev_join;
@@ -541,6 +548,82 @@ static void default_tuning(ft_batch *b) {
             g.width = 64;
         }
 }
+
+// Tables of the segmented build (trees, sort tiles, assembly chunks) and its
+// buffers, sized here so that the step allocates nothing.  The segmented build
+// needs every tree to share the curve and bits per dimension, and fewer than
+// 2^31 points in all; otherwise the trees are built one by one.
+template <typename T> static int upload_seg(ft_batch *b) {
+    const int nt = (int)b->trees.size();
+    b->seg_ok = false;
+    if (!b->build || nt == 0) return FT_OK;
+    const ft_config &c0 = b->trees[0]->config;
+    size_t total = 0;
+    for (int i = 0; i < nt; ++i) {
+        const ft_config &c = b->trees[i]->config;
+        if (c.sfc != c0.sfc || c.bits_per_dim != c0.bits_per_dim) return FT_OK;
+        total += b->bn[i];
+    }
+    if (total == 0 || total + (size_t)nt * SEG_CHUNK >= (size_t)INT_MAX) return FT_OK;
+    std::vector<SegTree<T>> st(nt);
+    std::vector<int4> tiles;
+    std::vector<int2> chunks;
+    int32_t off = 0;
+    for (int i = 0; i < nt; ++i) {
+        const int n = (int)b->bn[i];
+        SegTree<T> &q = st[i];
+        q.x = (const T *)b->bx[i];
+        q.y = (const T *)b->by[i];
+        q.z = (const T *)b->bz[i];
+        q.o = make_tree_out<T>(*b->trees[i]);
+        q.sorted_keys = b->trees[i]->sorted_keys;
+        q.off = off;
+        q.n = n;
+        q.tile0 = (int32_t)tiles.size();
+        for (int a = 0; a < n; a += SEG_TILE)
+            tiles.push_back(make_int4(off + a, off + std::min(a + SEG_TILE, n), i, a == 0 ? 1 : 0));
+        q.ntiles = (int32_t)tiles.size() - q.tile0;
+        for (int a = 0; a < n; a += SEG_CHUNK) chunks.push_back(make_int2(i, a));
+        off += n;
+    }
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t o_t = 0, s_t = al(sizeof(SegTree<T>) * nt);
+    const size_t o_i = o_t + s_t, s_i = al(sizeof(int4) * tiles.size());
+    const size_t o_c = o_i + s_i, s_c = al(sizeof(int2) * chunks.size());
+    const size_t o_h = o_c + s_c, s_h = al(4 * 2 * (size_t)nt * SEG_HIST_WORDS);
+    const size_t o_b = o_h + s_h, s_b = al((size_t)nt * 16 * sizeof(T));  // box records
+    const size_t o_p = o_b + s_b, s_p = al(tiles.size() * 6 * sizeof(T));
+    const size_t bytes = o_p + s_p;
+    if (bytes > b->seg_bytes) {
+        if (b->seg_block) FT_HIP(hipFree(b->seg_block));
+        b->seg_block = nullptr;
+        b->seg_bytes = 0;
+        if (hipMalloc(&b->seg_block, bytes) != hipSuccess)
+            return fail(FT_ENOMEM, "segmented build table allocation failed");
+        b->seg_bytes = bytes;
+    }
+    char *base = (char *)b->seg_block;
+    FT_HIP(hipMemcpy(base + o_t, st.data(), sizeof(SegTree<T>) * nt, hipMemcpyHostToDevice));
+    FT_HIP(hipMemcpy(base + o_i, tiles.data(), sizeof(int4) * tiles.size(), hipMemcpyHostToDevice));
+    FT_HIP(hipMemcpy(base + o_c, chunks.data(), sizeof(int2) * chunks.size(),
+                     hipMemcpyHostToDevice));
+    SegBuild &g = b->seg;
+    g.trees = base + o_t;
+    g.tiles = (const int4 *)(base + o_i);
+    g.chunks = (const int2 *)(base + o_c);
+    g.ntrees = nt;
+    g.ntiles = (int)tiles.size();
+    g.nchunks = (int)chunks.size();
+    g.total = total;
+    g.hist = (uint32_t *)(base + o_h);
+    g.bbox = base + o_b;
+    g.partials = base + o_p;
+    FT_TRY(scratch_ensure_public(b->seg_scratch, total + (size_t)nt * SEG_CHUNK));
+    b->seg_ok = true;
+    return FT_OK;
+}
+
+static bool use_seg(const ft_batch *b) { return b->seg_ok && b->seg_mode != 0; }
 
 // Device tables: trees, requests (with tuned settings), items per variant.
 template <typename T> static int upload_tables(ft_batch *b) {
@@ -644,6 +727,7 @@ template <typename T> static int upload_tables(ft_batch *b) {
                          hipMemcpyHostToDevice));
     FT_HIP(hipMemset(b->dctr, 0, 4 * (V_COUNT * FT_XCDS + 1)));
     FT_HIP(hipMemcpy(b->dbnd, bnd.data(), 4 * bnd.size(), hipMemcpyHostToDevice));
+    FT_TRY(upload_seg<T>(b));
     // The clear and copies above go through the null stream; the step runs on
     // non-blocking streams that do not wait for it.
     FT_HIP(hipDeviceSynchronize());
@@ -788,10 +872,14 @@ static std::vector<int> assign_branches(ft_batch *b, int nb, std::vector<int> &b
     return order;
 }
 
-// Builds every tree on nb branches (nb = 1: in order on s).
-template <typename T> static int enqueue_builds(ft_batch *b, hipStream_t s, int nb) {
+// Builds every tree: segmented (one launch of each build kernel for all
+// trees), or one build_bvh per tree on nb branches (nb = 1: in order on s).
+template <typename T> static int enqueue_builds(ft_batch *b, hipStream_t s, int nb,
+                                                bool per_tree = false) {
     const int nt = (int)b->trees.size();
     if (!b->build || nt == 0) return FT_OK;
+    if (use_seg(b) && !per_tree)
+        return build_segmented_t<T>(s, b->seg_scratch, b->seg, b->trees[0]->config);
     nb = std::max(1, std::min(nb, nt));
     FT_TRY(ensure_streams(b, nb));
     while ((int)b->build_scratch.size() < nb) b->build_scratch.emplace_back();
@@ -1117,16 +1205,17 @@ template <typename T> static int tune_t(ft_batch *b, hipStream_t s) {
         }
         S0.radix_bits = 0;
     }
-    // Build branches.
-    if (rc == FT_OK && b->build && b->trees.size() > 1) {
+    // Build phase: one build per tree on 1, 2, 4 or 8 branches, against the
+    // segmented build of every tree at once.
+    if (rc == FT_OK && b->build) {
         double best = 1e30;
         int best_nb = 1;
         for (int nb : {1, 2, 4, 8}) {
-            if (nb > (int)b->trees.size()) break;
+            if (nb > 1 && nb > (int)b->trees.size()) break;
             float t_best = 1e30f;
             for (int rep = 0; rep < 2 && rc == FT_OK; ++rep) {
                 (void)hipEventRecord(e0, s);
-                rc = enqueue_builds<T>(b, s, nb);
+                rc = enqueue_builds<T>(b, s, nb, true);
                 (void)hipEventRecord(e1, s);
                 (void)hipEventSynchronize(e1);
                 t_best = std::min(t_best, elapsed(e0, e1));
@@ -1137,6 +1226,18 @@ template <typename T> static int tune_t(ft_batch *b, hipStream_t s) {
             }
         }
         b->branches = best_nb;
+        b->seg_mode = 0;
+        if (b->seg_ok && rc == FT_OK) {
+            float t_seg = 1e30f;
+            for (int rep = 0; rep < 2 && rc == FT_OK; ++rep) {
+                (void)hipEventRecord(e0, s);
+                rc = build_segmented_t<T>(s, b->seg_scratch, b->seg, b->trees[0]->config);
+                (void)hipEventRecord(e1, s);
+                (void)hipEventSynchronize(e1);
+                t_seg = std::min(t_seg, elapsed(e0, e1));
+            }
+            if (t_seg < best) b->seg_mode = 1;
+        }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
@@ -1222,6 +1323,8 @@ int ft_batch_destroy(ft_batch *b) {
     if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
     if (b->dev_block) (void)hipFree(b->dev_block);
     if (b->sort_scratch.block) (void)hipFree(b->sort_scratch.block);
+    if (b->seg_scratch.block) (void)hipFree(b->seg_scratch.block);
+    if (b->seg_block) (void)hipFree(b->seg_block);
     for (auto &S : b->build_scratch)
         if (S.block) (void)hipFree(S.block);
     delete b;
@@ -1357,7 +1460,9 @@ int ft_batch_tuning(ft_batch *b, char *buf, size_t len) {
     if (!b->tuned) default_tuning(b);
     std::string out = "{\"tuned\": ";
     out += b->tuned ? "true" : "false";
-    out += ", \"build_branches\": " + std::to_string(b->branches);
+    out += ", \"build\": \"";
+    out += use_seg(b) ? "segmented" : "per-tree";
+    out += "\", \"build_branches\": " + std::to_string(b->branches);
     static const char *const cls[N_CLASS] = {"<40k", "<160k", "<640k", ">=640k"};
     out += ", \"radix_bits\": {";
     for (int c = 0; c < N_CLASS; ++c)

@@ -178,6 +178,47 @@ template <typename T> struct alignas(4 * sizeof(T)) Point4 {
 // the keys once more just to count them.  With `pts`, it also writes each
 // point as one record for the leaf gather.
 constexpr int ENC_HIST_WORDS = 8 * 512;  // up to 8 passes of 9-bit digits
+
+// The SFC key of one point (box lo / inv: FP coordinates only).
+template <typename T>
+__device__ __forceinline__ uint64_t encode_point(const T p[3], int B, int sfc, const T lo[3],
+                                                 const T inv[3], const uint16_t *tab) {
+    const T clamp_hi = CoordTraits<T>::fp ? (T)(sizeof(T) == 8 ? 0x1.fffffffffffffp-1 : 0x1.fffffep-1) : T(0);
+    uint32_t q[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        if constexpr (CoordTraits<T>::fp) {
+            T t = (p[a] - lo[a]) * inv[a];
+            T nx = ffmin(ffmax(t, T(0)), clamp_hi);  // sycl::clamp
+            q[a] = quantise<T>(nx, B, sfc);
+        } else {
+            q[a] = (uint32_t)((uint64_t)p[a] & ((1ull << B) - 1));  // x & (2^B-1)
+        }
+    }
+    if (sfc == FT_SFC_MORTON) return spread3_21(q[0]) | (spread3_21(q[1]) << 1) | (spread3_21(q[2]) << 2);
+    return ph_encode(q[0], q[1], q[2], B, tab);
+}
+
+// Adds one key's digits of every pass to the block's LDS histograms.  When a
+// wave's active keys share a digit (the high digits of spatially clustered
+// input runs), one add of the lane count replaces that many atomics on the
+// same LDS word.
+__device__ __forceinline__ void hist_add_key(uint32_t *h, uint64_t key, int passes, int rbits) {
+    const uint32_t dmask = (1u << rbits) - 1u;
+    const uint64_t act = __ballot(1);
+    const int lead = __ffsll((long long)act) - 1;
+    for (int p = 0; p < passes; ++p) {
+        const uint32_t dg = (uint32_t)(key >> (rbits * p)) & dmask;
+        const uint32_t d0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)dg);
+        if (__ballot(dg == d0) == act) {
+            if ((int)(threadIdx.x & 63) == lead)
+                atomicAdd(&h[(p << rbits) + d0], (uint32_t)__popcll(act));
+        } else {
+            atomicAdd(&h[(p << rbits) + dg], 1u);
+        }
+    }
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void encode_kernel(const T *__restrict__ x,
                                                      const T *__restrict__ y,
@@ -188,7 +229,6 @@ __global__ __launch_bounds__(256) void encode_kernel(const T *__restrict__ x,
                                                      Point4<T> *__restrict__ pts) {
     __shared__ uint16_t tab[384];
     __shared__ uint32_t h[ENC_HIST_WORDS];
-    const uint32_t dmask = (1u << rbits) - 1u;
     for (int i = threadIdx.x; i < 384; i += blockDim.x) tab[i] = c_ph_table[i];
     for (int i = threadIdx.x; i < ENC_HIST_WORDS; i += blockDim.x) h[i] = 0;
     __syncthreads();
@@ -199,43 +239,13 @@ __global__ __launch_bounds__(256) void encode_kernel(const T *__restrict__ x,
             inv[a] = bb->inv[a];
         }
     }
-    const T clamp_hi = CoordTraits<T>::fp ? (T)(sizeof(T) == 8 ? 0x1.fffffffffffffp-1 : 0x1.fffffep-1) : T(0);
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (size_t)gridDim.x * blockDim.x) {
         const T p[3] = {x[i], y[i], z[i]};
         if (pts) pts[i] = Point4<T>{{p[0], p[1], p[2], T(0)}};
-        uint32_t q[3];
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            if constexpr (CoordTraits<T>::fp) {
-                T t = (p[a] - lo[a]) * inv[a];
-                T nx = ffmin(ffmax(t, T(0)), clamp_hi);  // sycl::clamp
-                q[a] = quantise<T>(nx, B, sfc);
-            } else {
-                q[a] = (uint32_t)((uint64_t)p[a] & ((1ull << B) - 1));  // x & (2^B-1)
-            }
-        }
-        uint64_t key;
-        if (sfc == FT_SFC_MORTON) key = spread3_21(q[0]) | (spread3_21(q[1]) << 1) | (spread3_21(q[2]) << 2);
-        else key = ph_encode(q[0], q[1], q[2], B, tab);
+        const uint64_t key = encode_point<T>(p, B, sfc, lo, inv, tab);
         keys[i] = key;
-        if (hist) {
-            // When a wave's 64 keys share a digit (the high digits of spatially
-            // clustered input runs), one add of the lane count replaces 64
-            // atomics on the same LDS word.
-            const uint64_t act = __ballot(1);
-            const int lead = __ffsll((long long)act) - 1;
-            for (int p = 0; p < passes; ++p) {
-                const uint32_t dg = (uint32_t)(key >> (rbits * p)) & dmask;
-                const uint32_t d0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)dg);
-                if (__ballot(dg == d0) == act) {
-                    if ((int)(threadIdx.x & 63) == lead)
-                        atomicAdd(&h[(p << rbits) + d0], (uint32_t)__popcll(act));
-                } else {
-                    atomicAdd(&h[(p << rbits) + dg], 1u);
-                }
-            }
-        }
+        if (hist) hist_add_key(h, key, passes, rbits);
     }
     if (hist) {
         __syncthreads();
@@ -335,12 +345,17 @@ __device__ inline uint64_t lanemask_lt() {
     return lane ? (~0ull >> (64 - lane)) : 0ull;
 }
 
-template <bool FIRST, int BITS>
+// SEG: many trees' keys at once.  seg_tiles[tile] = (first key, end, tree,
+// first tile of its tree); tiles never cross trees, a tree's digit bases
+// (which include its first index) are digit_base + tree * RS_HIST_WORDS, and
+// the look-back stops at the tree's first tile.
+template <bool FIRST, int BITS, bool SEG = false>
 __global__ __launch_bounds__(RS_BLOCK) void onesweep_kernel(
     const uint64_t *__restrict__ keys_in, const uint32_t *__restrict__ vals_in,
     uint64_t *__restrict__ keys_out, uint32_t *__restrict__ vals_out, int n, int shift,
     const uint32_t *__restrict__ digit_base, uint64_t *__restrict__ lookback,
-    const uint32_t *__restrict__ epoch, int pass, uint32_t *__restrict__ tile_ctr) {
+    const uint32_t *__restrict__ epoch, int pass, uint32_t *__restrict__ tile_ctr,
+    const int4 *__restrict__ seg_tiles = nullptr) {
     constexpr int RADIX = 1 << BITS;
     constexpr uint32_t MASK = RADIX - 1;
     static_assert(RADIX <= RS_MAX_RADIX, "digit width");
@@ -364,7 +379,17 @@ __global__ __launch_bounds__(RS_BLOCK) void onesweep_kernel(
     for (int i = tid; i < RS_WAVES * RADIX; i += RS_BLOCK) (&s_wave[0][0])[i] = 0;
     __syncthreads();
     const int tile = s_tile;
-    const int base = tile * RS_TILE + wave * (64 * RS_ITEMS);
+    int tile_start = tile * RS_TILE, tile_end = n;
+    bool first_tile = tile == 0;
+    const uint32_t *dbase = digit_base;
+    if constexpr (SEG) {
+        const int4 ti = seg_tiles[tile];
+        tile_start = ti.x;
+        tile_end = ti.y;
+        first_tile = ti.w != 0;
+        dbase = digit_base + (size_t)ti.z * RS_HIST_WORDS;
+    }
+    const int base = tile_start + wave * (64 * RS_ITEMS);
 
     uint64_t k[RS_ITEMS];
     uint32_t v[RS_ITEMS];
@@ -372,7 +397,7 @@ __global__ __launch_bounds__(RS_BLOCK) void onesweep_kernel(
 #pragma unroll
     for (int j = 0; j < RS_ITEMS; ++j) {
         const int idx = base + j * 64 + lane;
-        const bool ok = idx < n;
+        const bool ok = idx < tile_end;
         k[j] = ok ? keys_in[idx] : ~0ull;
         v[j] = FIRST ? (uint32_t)idx : (ok ? vals_in[idx] : 0u);
     }
@@ -382,7 +407,7 @@ __global__ __launch_bounds__(RS_BLOCK) void onesweep_kernel(
 #pragma unroll
     for (int j = 0; j < RS_ITEMS; ++j) {
         const int idx = base + j * 64 + lane;
-        const bool ok = idx < n;
+        const bool ok = idx < tile_end;
         const uint32_t d = (uint32_t)(k[j] >> shift) & MASK;
         uint64_t peers = __ballot(ok);
 #pragma unroll
@@ -415,7 +440,7 @@ __global__ __launch_bounds__(RS_BLOCK) void onesweep_kernel(
         }
         const uint64_t tag = (uint64_t)((*epoch * 8u + (uint32_t)pass) & 0x3FFFFFFFu) << 34;
         uint64_t *slot = lookback + (size_t)tile * RADIX + dg;
-        if (tile == 0) {
+        if (first_tile) {
             __hip_atomic_store(slot, tag | (2ull << 32) | count, __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
         } else {
@@ -440,7 +465,7 @@ __global__ __launch_bounds__(RS_BLOCK) void onesweep_kernel(
     const uint32_t local = block_excl_scan(count, s_tmp);  // the first RADIX threads hold the digits
     if (dthr) {
         s_local[dg] = local;
-        s_global[dg] = digit_base[dg] + excl;
+        s_global[dg] = dbase[dg] + excl;
     }
     __syncthreads();
 
@@ -451,7 +476,7 @@ __global__ __launch_bounds__(RS_BLOCK) void onesweep_kernel(
     for (int j = 0; j < RS_ITEMS; ++j) {
         const int idx = base + j * 64 + lane;
         lpos[j] = 0xFFFFFFFFu;
-        if (idx < n) {
+        if (idx < tile_end) {
             const uint32_t d = (uint32_t)(k[j] >> shift) & MASK;
             lpos[j] = s_local[d] + s_wave[wave][d] + rank[j];
         }
@@ -461,7 +486,7 @@ __global__ __launch_bounds__(RS_BLOCK) void onesweep_kernel(
     for (int j = 0; j < RS_ITEMS; ++j)
         if (lpos[j] != 0xFFFFFFFFu) s_keys[lpos[j]] = k[j];
     __syncthreads();
-    const int tile_n = min(RS_TILE, n - tile * RS_TILE);
+    const int tile_n = min(RS_TILE, tile_end - tile_start);
     for (int i = tid; i < tile_n; i += RS_BLOCK) {
         const uint64_t key = s_keys[i];
         const uint32_t d = (uint32_t)(key >> shift) & MASK;
@@ -522,6 +547,8 @@ template <typename T> __device__ inline void st_agent(T *p, T v) {
 }
 
 constexpr int ASM_CHUNK = 512;      // leaves per assembly workgroup
+static_assert(ASM_CHUNK == SEG_CHUNK && RS_TILE == SEG_TILE && RS_HIST_WORDS == SEG_HIST_WORDS,
+              "segmented build tables use the build's tile, chunk and histogram sizes");
 constexpr int ASM_CHUNK_MIN = 256;  // sizes the per-chunk top counts in the scratch
 
 // Where the leaves come from: the input particles through the sort
@@ -538,13 +565,6 @@ template <typename T> struct LeafSrc {
     const Point4<T> *pts;   // LEAF_GATHER: the points as records (null: x, y, z)
 };
 
-template <typename T> struct TreeOut {
-    T *mnx, *mxx, *mny, *mxy, *mnz, *mxz;
-    int32_t *left, *right, *parent, *rfirst, *rlast;
-    uint32_t *id;
-    int8_t *ghost;
-    int32_t *orig;
-};
 
 // Writes internal node `lab` over leaves [l, r] split at g, with its box
 // (agent-scope stores: read by later climbers of the same launch).
@@ -578,11 +598,14 @@ __device__ inline void write_node(const TreeOut<T> &o, int n, int lab, int l, in
 // Arrival word of a split: arrivals << 16 | (far end - a + 1) summed.
 constexpr int32_t NO_PARENT = INT32_MIN;  // parent not known inside the chunk
 
+// One chunk: leaves [a, a + C) of a tree of n leaves.  Its chunk tops go to
+// tops_at[0 ..], their count to *ntops_at.  LEAF_GATHER: perm holds indices
+// into the point records that are qoff above the tree's own point indices.
 template <typename T, int LEAF, int C>
-__global__ __launch_bounds__(C) void assemble_kernel(int n, const uint64_t *__restrict__ k,
-                                                     LeafSrc<T> src, TreeOut<T> o,
-                                                     int32_t *__restrict__ tops,
-                                                     uint32_t *__restrict__ chunk_tops) {
+__device__ __forceinline__ void assemble_chunk(int n, const uint64_t *__restrict__ k,
+                                               const LeafSrc<T> &src, const TreeOut<T> &o, int a,
+                                               int32_t *__restrict__ tops_at,
+                                               uint32_t *__restrict__ ntops_at, uint32_t qoff) {
     static_assert(C <= 1024, "arrival words hold chunk offsets in 16 bits");
     __shared__ T spt[3][C];
     __shared__ T sbox[6][C];
@@ -591,9 +614,6 @@ __global__ __launch_bounds__(C) void assemble_kernel(int n, const uint64_t *__re
     __shared__ uint32_t sarr[C];
     __shared__ int8_t sdel[C + 1];  // sdel[j] = split_delta(a - 1 + j), in [-1, 96]
     __shared__ uint32_t s_ntop;
-    // XCD-ordered chunks: the gather's reads of one cluster go through one L2.
-    const int chunk = xcd_block();
-    const int a = chunk * C;
     const int t = threadIdx.x;
     const int i = a + t;
     const int off = n - 1;
@@ -601,9 +621,10 @@ __global__ __launch_bounds__(C) void assemble_kernel(int n, const uint64_t *__re
     if (i < n) {
         const int leaf = off + i;
         if constexpr (LEAF == LEAF_GATHER) {
-            const uint32_t q = src.perm[i];
+            const uint32_t qg = src.perm[i];
+            const uint32_t q = qg - qoff;
             if (src.pts) {
-                const Point4<T> p4 = src.pts[q];
+                const Point4<T> p4 = src.pts[qg];
                 px = p4.v[0]; py = p4.v[1]; pz = p4.v[2];
             } else {
                 px = src.x[q]; py = src.y[q]; pz = src.z[q];
@@ -664,7 +685,7 @@ __global__ __launch_bounds__(C) void assemble_kernel(int n, const uint64_t *__re
             }
             const int p = up_right ? r : l - 1;
             if (p < a || p > a + C - 2) {
-                tops[a + (int)atomicAdd(&s_ntop, 1u)] = lab;
+                tops_at[(int)atomicAdd(&s_ntop, 1u)] = lab;
                 break;
             }
             const uint32_t far = (uint32_t)((up_right ? l : r) - a + 1);
@@ -700,7 +721,7 @@ __global__ __launch_bounds__(C) void assemble_kernel(int n, const uint64_t *__re
                 const int f = (int)(v & 0xFFFFu) - 1 + a;
                 const int lab = (f <= p) ? ((f == p) ? off + p : p)
                                          : ((f == p + 1) ? off + p + 1 : p + 1);
-                tops[a + (int)atomicAdd(&s_ntop, 1u)] = lab;
+                tops_at[(int)atomicAdd(&s_ntop, 1u)] = lab;
             }
         }
     }
@@ -717,7 +738,19 @@ __global__ __launch_bounds__(C) void assemble_kernel(int n, const uint64_t *__re
     if (i < off && spar_in[t] != NO_PARENT) o.parent[i] = spar_in[t];
     if (i < n && spar_leaf[t] != NO_PARENT) o.parent[off + i] = spar_leaf[t];
     __syncthreads();
-    if (t == 0 && a < n) chunk_tops[chunk] = s_ntop;
+    if (t == 0 && a < n) *ntops_at = s_ntop;
+}
+
+// Pass 1 of one tree: chunk c = leaves [c C, c C + C), XCD-ordered so that the
+// gather's reads of one cluster go through one L2.
+template <typename T, int LEAF, int C>
+__global__ __launch_bounds__(C) void assemble_kernel(int n, const uint64_t *__restrict__ k,
+                                                     LeafSrc<T> src, TreeOut<T> o,
+                                                     int32_t *__restrict__ tops,
+                                                     uint32_t *__restrict__ chunk_tops) {
+    const int chunk = xcd_block();
+    assemble_chunk<T, LEAF, C>(n, k, src, o, chunk * C, tops + (size_t)chunk * C,
+                               chunk_tops + chunk, 0u);
 }
 
 // Pass 2: the chunk tops climb through the nodes that span chunks.  Arrival
@@ -730,23 +763,15 @@ __global__ __launch_bounds__(C) void assemble_kernel(int n, const uint64_t *__re
 // after the swap returned -- the agent-scope release/acquire pair without the
 // L2 write-back / invalidate that only non-atomic accesses need.  Pass 1's
 // writes are visible after the kernel boundary.
+// The tops of one chunk (tops_at[0 .. count)) climb, one lane each.
 template <typename T>
-__global__ __launch_bounds__(256) void assemble_top_kernel(int n, const uint64_t *__restrict__ k,
-                                                           TreeOut<T> o,
-                                                           const int32_t *__restrict__ tops,
-                                                           const uint32_t *__restrict__ chunk_tops,
-                                                           int chunks, int chunk_size,
-                                                           uint64_t *__restrict__ words,
-                                                           const uint32_t *__restrict__ epoch) {
-    const uint64_t ep = (uint64_t)*epoch << 32;
+__device__ __forceinline__ void climb_tops(int n, const uint64_t *__restrict__ k,
+                                           const TreeOut<T> &o, const int32_t *__restrict__ tops_at,
+                                           int count, uint64_t *__restrict__ words, uint64_t ep,
+                                           int lane) {
     const int off = n - 1;
-    const int lane = threadIdx.x & 63;
-    const int waves = gridDim.x * (blockDim.x >> 6);
-    // one wave per chunk, one lane per chunk top; XCD x takes a contiguous run
-    // of chunks, so the first climbs between neighbouring chunks stay on it
-    for (int c = xcd_block() * (blockDim.x >> 6) + (threadIdx.x >> 6); c < chunks; c += waves)
-    for (int q = lane; q < (int)chunk_tops[c]; q += 64) {
-        const int lab0 = tops[c * chunk_size + q];
+    for (int q = lane; q < count; q += 64) {
+        const int lab0 = tops_at[q];
         if (lab0 < 0 || lab0 > 2 * off) continue;  // never true for a list this build wrote
         int l, r;
         if (lab0 >= off) {
@@ -787,6 +812,23 @@ __global__ __launch_bounds__(256) void assemble_top_kernel(int n, const uint64_t
             if (root) o.parent[0] = -1;
         }
     }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void assemble_top_kernel(int n, const uint64_t *__restrict__ k,
+                                                           TreeOut<T> o,
+                                                           const int32_t *__restrict__ tops,
+                                                           const uint32_t *__restrict__ chunk_tops,
+                                                           int chunks, int chunk_size,
+                                                           uint64_t *__restrict__ words,
+                                                           const uint32_t *__restrict__ epoch) {
+    const uint64_t ep = (uint64_t)*epoch << 32;
+    const int lane = threadIdx.x & 63;
+    const int waves = gridDim.x * (blockDim.x >> 6);
+    // one wave per chunk, one lane per chunk top; XCD x takes a contiguous run
+    // of chunks, so the first climbs between neighbouring chunks stay on it
+    for (int c = xcd_block() * (blockDim.x >> 6) + (threadIdx.x >> 6); c < chunks; c += waves)
+        climb_tops<T>(n, k, o, tops + (size_t)c * chunk_size, (int)chunk_tops[c], words, ep, lane);
 }
 
 // Starts a topology pass: advances the arrival words' epoch.
@@ -1130,4 +1172,250 @@ template int build_tree_t<uint64_t>(hipStream_t, ft_tree *, Scratch &, const uin
                                     const uint64_t *, const uint64_t *, const uint64_t *,
                                     const uint32_t *, const int8_t *, const int32_t *);
 
+// ===========================================================================
+// Segmented build (the batched step): every tree of a batch in one launch of
+// each kernel.  Tree t's points are indices [off, off + n) of the combined
+// numbering; its sort tiles and assembly chunks are its own (tables built on
+// the host), its digit histograms and bases are its own (the bases include
+// off, so the sort writes each tree's keys into its own range), and the
+// look-back of a tile stops at its tree's first tile.  Keys, permutation,
+// topology and boxes are those of build_bvh on the tree alone: the sort is
+// the same stable sort per tree, and the assembly the same per chunk.
+// ===========================================================================
+template <typename T>
+__global__ __launch_bounds__(256) void seg_bbox_kernel(const SegTree<T> *__restrict__ tr,
+                                                       const int4 *__restrict__ tiles,
+                                                       T *__restrict__ partials) {
+    const int4 ti = tiles[blockIdx.x];
+    const SegTree<T> st = tr[ti.z];
+    const int i0 = ti.x - st.off, i1 = ti.y - st.off;
+    T v[6];
+    {
+        const T f = st.x[i0], g = st.y[i0], h = st.z[i0];
+        v[0] = f; v[1] = f; v[2] = g; v[3] = g; v[4] = h; v[5] = h;
+    }
+    for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
+        const T a = st.x[i], b = st.y[i], c = st.z[i];
+        v[0] = tmin(v[0], a); v[1] = tmax(v[1], a);
+        v[2] = tmin(v[2], b); v[3] = tmax(v[3], b);
+        v[4] = tmin(v[4], c); v[5] = tmax(v[5], c);
+    }
+    __shared__ T red[4][6];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int a = 0; a < 6; ++a) {
+        const T r = (a & 1) ? wave_max(v[a]) : wave_min(v[a]);
+        if (lane == 0) red[w][a] = r;
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int a = threadIdx.x;
+        T r = red[0][a];
+        for (int q = 1; q < 4; ++q) r = (a & 1) ? tmax(r, red[q][a]) : tmin(r, red[q][a]);
+        partials[(size_t)blockIdx.x * 6 + a] = r;
+    }
+}
+
+// One block per tree: its tiles' partials -> its box record (as bbox_final_kernel).
+template <typename T>
+__global__ __launch_bounds__(64) void seg_bbox_final_kernel(const SegTree<T> *__restrict__ tr,
+                                                            const T *__restrict__ partials,
+                                                            BboxDev<T> *__restrict__ out) {
+    const SegTree<T> st = tr[blockIdx.x];
+    if (threadIdx.x != 0 || st.n <= 0) return;
+    T v[6];
+    for (int a = 0; a < 6; ++a) v[a] = partials[(size_t)st.tile0 * 6 + a];
+    for (int b = 1; b < st.ntiles; ++b)
+        for (int a = 0; a < 6; ++a) {
+            const T pv = partials[(size_t)(st.tile0 + b) * 6 + a];
+            v[a] = (a & 1) ? tmax(v[a], pv) : tmin(v[a], pv);
+        }
+    BboxDev<T> &o = out[blockIdx.x];
+    for (int a = 0; a < 3; ++a) {
+        o.lo[a] = v[2 * a];
+        o.hi[a] = v[2 * a + 1];
+        const T d = v[2 * a + 1] - v[2 * a];
+        o.inv[a] = (d == T(0)) ? T(0) : (T(1) / d);
+    }
+}
+
+// One block per tile: keys, point records and the tile's tree's digit histograms.
+template <typename T>
+__global__ __launch_bounds__(256) void seg_encode_kernel(const SegTree<T> *__restrict__ tr,
+                                                         const int4 *__restrict__ tiles, int B,
+                                                         int sfc, const BboxDev<T> *__restrict__ bb,
+                                                         uint64_t *__restrict__ keys, int passes,
+                                                         int rbits, uint32_t *__restrict__ hist,
+                                                         Point4<T> *__restrict__ pts) {
+    __shared__ uint16_t tab[384];
+    __shared__ uint32_t h[ENC_HIST_WORDS];
+    for (int i = threadIdx.x; i < 384; i += blockDim.x) tab[i] = c_ph_table[i];
+    for (int i = threadIdx.x; i < ENC_HIST_WORDS; i += blockDim.x) h[i] = 0;
+    __syncthreads();
+    const int4 ti = tiles[blockIdx.x];
+    const SegTree<T> st = tr[ti.z];
+    T lo[3] = {T(0), T(0), T(0)}, inv[3] = {T(0), T(0), T(0)};
+    if constexpr (CoordTraits<T>::fp) {
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = bb[ti.z].lo[a];
+            inv[a] = bb[ti.z].inv[a];
+        }
+    }
+    for (int g = ti.x + (int)threadIdx.x; g < ti.y; g += blockDim.x) {
+        const int i = g - st.off;
+        const T p[3] = {st.x[i], st.y[i], st.z[i]};
+        pts[g] = Point4<T>{{p[0], p[1], p[2], T(0)}};
+        const uint64_t key = encode_point<T>(p, B, sfc, lo, inv, tab);
+        keys[g] = key;
+        hist_add_key(h, key, passes, rbits);
+    }
+    __syncthreads();
+    uint32_t *ht = hist + (size_t)ti.z * RS_HIST_WORDS;
+    for (int i = threadIdx.x; i < (passes << rbits); i += blockDim.x) {
+        const uint32_t v = h[i];
+        if (v) atomicAdd(&ht[i], v);
+    }
+}
+
+// One block per tree: digit bases = the tree's first index + the exclusive
+// scan of its histogram, per pass.  Block 0 also resets the tile counters
+// and advances the sort epoch (as radix_base_kernel).
+template <typename T>
+__global__ __launch_bounds__(RS_MAX_RADIX) void seg_radix_base_kernel(
+    const SegTree<T> *__restrict__ tr, const uint32_t *__restrict__ hist, int passes, int rbits,
+    uint32_t *__restrict__ base, uint32_t *__restrict__ tile_ctr, uint32_t *__restrict__ epoch) {
+    __shared__ uint32_t tmp[RS_MAX_RADIX / 64];
+    const int t = blockIdx.x;
+    const uint32_t off = (uint32_t)tr[t].off;
+    for (int p = 0; p < passes; ++p) {
+        const size_t w = (size_t)t * RS_HIST_WORDS + ((size_t)p << rbits) + threadIdx.x;
+        base[w] = off + block_excl_scan(hist[w], tmp);
+    }
+    if (t == 0) {
+        if (threadIdx.x < RS_MAX_PASSES) tile_ctr[threadIdx.x] = 0;
+        if (threadIdx.x == 0) {
+            const uint32_t e = *epoch + 1u;
+            *epoch = (e >= (1u << 27)) ? 1u : e;
+        }
+    }
+}
+
+// Pass 1 of every tree: chunk (tree, first leaf) from the chunk table.  Each
+// chunk also copies its sorted keys into its tree's sorted_keys.
+template <typename T, int C>
+__global__ __launch_bounds__(C) void seg_assemble_kernel(const SegTree<T> *__restrict__ tr,
+                                                         const int2 *__restrict__ chunks,
+                                                         int nchunks,
+                                                         const uint64_t *__restrict__ sorted,
+                                                         const uint32_t *__restrict__ perm,
+                                                         const Point4<T> *__restrict__ pts,
+                                                         int32_t *__restrict__ tops,
+                                                         uint32_t *__restrict__ chunk_tops) {
+    const int chunk = xcd_block();
+    if (chunk >= nchunks) return;  // padding block of the XCD-ordered grid
+    const int2 ci = chunks[chunk];
+    const SegTree<T> st = tr[ci.x];
+    const uint64_t *k = sorted + st.off;
+    const int a = ci.y;
+    if (a + (int)threadIdx.x < st.n) st.sorted_keys[a + threadIdx.x] = k[a + threadIdx.x];
+    LeafSrc<T> src{st.x, st.y, st.z, nullptr, nullptr, nullptr, perm + st.off, pts};
+    assemble_chunk<T, LEAF_GATHER, C>(st.n, k, src, st.o, a, tops + (size_t)chunk * C,
+                                      chunk_tops + chunk, (uint32_t)st.off);
+}
+
+// Pass 2 of every tree: each wave takes chunks and climbs their tops within
+// their trees; the arrival words of tree t are words[off ..].
+template <typename T>
+__global__ __launch_bounds__(256) void seg_top_kernel(const SegTree<T> *__restrict__ tr,
+                                                      const int2 *__restrict__ chunks,
+                                                      int nchunks,
+                                                      const uint64_t *__restrict__ sorted,
+                                                      const int32_t *__restrict__ tops,
+                                                      const uint32_t *__restrict__ chunk_tops,
+                                                      uint64_t *__restrict__ words,
+                                                      const uint32_t *__restrict__ epoch) {
+    const uint64_t ep = (uint64_t)*epoch << 32;
+    const int lane = threadIdx.x & 63;
+    const int waves = gridDim.x * (blockDim.x >> 6);
+    for (int c = xcd_block() * (blockDim.x >> 6) + (threadIdx.x >> 6); c < nchunks; c += waves) {
+        const int cnt = (int)chunk_tops[c];
+        if (cnt == 0) continue;
+        const SegTree<T> st = tr[chunks[c].x];
+        climb_tops<T>(st.n, sorted + st.off, st.o, tops + (size_t)c * ASM_CHUNK, cnt,
+                      words + st.off, ep, lane);
+    }
+}
+
+template <typename T>
+int build_segmented_t(hipStream_t s, Scratch &S, const SegBuild &g, const ft_config &cfg) {
+    if (g.ntiles == 0) return FT_OK;
+    // tops are indexed chunk * ASM_CHUNK: at most total + ntrees * ASM_CHUNK entries
+    FT_TRY(scratch_ensure(S, g.total + (size_t)g.ntrees * ASM_CHUNK));
+    if ((size_t)g.ntiles > S.lookback_words / RS_MAX_RADIX)
+        return fail(FT_EINVAL, "segmented build: too many sort tiles for the scratch");
+    const SegTree<T> *tr = (const SegTree<T> *)g.trees;
+    const int key_bits = 3 * cfg.bits_per_dim;
+    const int rbits = radix_width(S, key_bits);
+    const int passes = radix_passes(key_bits, rbits);
+    if (passes > RS_MAX_PASSES) return fail(FT_EINVAL, "radix sort: too many passes");
+    BboxDev<T> *bb = (BboxDev<T> *)g.bbox;
+    if constexpr (CoordTraits<T>::fp) {
+        hipLaunchKernelGGL(seg_bbox_kernel<T>, dim3(g.ntiles), dim3(256), 0, s, tr, g.tiles,
+                           (T *)g.partials);
+        hipLaunchKernelGGL(seg_bbox_final_kernel<T>, dim3(g.ntrees), dim3(64), 0, s, tr,
+                           (const T *)g.partials, bb);
+        FT_CHECK_LAUNCH();
+    }
+    const size_t hw = (size_t)g.ntrees * RS_HIST_WORDS;
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(grid_for(hw, 256)), dim3(256), 0, s, g.hist, 0u, hw);
+    hipLaunchKernelGGL(seg_encode_kernel<T>, dim3(g.ntiles), dim3(256), 0, s, tr, g.tiles,
+                       cfg.bits_per_dim, cfg.sfc, (const BboxDev<T> *)bb, S.keys_in, passes, rbits,
+                       g.hist, (Point4<T> *)S.points);
+    uint32_t *base = g.hist + hw;
+    hipLaunchKernelGGL(seg_radix_base_kernel<T>, dim3(g.ntrees), dim3(1 << rbits), 0, s, tr,
+                       (const uint32_t *)g.hist, passes, rbits, base, S.tile_ctr, S.epoch);
+    FT_CHECK_LAUNCH();
+    // the sweeps: the last one writes the sorted keys over keys_in, which only
+    // the first reads
+    const uint64_t *kin = S.keys_in;
+    const uint32_t *vin = nullptr;
+    for (int p = 0; p < passes; ++p) {
+        const bool last = p == passes - 1;
+        uint64_t *kout = last ? S.keys_in : ((p & 1) ? S.keys_b : S.keys_a);
+        uint32_t *vout = last ? S.perm : ((p & 1) ? S.vals_b : S.vals_a);
+        const uint32_t *pb = base + ((size_t)p << rbits);
+#define FT_SEG_SWEEP(F, BT)                                                                        \
+    hipLaunchKernelGGL((onesweep_kernel<F, BT, true>), dim3(g.ntiles), dim3(RS_BLOCK), 0, s, kin, \
+                       vin, kout, vout, (int)g.total, rbits * p, pb, S.lookback,                  \
+                       (const uint32_t *)S.epoch, p, S.tile_ctr + p, g.tiles)
+        if (rbits == 9) {
+            if (p == 0) FT_SEG_SWEEP(true, 9);
+            else FT_SEG_SWEEP(false, 9);
+        } else {
+            if (p == 0) FT_SEG_SWEEP(true, 8);
+            else FT_SEG_SWEEP(false, 8);
+        }
+#undef FT_SEG_SWEEP
+        kin = kout;
+        vin = vout;
+    }
+    FT_CHECK_LAUNCH();
+    hipLaunchKernelGGL(topo_begin_kernel, dim3(1), dim3(64), 0, s, S.topo_epoch);
+    hipLaunchKernelGGL((seg_assemble_kernel<T, ASM_CHUNK>), dim3(xcd_grid(g.nchunks)),
+                       dim3(ASM_CHUNK), 0, s, tr, g.chunks, g.nchunks,
+                       (const uint64_t *)S.keys_in, (const uint32_t *)S.perm,
+                       (const Point4<T> *)S.points, S.tops, S.chunk_tops);
+    hipLaunchKernelGGL(seg_top_kernel<T>, dim3(xcd_grid((g.nchunks + 3) / 4)), dim3(256), 0, s, tr,
+                       g.chunks, g.nchunks, (const uint64_t *)S.keys_in, (const int32_t *)S.tops,
+                       (const uint32_t *)S.chunk_tops, S.split_words,
+                       (const uint32_t *)S.topo_epoch);
+    FT_CHECK_LAUNCH();
+    return FT_OK;
+}
+template int build_segmented_t<float>(hipStream_t, Scratch &, const SegBuild &, const ft_config &);
+template int build_segmented_t<double>(hipStream_t, Scratch &, const SegBuild &, const ft_config &);
+template int build_segmented_t<uint32_t>(hipStream_t, Scratch &, const SegBuild &, const ft_config &);
+template int build_segmented_t<uint64_t>(hipStream_t, Scratch &, const SegBuild &, const ft_config &);
+
 }  // namespace ft
+

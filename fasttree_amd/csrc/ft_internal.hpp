@@ -249,6 +249,49 @@ template <typename T> inline DTree<T> make_dtree(const ft_tree &t) {
     return d;
 }
 
+// Output arrays of a tree build.
+template <typename T> struct TreeOut {
+    T *mnx, *mxx, *mny, *mxy, *mnz, *mxz;
+    int32_t *left, *right, *parent, *rfirst, *rlast;
+    uint32_t *id;
+    int8_t *ghost;
+    int32_t *orig;
+};
+template <typename T> inline TreeOut<T> make_tree_out(const ft_tree &t) {
+    return TreeOut<T>{(T *)t.min_x,   (T *)t.max_x,   (T *)t.min_y,  (T *)t.max_y,
+                      (T *)t.min_z,   (T *)t.max_z,   t.left_child,  t.right_child,
+                      t.parent,       t.range_first,  t.range_last,  t.id,
+                      t.is_ghost,     t.orig_idx};
+}
+
+// ---------------------------------------------------------------------------
+// Segmented build: the trees of a batch built together, one launch of each
+// build kernel for all of them.  Their points are numbered one after another
+// (tree t owns indices [off, off + n), all below 2^31); sort tiles
+// (SEG_TILE keys) and assembly chunks (SEG_CHUNK leaves) never cross a tree.
+// ---------------------------------------------------------------------------
+template <typename T> struct SegTree {
+    const T *x, *y, *z;     // the tree's input points
+    TreeOut<T> o;           // its TreeSoA arrays
+    uint64_t *sorted_keys;  // its sorted_keys array
+    int32_t off, n;         // first point index, points
+    int32_t tile0, ntiles;  // its sort tiles (tile table entries)
+};
+constexpr int SEG_TILE = 6144;   // keys per sort tile
+constexpr int SEG_CHUNK = 512;   // leaves per assembly chunk
+constexpr int SEG_HIST_WORDS = 8 * 512;  // digit histogram words per tree (8 passes of 9 bits)
+// Device tables and buffers of one segmented build (owned by the caller).
+struct SegBuild {
+    const void *trees;   // SegTree<T>[ntrees]
+    const int4 *tiles;   // per sort tile: first key, end, tree, first tile of its tree
+    const int2 *chunks;  // per assembly chunk: tree, first leaf within the tree
+    int ntrees, ntiles, nchunks;
+    size_t total;        // points of all trees
+    uint32_t *hist;      // 2 * ntrees * SEG_HIST_WORDS: digit histograms, then digit bases
+    void *bbox;          // FP coordinates: one box record (lo, hi, inverse extents) per tree
+    void *partials;      // FP coordinates: 6 values per tile
+};
+
 // ---------------------------------------------------------------------------
 // Host staging (ensure_device_readable / copy_back_and_free, src/hlbvh.hpp:125-200).
 // ---------------------------------------------------------------------------
@@ -318,6 +361,11 @@ struct Scratch {
     double stage_ms[8] = {};
     int n_stages = 0;
 };
+
+// Builds every tree of g (all of one configuration) in one launch of each
+// build kernel; each tree comes out as build_bvh builds it alone.
+template <typename T>
+int build_segmented_t(hipStream_t s, Scratch &S, const SegBuild &g, const ft_config &cfg);
 
 }  // namespace ft
 

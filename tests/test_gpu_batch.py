@@ -250,3 +250,67 @@ def test_batch_pinned_tiers_every_size_class(ft, orc, ct):
     res.zero()
     b.launch()
     check(reqs, exp, res, dst, "tuned with a pinned group")
+
+
+SEG_CASES = [
+    ("u32-morton", O.CT_U32, 0, 21, 0),
+    ("u64-ph-periodic", O.CT_U64, 1, 21, 1),
+    ("f64-ph", O.CT_F64, 1, 21, 0),
+    ("f32-morton10", O.CT_F32, 0, 10, 0),
+]
+
+
+@pytest.mark.parametrize("case", SEG_CASES, ids=[c[0] for c in SEG_CASES])
+def test_segmented_build_equals_build_bvh(ft, orc, case):
+    """The batch's segmented build (every tree in one launch of each build
+    kernel: per-tree sort tiles, digit bases and look-back chains, per-tree
+    assembly chunks) gives every tree exactly the arrays build_bvh gives it
+    alone: keys, permutation, topology, boxes, leaf ranges -- for trees of 0,
+    1 and 2 points, at and around the 512-leaf chunk and 6,144-key tile
+    boundaries, and large ones."""
+    from fasttree_amd.batch import Batch, plan_requests
+    name, ct, sfc, bits, periodic = case
+    dt = O.COORD_DTYPES[ct]
+    fp = ct in (O.CT_F32, O.CT_F64)
+    cfg = ft.Config(sfc=sfc, bits_per_dim=bits, periodic=periodic)
+    sizes = [0, 1, 2, 511, 512, 513, 6143, 6144, 6145, 20000, 130000, 3]
+    trees, parts, hosts = [], [], []
+    for i, n in enumerate(sizes):
+        if n >= 64:
+            x, y, z = synth.nfw_clustered(n, 900 + i, box=100.0, n_halos=max(2, n // 3000))
+        else:
+            x, y, z = synth.uniform(max(n, 1), 900 + i, 100.0)
+            x, y, z = x[:n], y[:n], z[:n]
+        pts = tuple(a.astype(dt) if fp else synth.to_fixed(a, 100.0, bits, dt) for a in (x, y, z))
+        if n and not fp:  # duplicate keys: equal points
+            for a in pts:
+                a[n // 2] = a[0]
+        hosts.append(pts)
+        trees.append(ft.TreeSoA(max(n, 1), ct, cfg))
+        parts.append(tuple(ft.DeviceArray.from_host(a) for a in pts))
+    reqs, total_q, total_r = plan_requests([5 if n else 0 for n in sizes], [4] * len(sizes),
+                                           list(range(len(sizes))))
+    rng = np.random.default_rng(1)
+    q = tuple((rng.uniform(0, 100.0, total_q) if fp else
+               rng.integers(0, 1 << bits, total_q)).astype(dt) for _ in range(3))
+    dq = [ft.DeviceArray.from_host(a) for a in q]
+    res = ft.DeviceArray(total_r, np.uint64)
+    dst = ft.DeviceArray(total_r, ft.dist_dtype(ct))
+    b = Batch(ct)
+    b.set_trees(trees, parts)
+    b.set_queries(reqs, *dq, res, dst)
+    b.run()
+    assert b.tuning()["build"] == "segmented"
+    for i, n in enumerate(sizes):
+        if n == 0:
+            continue
+        ref = ft.TreeSoA(n, ct, cfg)
+        ft.build_bvh(ft.particles(*(ft.DeviceArray.from_host(a) for a in hosts[i])), ref)
+        got, want = trees[i].to_host(), ref.to_host()
+        leaves = slice(n - 1, 2 * n - 1)
+        for key in want:
+            g, w = got[key], want[key]
+            if key in ("id", "is_ghost", "orig_idx"):
+                g, w = g[leaves], w[leaves]
+            assert np.array_equal(g, w), f"{name} tree {i} (n={n}): {key}"
+    check(reqs, expected(ft, trees, reqs, q), res, dst, f"segmented {name}")
