@@ -320,7 +320,7 @@ __global__ __launch_bounds__(64) void batch_wave_kernel(BatchView<T> b,
             const int local = uni((int)b.order[q.entry_first + e] - q.entry_first);
             const int64_t qi = q.query_first + local;
             const T px = b.qx[qi], py = b.qy[qi], pz = b.qz[qi];
-            const int anchor = descend_rank<T, MK>(t, px, py, pz);
+            const int anchor = descend_rank_v<T, MK>(t, px, py, pz);
             WaveList<S> L;
             L.init(q.k);
             if (wave_walk<T, S, MK>(t, px, py, pz, -1, anchor, L, c)) {

@@ -333,7 +333,7 @@ __global__ __launch_bounds__(256) void wave_knn_kernel(DTree<T> t, QSrc<T> qs, i
         int rank = -1;
         load_query(t, qs, q, px, py, pz, rank);
         const int self = excl ? rank : -1;
-        const int anchor = rank >= 0 ? rank : descend_rank<T, MK>(t, px, py, pz);
+        const int anchor = rank >= 0 ? rank : descend_rank_v<T, MK>(t, px, py, pz);
         WaveList<S> L;
         L.init(k);
         if (wave_walk<T, S, MK>(t, px, py, pz, self, anchor, L, c)) {
@@ -1220,15 +1220,15 @@ static int knn_dispatch(hipStream_t s, const ft_tree *tree, const QSrc<T> &qs, i
         return FT_OK;
     }
     // Canonical mode.  Packet tiers: registers for k <= 32, passes of 32 up
-    // to k = 128.  Larger k, and external queries above k = 64, take the
+    // to k = 128.  Larger k, and external queries above k = 32, take the
     // wave-per-query tier (at 10M clustered points, 2.5M self queries at
-    // k = 256: 94.5 ms against 1,100 ms for the per-lane LDS heaps; 1M
-    // external queries at k = 128: 34.9 against 59.8 ms for the passes, at
-    // k = 64 28.7 against 29.5 ms), above k = 1024 the per-lane heap walk.
+    // k = 256: 76 ms against 1,100 ms for the per-lane LDS heaps; 1M
+    // external queries at k = 64 / 128: 16.9 / 22.1 ms against 29.7 / 59.8 ms
+    // for the passes), above k = 1024 the per-lane heap walk.
     const bool packet_ok = k <= 128;  // above 32: passes of 32
     if (algo == FT_ALGO_AUTO) {
         if (k > WAVE_MAX_K) algo = FT_ALGO_GROUPED;
-        else if (k > 128 || (external && k > 64)) algo = FT_ALGO_WAVE;
+        else if (k > 128 || (external && k > 32)) algo = FT_ALGO_WAVE;
         else algo = FT_ALGO_PACKET;
     }
     if (o.seed_bound && packet_ok) algo = FT_ALGO_PACKET;

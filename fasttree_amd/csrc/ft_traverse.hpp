@@ -1085,6 +1085,22 @@ __device__ inline int descend_rank(const DTree<T> &t, T cx, T cy, T cz) {
     return v - off;
 }
 
+// descend_rank with both children's boxes tested in one vector pass (even
+// lanes the left child, odd lanes the right): for walks that run one query
+// per wavefront, where the scalar unit is the scarcer issue slot.
+template <typename T, int MK>
+__device__ inline int descend_rank_v(const DTree<T> &t, T cx, T cy, T cz) {
+    const int off = t.n - 1;
+    const int lane = threadIdx.x & 63;
+    int v = 0;
+    while (v < off) {
+        const int l = uni(t.left[v]), r = uni(t.right[v]);
+        const auto d = node_d2_k<T, MK>(t, cx, cy, cz, (lane & 1) ? r : l);
+        v = uni(rl(d, 0) <= rl(d, 1) ? l : r);
+    }
+    return v - off;
+}
+
 // First leaf rank of the 64-leaf run centred on rank rc, inside [0, n).
 __device__ inline int run_start(int rc, int n) {
     int r0 = rc - 32 < 0 ? 0 : rc - 32;
@@ -1460,8 +1476,13 @@ __device__ __forceinline__ bool wave_walk(const DTree<T> &t, T px, T py, T pz, i
         if (node >= off) {
             const int r = node - off;
             if (r != self && (r < w0 || r >= w1)) {
-                const D d2 = leaf_d2_k<T, MK>(t.m, px, py, pz, t.mnx[node], t.mny[node], t.mnz[node]);
-                const uint64_t cd = dist_bits(d2);
+                // on the vector unit (the node index copied into a VGPR): the
+                // scalar unit, shared by the CU's four SIMDs, is this walk's
+                // scarcer issue slot
+                int nv;
+                asm("v_mov_b32 %0, %1" : "=v"(nv) : "s"(node));
+                const D d2v = leaf_d2_k<T, MK>(t.m, px, py, pz, t.mnx[nv], t.mny[nv], t.mnz[nv]);
+                const uint64_t cd = dist_bits(rl(d2v, 0));
                 const uint32_t ci = (uint32_t)ret_index(t, node);
                 ++c.leaves;
                 if (L.admits(cd, ci)) {
@@ -1478,8 +1499,12 @@ __device__ __forceinline__ bool wave_walk(const DTree<T> &t, T px, T py, T pz, i
                 scan(rf, rl_ - rf + 1, true);
             } else {
                 const int l = uni(t.left[node]), r = uni(t.right[node]);
-                const uint64_t dl = dist_bits(node_d2_k<T, MK>(t, px, py, pz, l));
-                const uint64_t dr = dist_bits(node_d2_k<T, MK>(t, px, py, pz, r));
+                // both children's box distances in one vector pass (even
+                // lanes the left child, odd lanes the right) instead of two
+                // wave-uniform computations on the scalar unit
+                const int ch = (lane & 1) ? r : l;
+                const D dch = node_d2_k<T, MK>(t, px, py, pz, ch);
+                const uint64_t dl = dist_bits(rl(dch, 0)), dr = dist_bits(rl(dch, 1));
                 const bool lfirst = dl <= dr;
                 const int nn = uni(lfirst ? l : r), fn = uni(lfirst ? r : l);
                 const uint64_t dn = lfirst ? dl : dr, df = lfirst ? dr : dl;
