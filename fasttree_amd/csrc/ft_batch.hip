@@ -306,6 +306,7 @@ __global__ __launch_bounds__(64) void batch_wave_kernel(BatchView<T> b,
                                                         const uint2 *__restrict__ items,
                                                         const uint32_t *__restrict__ bnd,
                                                         uint32_t *__restrict__ ctr) {
+    __shared__ WaveBuf wbuf;
     const int lane = threadIdx.x;
     WaveCounters c;
     while (true) {
@@ -323,7 +324,7 @@ __global__ __launch_bounds__(64) void batch_wave_kernel(BatchView<T> b,
             const int anchor = descend_rank_v<T, MK>(t, px, py, pz);
             WaveList<S> L;
             L.init(q.k);
-            if (wave_walk<T, S, MK>(t, px, py, pz, -1, anchor, L, c)) {
+            if (wave_walk<T, S, MK>(t, px, py, pz, -1, anchor, L, c, &wbuf)) {
                 if (lane == 0) {
                     const uint32_t s2 = atomicAdd(b.rerun_count, 1u);
                     if (s2 < b.rerun_cap) b.rerun[s2] = q.entry_first + e;

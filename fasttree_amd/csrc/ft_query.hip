@@ -324,6 +324,7 @@ __global__ __launch_bounds__(256) void wave_knn_kernel(DTree<T> t, QSrc<T> qs, i
                                                        uint64_t *__restrict__ res,
                                                        typename CoordTraits<T>::D *__restrict__ dists,
                                                        uint64_t *__restrict__ stats) {
+    __shared__ WaveBuf wbuf[4];
     const int lane = threadIdx.x & 63;
     const int slot = uni(xcd_block() * 4 + (int)(threadIdx.x >> 6));
     WaveCounters c;
@@ -336,7 +337,7 @@ __global__ __launch_bounds__(256) void wave_knn_kernel(DTree<T> t, QSrc<T> qs, i
         const int anchor = rank >= 0 ? rank : descend_rank_v<T, MK>(t, px, py, pz);
         WaveList<S> L;
         L.init(k);
-        if (wave_walk<T, S, MK>(t, px, py, pz, self, anchor, L, c)) {
+        if (wave_walk<T, S, MK>(t, px, py, pz, self, anchor, L, c, &wbuf[threadIdx.x >> 6])) {
             if (lane == 0) qm.redo[slot] = 1;
         } else {
             L.store(k, res, dists, (size_t)q, lane);

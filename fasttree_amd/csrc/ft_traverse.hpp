@@ -1299,8 +1299,9 @@ constexpr int WAVE_MAX_K = 1024;
 template <int S> struct WaveList {
     static constexpr int C = 64 * S;
     // Candidates of one leaf batch inserted one at a time up to this count;
-    // more take the sort-and-merge path.
-    static constexpr int kSparse = S == 1 ? 12 : 16;
+    // more take the sort-and-merge path (at 10M clustered points, external
+    // k = 128: 22.0 ms with 16 for every S, 21.4 ms with these).
+    static constexpr int kSparse = S == 1 ? 8 : (S == 2 ? 10 : 12);
     uint64_t d[S];
     uint32_t i[S];
     uint64_t wd;  // the k-th entry (wave-uniform): pruning and admission bound
@@ -1405,21 +1406,89 @@ struct WaveCounters {
     uint64_t nodes = 0, leaves = 0, inserts = 0;
 };
 
+// A wave's candidate buffer in LDS: leaves below the list's k-th entry wait
+// here, unsorted, and join the list 64 at a time through merge_batch.
+struct WaveBuf {
+    uint64_t d[64];
+    uint32_t i[64];
+};
+
 // The wave's walk for one query (px, py, pz): self = leaf rank excluded from
 // the results (-1: none); anchor = a leaf rank near the query (-1: none).
-// The 64 (S + 1) leaves around the anchor are inserted first, so the walk
+// The 64 (S + 2) leaves around the anchor are inserted first, so the walk
 // starts with a finite bound; it then skips them (and any subtree inside
 // them).  Returns true if the stack would overflow (never for a tree over
 // distinct (64-bit key, 32-bit index) strings, whose depth is below 97).
 template <typename T, int S, int MK>
 __device__ __forceinline__ bool wave_walk(const DTree<T> &t, T px, T py, T pz, int self,
-                                          int anchor, WaveList<S> &L, WaveCounters &c) {
+                                          int anchor, WaveList<S> &L, WaveCounters &c,
+                                          WaveBuf *buf) {
     using D = typename CoordTraits<T>::D;
     const int lane = threadIdx.x & 63;
     const int n = t.n;
     const int off = n - 1;
     int w0 = 0, w1 = 0;
-    // Tests leaves [r0, r0 + cnt) (cnt <= 64), one per lane, and inserts the
+    // Lists of 256 entries or more (S >= 4) buffer their candidates (the
+    // merge is cheap per candidate and the list's bound lags by at most 64 of
+    // its k entries: at 10M clustered points, 2.5M self queries at k = 256
+    // took 60.5 ms buffered, 72.1 ms inserting at once); shorter lists insert
+    // at once (buffered, k = 64 external queries took 16.6 against 15.8 ms).
+    constexpr bool BUFFERED = S >= 4;
+    int nbuf = 0;  // candidates waiting in buf (wave-uniform)
+    // The buffered candidates join the list (one merge of up to 64).
+    auto flush = [&]() {
+        if (!BUFFERED || nbuf == 0) return;
+        const bool has = lane < nbuf;
+        const uint64_t bd = has ? buf->d[lane] : ~0ull;
+        const uint32_t bi = has ? buf->i[lane] : ~0u;
+        L.merge_batch(bd, bi, lane);
+        L.refresh();
+        nbuf = 0;
+    };
+    // Appends the lanes' candidates that the list admits (flushing first when
+    // they would not fit), or, for short lists, inserts them at once: a few
+    // one at a time, more by one sort-and-merge.
+    auto offer = [&](bool ok, uint64_t cd, uint32_t ci) {
+        uint64_t m = __ballot(ok);
+        if (!m) return;
+        int cnt_in = __popcll(m);
+        if constexpr (!BUFFERED) {
+            c.inserts += (uint64_t)cnt_in;
+            if (cnt_in <= WaveList<S>::kSparse) {
+                uint64_t mm = m;
+                while (mm) {
+                    const int j = __builtin_ctzll(mm);
+                    mm &= mm - 1;
+                    const uint64_t xd = rl(cd, j);
+                    const uint32_t xi = rl32(ci, j);
+                    if (L.admits(xd, xi)) {
+                        L.insert_one(xd, xi);
+                        L.refresh();
+                    }
+                }
+            } else {
+                L.merge_batch(ok ? cd : ~0ull, ok ? ci : ~0u, lane);
+                L.refresh();
+            }
+            return;
+        }
+        if (nbuf + cnt_in > 64) {
+            flush();
+            ok = ok && L.admits(cd, ci);  // against the merged list's k-th entry
+            m = __ballot(ok);
+            cnt_in = __popcll(m);
+            if (!m) return;
+        }
+        c.inserts += (uint64_t)cnt_in;
+        const int pos = nbuf + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                              __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+        if (ok) {
+            buf->d[pos] = cd;
+            buf->i[pos] = ci;
+        }
+        nbuf += cnt_in;
+    };
+    // Tests leaves [r0, r0 + cnt) (cnt <= 64), one per lane, and offers the
     // ones the list admits.
     auto scan = [&](int r0, int cnt, bool skip_win) {
         const int r = r0 + lane;
@@ -1427,44 +1496,25 @@ __device__ __forceinline__ bool wave_walk(const DTree<T> &t, T px, T py, T pz, i
         if (skip_win) ok = ok && (r < w0 || r >= w1);
         const int leaf = off + (lane < cnt ? r : r0);
         const D d2 = leaf_d2_k<T, MK>(t.m, px, py, pz, t.mnx[leaf], t.mny[leaf], t.mnz[leaf]);
-        uint64_t cd = dist_bits(d2);
-        uint32_t ci = (uint32_t)ret_index(t, leaf);
+        const uint64_t cd = dist_bits(d2);
+        const uint32_t ci = (uint32_t)ret_index(t, leaf);
         ok = ok && L.admits(cd, ci);
         c.leaves += (uint64_t)cnt;
-        const uint64_t m = __ballot(ok);
-        if (!m) return;
-        const int cnt_in = __popcll(m);
-        c.inserts += (uint64_t)cnt_in;
-        if (cnt_in <= WaveList<S>::kSparse) {
-            uint64_t mm = m;
-            while (mm) {
-                const int j = __builtin_ctzll(mm);
-                mm &= mm - 1;
-                const uint64_t xd = rl(cd, j);
-                const uint32_t xi = rl32(ci, j);
-                if (L.admits(xd, xi)) {
-                    L.insert_one(xd, xi);
-                    L.refresh();
-                }
-            }
-        } else {
-            cd = ok ? cd : ~0ull;
-            ci = ok ? ci : ~0u;
-            L.merge_batch(cd, ci, lane);
-            L.refresh();
-        }
+        offer(ok, cd, ci);
     };
-    const int nwin = 64 * (S + 1);
+    const int nwin = 64 * (S + 2);
     if (anchor >= 0) {
         w0 = anchor - nwin / 2;
         if (w0 > n - nwin) w0 = n - nwin;
         if (w0 < 0) w0 = 0;
         w1 = w0 + nwin < n ? w0 + nwin : n;
         for (int r0 = w0; r0 < w1; r0 += 64) scan(r0, w1 - r0 < 64 ? w1 - r0 : 64, false);
+        flush();
         if (w0 == 0 && w1 == n) return false;
     }
     if (n <= 64) {
         scan(0, n, true);
+        flush();
         return false;
     }
     // stack entry e: lane e & 63 of register (e >> 6)
@@ -1485,11 +1535,7 @@ __device__ __forceinline__ bool wave_walk(const DTree<T> &t, T px, T py, T pz, i
                 const uint64_t cd = dist_bits(rl(d2v, 0));
                 const uint32_t ci = (uint32_t)ret_index(t, node);
                 ++c.leaves;
-                if (L.admits(cd, ci)) {
-                    ++c.inserts;
-                    L.insert_one(cd, ci);
-                    L.refresh();
-                }
+                offer(lane == 0 && L.admits(cd, ci), cd, ci);
             }
         } else {
             const int rf = uni(t.rfirst[node]), rl_ = uni(t.rlast[node]);
@@ -1550,6 +1596,7 @@ __device__ __forceinline__ bool wave_walk(const DTree<T> &t, T px, T py, T pz, i
         }
         if (!found) break;
     }
+    flush();
     return false;
 }
 
