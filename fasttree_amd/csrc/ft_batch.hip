@@ -425,6 +425,7 @@ struct GroupTune {
     bool pinned = false;  // set by the caller (ft_batch_set_tier): the tuner keeps it
 };
 constexpr int WAVE_ITEM = 8;  // wave tiers: queries per item
+constexpr int QUERY_CHAINS = 4;  // parallel chains of tier kernels in the query phase
 
 }  // namespace ft
 
@@ -459,6 +460,7 @@ struct ft_batch {
     int32_t *drerun = nullptr;
     uint64_t *dkeys = nullptr, *dsorted = nullptr;
     int item_first[ft::V_COUNT] = {}, item_count[ft::V_COUNT] = {};
+    double var_cost[ft::V_COUNT] = {};  // estimated cost of each tier's item list
     int lane_width = 64;
     int mk = 0;  // metric kind shared by every tree (MK_ANY when they differ)
     ft::Scratch sort_scratch;
@@ -684,6 +686,7 @@ template <typename T> static int upload_tables(ft_batch *b) {
         // FT_XCDS contiguous chunks of about equal measured cost
         double total = 0;
         for (auto &p : L) total += p.first;
+        b->var_cost[v] = total;
         uint32_t *bv = &bnd[(size_t)v * (FT_XCDS + 1)];
         double run = 0;
         int c = 1;
@@ -940,18 +943,26 @@ template <typename T> static int enqueue_queries(ft_batch *b, hipStream_t s) {
     hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(128), 0, s, b->dctr,
                        V_COUNT * FT_XCDS + 1);
     FT_CHECK_LAUNCH();
+    // Tier kernels, most expensive first, on at most QUERY_CHAINS parallel
+    // chains (the hardware queues a process gets by default): each kernel
+    // goes to the chain with the least estimated work so far, so the largest
+    // starts at once instead of queueing behind a small one.
     std::vector<int> used;
     for (int v = 0; v < V_COUNT; ++v)
         if (b->item_count[v] > 0) used.push_back(v);
-    const int nb = (int)used.size();
+    std::stable_sort(used.begin(), used.end(),
+                     [b](int x, int y) { return b->var_cost[x] > b->var_cost[y]; });
+    const int nb = std::min((int)used.size(), QUERY_CHAINS);
     if (nb > 1) {
         FT_TRY(ensure_streams(b, nb));
         FT_HIP(hipEventRecord(b->ev_fork, s));
         for (int i = 0; i < nb; ++i) FT_HIP(hipStreamWaitEvent(b->side[i], b->ev_fork, 0));
     }
-    for (int i = 0; i < nb; ++i) {
-        const int v = used[i];
-        FT_TRY(launch_variant<T>(b, nb > 1 ? b->side[i] : s, v, b->ditems + b->item_first[v],
+    std::vector<double> load(std::max(nb, 1), 0.0);
+    for (int v : used) {
+        const int ch = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        load[ch] += b->var_cost[v];
+        FT_TRY(launch_variant<T>(b, nb > 1 ? b->side[ch] : s, v, b->ditems + b->item_first[v],
                                  b->item_count[v], b->dbnd + v * (FT_XCDS + 1),
                                  b->dctr + v * FT_XCDS));
     }
