@@ -61,6 +61,10 @@ def main():
     ap.add_argument("--pmc-probe", action="store_true",
                     help="one build, one self-kNN and one 1 GiB copy kernel, then exit (for "
                          "hardware counter runs); prints no JSON")
+    ap.add_argument("--wave-probe", action="store_true",
+                    help="one build and 1M external k=128 queries through the wave-per-query "
+                         "tier (half near the particles, half uniform in the box), twice, "
+                         "then exit (for hardware counter runs); prints no JSON")
     ap.add_argument("--batched-only", action="store_true",
                     help="run only the batched multi-tree workload (development)")
     ap.add_argument("--no-batched", action="store_true",
@@ -114,6 +118,33 @@ def main():
         ev[2].record(stream)
         stream.synchronize()
         return ev[0].elapsed_ms(ev[1]), ev[1].elapsed_ms(ev[2])
+
+    if args.wave_probe:
+        ft.build_bvh(p, tree, scratch=scratch, stream=stream.handle)
+        rng = np.random.default_rng(3)
+        nq, kw = 1_000_000, 128
+        sel = rng.integers(0, n, nq)
+        mask = np.uint64((1 << 21) - 1)
+        qs = []
+        for a in (x, y, z):
+            v = (a[sel] + rng.integers(0, 64, nq).astype(np.uint64)) & mask
+            v[nq // 2:] = rng.integers(0, 1 << 21, nq - nq // 2).astype(np.uint64)
+            qs.append(ft.DeviceArray.from_host(v))
+        rw, dw = ft.DeviceArray(nq * kw, np.uint64), ft.DeviceArray(nq * kw, np.uint64)
+        stw = ft.DeviceArray(8, np.uint64).zero()
+        # the first call sizes the query-order buffers and counts the work;
+        # the second (timed) runs without counters
+        ft.knn_query(tree, *qs, kw, results=rw, dists=dw, algorithm=ft.ALGO_WAVE,
+                     stream=stream.handle, stats=stw)
+        ev[0].record(stream)
+        ft.knn_query(tree, *qs, kw, results=rw, dists=dw, algorithm=ft.ALGO_WAVE,
+                     stream=stream.handle)
+        ev[1].record(stream)
+        stream.synchronize()
+        sv = stw.to_host() / nq
+        log(f"wave probe: 1M external k=128 in {ev[0].elapsed_ms(ev[1]):.2f} ms; per query "
+            f"{sv[0]:.1f} nodes, {sv[1]:.1f} leaves, {sv[4]:.1f} candidates merged")
+        return
 
     if args.pmc_probe:
         b, q = step()

@@ -343,9 +343,20 @@ __global__ __launch_bounds__(256) void wave_knn_kernel(DTree<T> t, QSrc<T> qs, i
             L.store(k, res, dists, (size_t)q, lane);
         }
     }
-    wave_add_stats(stats, 0, lane == 0 ? c.nodes : 0);
-    wave_add_stats(stats, 1, lane == 0 ? c.leaves : 0);
-    wave_add_stats(stats, 4, lane == 0 ? c.inserts : 0);
+    if (stats) {  // one atomic per block and counter: a wave per query would contend
+        __shared__ unsigned long long s_st[3];
+        if (threadIdx.x < 3) s_st[threadIdx.x] = 0;
+        __syncthreads();
+        if (lane == 0) {
+            atomicAdd(&s_st[0], (unsigned long long)c.nodes);
+            atomicAdd(&s_st[1], (unsigned long long)c.leaves);
+            atomicAdd(&s_st[2], (unsigned long long)c.inserts);
+        }
+        __syncthreads();
+        if (threadIdx.x < 3 && s_st[threadIdx.x])
+            atomicAdd((unsigned long long *)&stats[threadIdx.x == 2 ? 4 : threadIdx.x],
+                      s_st[threadIdx.x]);
+    }
 }
 
 template <typename T, int M, bool HEAP, bool SEEDED, bool PACKED, int MK, bool MULTI = false>
