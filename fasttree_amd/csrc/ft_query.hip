@@ -1232,15 +1232,19 @@ static int knn_dispatch(hipStream_t s, const ft_tree *tree, const QSrc<T> &qs, i
         return FT_OK;
     }
     // Canonical mode.  Packet tiers: registers for k <= 32, passes of 32 up
-    // to k = 128.  Larger k, and external queries above k = 32, take the
-    // wave-per-query tier (at 10M clustered points, 2.5M self queries at
-    // k = 256: 76 ms against 1,100 ms for the per-lane LDS heaps; 1M
-    // external queries at k = 64 / 128: 16.9 / 22.1 ms against 29.7 / 59.8 ms
-    // for the passes), above k = 1024 the per-lane heap walk.
+    // to k = 128.  Above k = 32 the wave-per-query tier, except for self
+    // queries over all leaves of integer trees, whose packets share their
+    // walks best (10M clustered int64 points: self k = 64 / 128 73.7 / 158.8 ms
+    // in passes against 116.8 / 166.3 ms; a 20 % subset k = 64 / 128 30.6 /
+    // 65.0 against 24.7 / 34.7 ms; 1M external queries k = 64 / 128 29.7 /
+    // 59.8 against 16.7 / 21.2 ms; 20M clustered doubles: self k = 64 290 ms
+    // against 226 ms, a 20 % subset k = 64 115 against 48 ms).  Above
+    // k = 1024, the per-lane heap walk.
     const bool packet_ok = k <= 128;  // above 32: passes of 32
     if (algo == FT_ALGO_AUTO) {
+        const bool int_self = !CoordTraits<T>::fp && !external && !qs.leaf_ids;
         if (k > WAVE_MAX_K) algo = FT_ALGO_GROUPED;
-        else if (k > 128 || (external && k > 32)) algo = FT_ALGO_WAVE;
+        else if (k > 128 || (k > 32 && !int_self)) algo = FT_ALGO_WAVE;
         else algo = FT_ALGO_PACKET;
     }
     if (o.seed_bound && packet_ok) algo = FT_ALGO_PACKET;
