@@ -286,6 +286,7 @@ template <typename T> __device__ inline int32_t ret_index(const DTree<T> &t, int
 template <typename D, int M> struct TopK {
     static constexpr bool kPacked = false;
     static constexpr bool kSortFill = false;
+    static constexpr int kRoundsPerTrip = 2;
     D d[M];
     int32_t i[M];
     __device__ inline void init(int k, D bound) {
@@ -479,6 +480,8 @@ __device__ __forceinline__ uint64_t key_u64(double k) { return (uint64_t)__doubl
 template <int M> struct TopKPacked {
     static constexpr bool kPacked = true;
     static constexpr bool kSortFill = true;
+    // insertion rounds per trip of the leaf loop (measured: two from 32 slots)
+    static constexpr int kRoundsPerTrip = M >= 32 ? 2 : 1;
     static constexpr uint64_t BIAS = 1ull << 62;
     static constexpr uint64_t NOKEY = BIAS | (1ull << 61);  // above every key, still finite
     uint64_t key[M];
@@ -501,29 +504,40 @@ template <int M> struct TopKPacked {
     // Sorted insertion, in place from the top slot down: with the slots
     // sorted, the new content of slot s is the median of key[s-1], c and
     // key[s], i.e. max(key[s-1], min(c, key[s])) -- key[s-1] when c is below
-    // it, c when c falls between, key[s] when c is not below it.  Slots s >= L
-    // only, when no taking lane's key is below key[L-1] (slots below L then
-    // keep their keys, and slot L becomes min(c, key[L])).  A lane that takes
-    // nothing inserts NOKEY, which changes no slot.
-    template <int L> __device__ inline void chain_from(double c) {
+    // it, c when c falls between, key[s] when c is not below it.  Slot 0 has
+    // no key below it and becomes min(c, key[0]).  A lane that takes nothing
+    // inserts NOKEY, which changes no slot.
+    //
+    // The chain runs in segments from the top: slots [A, B) with the median
+    // rule.  When no taking lane's key is below key[A-1], the slots below A
+    // keep their keys, and the median rule at slot A already gave
+    // min(c, key[A]) (c >= key[A-1] and key[A] >= key[A-1]), so the chain can
+    // stop there.
+    template <int A, int B> __device__ inline void chain_seg(uint64_t c) {
 #pragma unroll
-        for (int s = M - 1; s >= L + 1; --s)
-            key[s] = key_u64(f64_max(key_f64(key[s - 1]), f64_min(c, key_f64(key[s]))));
-        key[L] = key_u64(f64_min(c, key_f64(key[L])));
+        for (int s = B - 1; s >= A; --s) {
+            if (s == 0) key[0] = key_u64(f64_min(key_f64(c), key_f64(key[0])));
+            else key[s] = key_u64(f64_max(key_f64(key[s - 1]), f64_min(key_f64(c), key_f64(key[s]))));
+        }
     }
     __device__ inline bool beats_worst_key(uint64_t kb) const { return kb < key[M - 1]; }
     __device__ inline void insert(bool take, uint64_t d, int32_t i) { insert_key(take, make(d, i)); }
     __device__ inline void insert_key(bool take, uint64_t kb) {
         const uint64_t c = take ? kb : NOKEY;
-        const double cd = key_f64(c);
         // New keys mostly land near the top: run only the part of the chain a
         // taking lane can reach (one compare and a wave vote per level).
         if constexpr (M >= 16) {
-            if (!__any(c < key[7 * M / 8 - 1])) { chain_from<7 * M / 8>(cd); return; }
-            if (!__any(c < key[3 * M / 4 - 1])) { chain_from<3 * M / 4>(cd); return; }
-            if (!__any(c < key[M / 2 - 1])) { chain_from<M / 2>(cd); return; }
+            chain_seg<7 * M / 8, M>(c);
+            if (__any(c < key[7 * M / 8 - 1])) {
+                chain_seg<3 * M / 4, 7 * M / 8>(c);
+                if (__any(c < key[3 * M / 4 - 1])) {
+                    chain_seg<M / 2, 3 * M / 4>(c);
+                    if (__any(c < key[M / 2 - 1])) chain_seg<0, M / 2>(c);
+                }
+            }
+        } else {
+            chain_seg<0, M>(c);
         }
-        chain_from<0>(cd);
     }
     // Bitonic sort of the M slots, ascending, in registers (each
     // compare-exchange is one v_min_f64 and one v_max_f64).
@@ -841,11 +855,14 @@ __device__ __forceinline__ void packet_walk(const DTree<T> &t, int bucket, bool 
             }
             c.inserts += take ? 1 : 0;
         };
-        // Two rounds per trip: the loop-carried top-k is copied once per trip.
-        while (__any(cand != 0)) {
-            round();
-            if (!__any(cand != 0)) break;
-            round();
+        if constexpr (H::kRoundsPerTrip == 2) {
+            while (__any(cand != 0)) {
+                round();
+                if (!__any(cand != 0)) break;
+                round();
+            }
+        } else {
+            while (__any(cand != 0)) round();
         }
     };
 
@@ -991,6 +1008,7 @@ __device__ __forceinline__ int fill_from_run(const DTree<T> &t, H &h, int k, boo
 template <int KCAP> struct TopKHeapLds {
     static constexpr bool kPacked = true;
     static constexpr bool kSortFill = false;
+    static constexpr int kRoundsPerTrip = 2;
     using Floor = uint64_t;  // (single pass only)
     __device__ static inline Floor no_floor() { return 0ull; }
     uint64_t *key;  // [KCAP][64], owned by the kernel
