@@ -409,13 +409,20 @@ __global__ __launch_bounds__(RS_BLOCK) void onesweep_kernel(
         const int idx = base + j * 64 + lane;
         const bool ok = idx < tile_end;
         const uint32_t d = (uint32_t)(k[j] >> shift) & MASK;
-        uint64_t peers = __ballot(ok);
+        // Per digit bit: bs = all ones when the bit is set, m = the lanes with
+        // the bit set; a lane keeps the peers whose bit agrees with its own,
+        // peers & ~(m ^ bs), one v_bitop3_b32 per mask half (truth table 0x90
+        // for src0 & ~(src1 ^ src2)).
+        const uint64_t okm = __ballot(ok);
+        uint32_t plo = (uint32_t)okm, phi = (uint32_t)(okm >> 32);
 #pragma unroll
         for (int b = 0; b < BITS; ++b) {
-            const bool bit = (d >> b) & 1u;
-            const uint64_t m = __ballot(bit);
-            peers &= bit ? m : ~m;
+            const int32_t bs = __builtin_amdgcn_sbfe((int32_t)d, b, 1);
+            const uint64_t m = __ballot(bs < 0);
+            plo = __builtin_amdgcn_bitop3_b32(plo, (uint32_t)m, (uint32_t)bs, 0x90);
+            phi = __builtin_amdgcn_bitop3_b32(phi, (uint32_t)(m >> 32), (uint32_t)bs, 0x90);
         }
+        const uint64_t peers = ((uint64_t)phi << 32) | plo;
         const uint32_t before = s_wave[wave][d];
         const uint32_t below = (uint32_t)__popcll(peers & lt);
         rank[j] = before + below;
