@@ -332,6 +332,7 @@ def main():
             "algorithmic_bytes": knn_bytes,
             "traffic_source": TRAFFIC_PROFILE if traffic is not None else None}
     build_gbs = build_bytes / (build_ms * 1e-3) / 1e9
+    issue = valu_issue(query_ms)
 
     cpu = None
     if not args.no_cpu:
@@ -374,6 +375,7 @@ def main():
                       "insert_rounds_per_wave": float(st[5]) / ((n + 63) // 64),
                       "chains_per_wave": float(st[6]) / ((n + 63) // 64)},
         "roofline": roof,
+        "valu_issue": issue,
         "cpu_baseline": cpu,
         "large_k": large_k,
         "batched": batched,
@@ -443,6 +445,30 @@ def large_k_bench(ft, tree, x, y, z, n, stream, ev):
 
 
 TRAFFIC_PROFILE = "profiles/c2_hbm_traffic_counters.md"
+SQ_PROFILE = "profiles/c2_sq_instruction_counters.md"
+# MI355X: 256 CUs x 4 SIMDs, 2.4 GHz peak engine clock; a wave64 VALU
+# instruction occupies its SIMD for 4 cycles (FP64 included: full-rate FP64).
+SIMDS, CLOCK_HZ, CYCLES_PER_VALU = 1024, 2.4e9, 4
+
+
+def valu_issue(kernel_ms):
+    """The self-kNN kernel against its VALU-issue bound: the committed SQ counter
+    summary's VALU instructions per wave times waves, at 4 cycles each spread
+    over every SIMD, against the kernel's measured time; or None."""
+    import re
+    try:
+        text = open(os.path.join(ROOT, SQ_PROFILE)).read()
+    except OSError:
+        return None
+    m = re.search(r"Self-kNN VALU: ([0-9,]+) instructions per wave over ([0-9,]+) waves", text)
+    if not m:
+        return None
+    per_wave, waves = (int(g.replace(",", "")) for g in m.groups())
+    issue_ms = per_wave * waves * CYCLES_PER_VALU / (SIMDS * CLOCK_HZ) * 1e3
+    return {"valu_per_wave": per_wave, "waves": waves, "issue_ms": round(issue_ms, 3),
+            "kernel_ms": round(kernel_ms, 3), "frac": round(issue_ms / kernel_ms, 4),
+            "model": "VALU instructions x 4 cycles / (1024 SIMDs x 2.4 GHz)",
+            "source": SQ_PROFILE}
 
 
 def measured_traffic():
