@@ -37,7 +37,7 @@ int radix_sort_u64(hipStream_t s, Scratch &S, const uint64_t *keys_in, size_t n,
 // ===========================================================================
 struct QueryMap {
     const uint32_t *order;      // slot -> query (null: identity)
-    const int32_t *list;        // LISTED: slots to run
+    const int32_t *list;        // LISTED: slots to run (-1: already answered)
     const uint32_t *list_count; // LISTED: number of list entries (device)
     uint8_t *redo;              // direct passes: per-slot flag, set when the slot
                                 // must be re-run exactly
@@ -71,8 +71,9 @@ __global__ __launch_bounds__(LANE_BLOCK) void lane_knn_reg_kernel(
     for (int base = (LISTED ? (int)blockIdx.x : xcd_block()) * LANE_BLOCK; base < nlist;
          base += (LISTED ? gridDim.x * LANE_BLOCK : nlist)) {
         const int pos = base + lane;
-        const bool active = pos < nlist;
-        const int slot = active ? (LISTED ? qm.list[pos] : pos) : 0;
+        const int lslot = (LISTED && pos < nlist) ? qm.list[pos] : 0;
+        const bool active = pos < nlist && (!LISTED || lslot >= 0);
+        const int slot = active ? (LISTED ? lslot : pos) : 0;
         const int q = active ? slot_query(qm, slot) : 0;
         T px = T(0), py = T(0), pz = T(0);
         int self = -1;
@@ -133,8 +134,9 @@ __global__ __launch_bounds__(LANE_BLOCK) void lane_knn_lds_kernel(
     for (int base = (LISTED ? (int)blockIdx.x : xcd_block()) * lanes; base < nlist;
          base += (LISTED ? gridDim.x * lanes : nlist)) {
         const int pos = base + lane;
-        const bool active = lane_ok && pos < nlist;
-        const int slot = active ? (LISTED ? qm.list[pos] : pos) : 0;
+        const int lslot = (LISTED && lane_ok && pos < nlist) ? qm.list[pos] : 0;
+        const bool active = lane_ok && pos < nlist && (!LISTED || lslot >= 0);
+        const int slot = active ? (LISTED ? lslot : pos) : 0;
         const int q = active ? slot_query(qm, slot) : 0;
         T px = T(0), py = T(0), pz = T(0);
         int self = -1;
@@ -296,7 +298,7 @@ __global__ __launch_bounds__(64) void merge_knn_kernel(DTree<T> t, QSrc<T> qs, i
 // SEEDED: each lane's k-th distance starts at its seed bound; a lane that ends
 // with fewer than k neighbours inside the seed flags its slot for an exact
 // re-run.  PACKED: 64-bit (d2, index) keys; queries the packed form cannot
-// order exactly are flagged too.  The re-run is the per-lane walk over the
+// order exactly are flagged too.  The re-run (launch_exact_listed) takes the
 // compacted list of flagged slots, unseeded and lexicographic, so exact.
 // ===========================================================================
 struct PacketArgs {
@@ -317,18 +319,26 @@ struct PacketArgs {
 // k up to 1024.  Four queries per 256-thread block; self and subset queries
 // anchor their first leaves at their own rank, external ones at the leaf a
 // greedy descent reaches.
+//
+// LISTED: the first step of the exact re-run of a direct pass's flagged
+// slots (qm.list, *qm.list_count).  The waves take list entries in turn, and
+// an entry whose query they answer is set to -1 in done_list (the list
+// itself), so the per-lane re-run that follows takes only the entries whose
+// walk would overflow the wave's stack (none for a tree this library built).
 // ===========================================================================
-template <typename T, int S, int MK>
+template <typename T, int S, int MK, bool LISTED = false>
 __global__ __launch_bounds__(256) void wave_knn_kernel(DTree<T> t, QSrc<T> qs, int nq, int k,
                                                        int excl, QueryMap qm,
                                                        uint64_t *__restrict__ res,
                                                        typename CoordTraits<T>::D *__restrict__ dists,
-                                                       uint64_t *__restrict__ stats) {
+                                                       uint64_t *__restrict__ stats,
+                                                       int32_t *__restrict__ done_list) {
     __shared__ WaveBuf wbuf[4];
     const int lane = threadIdx.x & 63;
-    const int slot = uni(xcd_block() * 4 + (int)(threadIdx.x >> 6));
+    const int wave = uni(xcd_block() * 4 + (int)(threadIdx.x >> 6));
     WaveCounters c;
-    if (slot < nq) {
+    // Answers the query of one slot; false if its walk would overflow.
+    auto answer = [&](int slot) -> bool {
         const int q = uni(slot_query(qm, slot));
         T px = T(0), py = T(0), pz = T(0);
         int rank = -1;
@@ -337,11 +347,17 @@ __global__ __launch_bounds__(256) void wave_knn_kernel(DTree<T> t, QSrc<T> qs, i
         const int anchor = rank >= 0 ? rank : descend_rank_v<T, MK>(t, px, py, pz);
         WaveList<S> L;
         L.init(k);
-        if (wave_walk<T, S, MK>(t, px, py, pz, self, anchor, L, c, &wbuf[threadIdx.x >> 6])) {
-            if (lane == 0) qm.redo[slot] = 1;
-        } else {
-            L.store(k, res, dists, (size_t)q, lane);
-        }
+        if (wave_walk<T, S, MK>(t, px, py, pz, self, anchor, L, c, &wbuf[threadIdx.x >> 6]))
+            return false;
+        L.store(k, res, dists, (size_t)q, lane);
+        return true;
+    };
+    if constexpr (LISTED) {
+        const int cnt = (int)*qm.list_count;
+        for (int e = wave; e < cnt; e += (int)gridDim.x * 4)
+            if (answer(uni(qm.list[e])) && lane == 0) done_list[e] = -1;
+    } else if (wave < nq) {
+        if (!answer(wave) && lane == 0) qm.redo[wave] = 1;
     }
     if (stats) {  // one atomic per block and counter: a wave per query would contend
         __shared__ unsigned long long s_st[3];
@@ -1022,13 +1038,28 @@ static int compact_flags(hipStream_t s, const RerunBuf &g_rerun, int n) {
         else { constexpr int M = 32; __VA_ARGS__; }                 \
     } while (0)
 
-// Exact re-run of the listed slots: per-lane walk with a 128-entry stack,
-// unseeded, lexicographic top-k (registers for k <= 32, LDS heap above).
+// Exact re-run of the listed slots.  Up to k = 128, one wave per listed query
+// first (the wave-per-query tier, LISTED); it marks the entries it answers.
+// Then the per-lane walk with a 128-entry stack, unseeded, lexicographic top-k
+// (registers for k <= 32, LDS heap above), over the entries left.  The flagged
+// queries of a seeded pass are few and scattered (0.2 % of C2's 10M): one
+// lane each left the re-run latency-bound on a few hundred waves, 1.7 ms; one
+// wave each fills the chip.
 template <typename T, int MK>
 static int launch_exact_listed(hipStream_t s, const DTree<T> &dt, const QSrc<T> &qs, int nq,
                                int k, int excl, const uint32_t *order, const RerunBuf &g_rerun,
                                uint64_t *res, typename CoordTraits<T>::D *d) {
     const QueryMap qm{order, g_rerun.list, g_rerun.count, nullptr};
+    if (k <= 128) {
+        const dim3 g(xcd_grid(std::min<unsigned>(blocks_for(nq, 4), 2048u))), b(256);
+        if (k <= 64)
+            hipLaunchKernelGGL((wave_knn_kernel<T, 1, MK, true>), g, b, 0, s, dt, qs, nq, k, excl,
+                               qm, res, d, (uint64_t *)nullptr, g_rerun.list);
+        else
+            hipLaunchKernelGGL((wave_knn_kernel<T, 2, MK, true>), g, b, 0, s, dt, qs, nq, k, excl,
+                               qm, res, d, (uint64_t *)nullptr, g_rerun.list);
+        FT_CHECK_LAUNCH();
+    }
     if (k <= 32) {
         const unsigned g = std::min<unsigned>(blocks_for(nq, LANE_BLOCK), 2048u);
         FT_BY_M(k, hipLaunchKernelGGL((lane_knn_reg_kernel<T, M, true, true, MK>), dim3(g),
@@ -1122,7 +1153,7 @@ static int run_canonical(hipStream_t s, const DTree<T> &dt, const QSrc<T> &qs, i
         const dim3 g(xcd_grid(blocks_for(nq, 4))), b(256);
 #define FT_WAVE_LAUNCH(S_)                                                                      \
     hipLaunchKernelGGL((wave_knn_kernel<T, S_, MK>), g, b, 0, s, dt, qs, nq, k, excl, qm, res, d, \
-                       stats)
+                       stats, (int32_t *)nullptr)
         if (k <= 64) FT_WAVE_LAUNCH(1);
         else if (k <= 128) FT_WAVE_LAUNCH(2);
         else if (k <= 256) FT_WAVE_LAUNCH(4);
