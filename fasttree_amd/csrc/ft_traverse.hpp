@@ -301,14 +301,20 @@ template <typename D, int M> struct TopK {
     __device__ inline bool beats_worst(D cd, int32_t ci) const {
         return (cd < d[M - 1]) | ((cd == d[M - 1]) & (ci < i[M - 1]));
     }
+    // (cd, ci) below the entry in slot s, in (d, index) order.
+    __device__ inline bool below(int s, D cd, int32_t ci) const {
+        return (cd < d[s]) | ((cd == d[s]) & (ci < i[s]));
+    }
     // Predicated insertion as a compare-and-swap chain from the smallest slot:
     // the carried entry swaps into every slot it is below, so it lands in order
     // and the former worst drops off the top.  Every slot stays in its own
-    // registers; the comparisons are branch-free.
-    __device__ inline void insert(bool take, D cd, int32_t ci) {
+    // registers; the comparisons are branch-free.  A candidate not below the
+    // entry in slot L-1 swaps with no slot under L (the slots are sorted), so
+    // the chain can start at L.
+    template <int L> __device__ inline void insert_from(bool take, D cd, int32_t ci) {
 #pragma unroll
-        for (int s = 0; s < M; ++s) {
-            const bool lt = take & ((cd < d[s]) | ((cd == d[s]) & (ci < i[s])));
+        for (int s = L; s < M; ++s) {
+            const bool lt = take & below(s, cd, ci);
             const D td = d[s];
             const int32_t ti = i[s];
             d[s] = lt ? cd : td;
@@ -316,6 +322,17 @@ template <typename D, int M> struct TopK {
             cd = lt ? td : cd;
             ci = lt ? ti : ci;
         }
+    }
+    // New entries mostly land near the top: the chain starts at the lowest of
+    // slots 7M/8, 3M/4, M/2 and 0 that a taking lane of the wave can reach
+    // (a compare and a wave vote per level).
+    __device__ inline void insert(bool take, D cd, int32_t ci) {
+        if constexpr (M >= 16) {
+            if (!__any(take && below(7 * M / 8 - 1, cd, ci))) { insert_from<7 * M / 8>(take, cd, ci); return; }
+            if (!__any(take && below(3 * M / 4 - 1, cd, ci))) { insert_from<3 * M / 4>(take, cd, ci); return; }
+            if (!__any(take && below(M / 2 - 1, cd, ci))) { insert_from<M / 2>(take, cd, ci); return; }
+        }
+        insert_from<0>(take, cd, ci);
     }
     __device__ inline D worst_d() const { return d[M - 1]; }
     // Multi-pass queries: a pass takes only (d, index) at or above the floor.
