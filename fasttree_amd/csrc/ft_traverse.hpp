@@ -696,6 +696,11 @@ __device__ __forceinline__ void packet_walk(const DTree<T> &t, int bucket, bool 
     // Integer periodic metric: pre-shifted 32-bit query (wrap_gap_pre) and the
     // query masked to B bits for the exact inside-the-box tests.
     constexpr bool PRE = !CoordTraits<T>::fp && MK == MK_WRAP;
+    // The leaf filter's operands (pre-shifted integer or float32 coordinates
+    // of leaf j in slot j): one 16-byte LDS broadcast read per leaf serves
+    // every lane, instead of three readlanes on the vector unit.  The walk
+    // runs on one wave, whose LDS operations complete in order.
+    __shared__ uint4 s_leaf[64];
     [[maybe_unused]] const int wsh = PRE ? 32 - t.m.bits : 0;
     [[maybe_unused]] uint32_t qxs = 0, qys = 0, qzs = 0;
     [[maybe_unused]] T qmx = px, qmy = py, qmz = pz;
@@ -778,31 +783,43 @@ __device__ __forceinline__ void packet_walk(const DTree<T> &t, int bucket, bool 
             ex = (uint32_t)bx << wsh;
             ey = (uint32_t)by << wsh;
             ez = (uint32_t)bz << wsh;
+            s_leaf[lane] = make_uint4(ex, ey, ez, 0u);
             const float wf = wrap_filter_bound_pre((uint64_t)w, wsh);
             [[maybe_unused]] const float lf = wrap_filter_floor_pre((uint64_t)lo_d, wsh);
             auto near = [&](int j) -> uint32_t {
-                const float d = wrap_sq_pre(qxs, qys, qzs, rl(ex, j), rl(ey, j), rl(ez, j));
+                const uint4 e = s_leaf[j];
+                const float d = wrap_sq_pre(qxs, qys, qzs, e.x, e.y, e.z);
                 if constexpr (FLOOR) return (d <= wf && d >= lf) ? 1u : 0u;
                 else return d <= wf ? 1u : 0u;
             };
-            // leaf j's bit shifted in from the top, leaves taken last to first
+            // leaf j's bit shifted in from the top, leaves taken last to first;
+            // unrolled so that several broadcast reads are in flight (8 for
+            // lists of 32 slots and more, 4 below: measured)
+            constexpr int UNROLL = H::kRoundsPerTrip == 2 ? 8 : 4;
+#pragma unroll UNROLL
             for (int j = c0 - 1; j >= 0; --j) clo = (clo << 1) | near(j);
+#pragma unroll UNROLL
             for (int j = cnt - 1; j >= 32; --j) chi = (chi << 1) | near(j);
         } else if constexpr (FPF) {
             // float32 pre-filter of double coordinates (fpf_*): never rejects
             // a leaf within w; the insertion below tests every passing leaf
             // exactly.
-            const float fbx = (float)bx, fby = (float)by, fbz = (float)bz;
+            s_leaf[lane] = make_uint4(__float_as_uint((float)bx), __float_as_uint((float)by),
+                                      __float_as_uint((float)bz), 0u);
             const float wf = fpf_upper((double)w, fE);
             [[maybe_unused]] const float lf = FLOOR ? fpf_lower((double)lo_d, fE) : 0.0f;
             constexpr bool WR = MK == MK_WRAP;
             auto near = [&](int j) -> uint32_t {
-                const float d = fpf_sq(fpf_gap(fqx, rl(fbx, j), fL, WR), fpf_gap(fqy, rl(fby, j), fL, WR),
-                                       fpf_gap(fqz, rl(fbz, j), fL, WR));
+                const uint4 e = s_leaf[j];
+                const float d = fpf_sq(fpf_gap(fqx, __uint_as_float(e.x), fL, WR),
+                                       fpf_gap(fqy, __uint_as_float(e.y), fL, WR),
+                                       fpf_gap(fqz, __uint_as_float(e.z), fL, WR));
                 if constexpr (FLOOR) return (d <= wf && d >= lf) ? 1u : 0u;
                 else return d <= wf ? 1u : 0u;
             };
+#pragma unroll 4
             for (int j = c0 - 1; j >= 0; --j) clo = (clo << 1) | near(j);
+#pragma unroll 4
             for (int j = cnt - 1; j >= 32; --j) chi = (chi << 1) | near(j);
         } else {
             auto near = [&](int j) -> bool {
