@@ -674,6 +674,16 @@ struct PacketCounters {
     uint64_t rounds = 0, chains = 0;  // per wave: insertion rounds, and those that ran the chain
 };
 
+// A 32-bit coordinate's bits, and back (LDS staging of leaf coordinates).
+template <typename T> __device__ __forceinline__ uint32_t bits32(T v) {
+    if constexpr (std::is_same<T, float>::value) return __float_as_uint(v);
+    else return (uint32_t)v;
+}
+template <typename T> __device__ __forceinline__ T from_bits32(uint32_t b) {
+    if constexpr (std::is_same<T, float>::value) return __uint_as_float(b);
+    else return (T)b;
+}
+
 template <typename T, typename H, int MK, bool FLOOR = false>
 __device__ __forceinline__ void packet_walk(const DTree<T> &t, int bucket, bool active, T px, T py,
                                             T pz, int self, H &h, int32_t *stk,
@@ -821,6 +831,21 @@ __device__ __forceinline__ void packet_walk(const DTree<T> &t, int bucket, bool 
             for (int j = c0 - 1; j >= 0; --j) clo = (clo << 1) | near(j);
 #pragma unroll 4
             for (int j = cnt - 1; j >= 32; --j) chi = (chi << 1) | near(j);
+        } else if constexpr (sizeof(T) == 4) {
+            // 32-bit coordinates (integer open box, float): exact test, operands
+            // broadcast from LDS as above
+            s_leaf[lane] = make_uint4(bits32(bx), bits32(by), bits32(bz), 0u);
+            auto near = [&](int j) -> bool {
+                const uint4 e = s_leaf[j];
+                const D d = leaf_d2_k<T, MK>(t.m, px, py, pz, from_bits32<T>(e.x),
+                                             from_bits32<T>(e.y), from_bits32<T>(e.z));
+                if constexpr (FLOOR) return d <= w && d >= lo_d;
+                else return d <= w;
+            };
+#pragma unroll 4
+            for (int j = 0; j < c0; ++j) clo |= near(j) ? (1u << j) : 0u;
+#pragma unroll 4
+            for (int j = 32; j < cnt; ++j) chi |= near(j) ? (1u << (j - 32)) : 0u;
         } else {
             auto near = [&](int j) -> bool {
                 const D d = leaf_d2_k<T, MK>(t.m, px, py, pz, rl(bx, j), rl(by, j), rl(bz, j));
