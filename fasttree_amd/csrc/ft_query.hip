@@ -1289,7 +1289,7 @@ static int knn_dispatch(hipStream_t s, const ft_tree *tree, const QSrc<T> &qs, i
         FT_TRY(query_order<T>(s, dt, qs, nq, order));
     // auto: 128 leaves on each side (at 10M clustered points 64 / 128 / 192 /
     // 256 / 384 took 28.7 / 28.55 / 28.5 / 28.6 / 29.0 ms)
-    const int window = o.window < 0 ? 0 : (o.window == 0 ? 128 : std::min(o.window, 1 << 20));
+    const int window = o.window < 0 ? 0 : (o.window == 0 ? 192 : std::min(o.window, 1 << 20));
     // queries per packet: 64 (one per lane) unless 16 or 32 is asked for
     if (o.group_width != 0 && o.group_width != 16 && o.group_width != 32 && o.group_width != 64)
         return fail(FT_EINVAL, "group_width must be 0, 16, 32 or 64");

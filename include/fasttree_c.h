@@ -137,7 +137,7 @@ typedef struct ft_query_opts {
     int32_t window;      /* FT_ALGO_PACKET self queries: leaves scanned on each side of  */
                          /* a packet's own run (subsets: the run around its middle      */
                          /* lane's leaf, widened by up to 4x for sparse subsets) before */
-                         /* the walk (0 = auto: 128, negative = none)                   */
+                         /* the walk (0 = auto: 192, negative = none)                   */
     int32_t reserved2;
 } ft_query_opts;
 
