@@ -340,10 +340,17 @@ __global__ __launch_bounds__(256) void mlp_infer_kernel(const float *__restrict_
     const float ymean = norm[2 * NF], ystd = norm[2 * NF + 1];
     const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
     const int nwaves = gridDim.x * 4;
+    // The next tile's features are loaded while this tile is computed.
+    auto load_tile = [&](int t0) -> float4 {
+        const int s = t0 + lr;
+        return s < n ? ((const float4 *)(feat + (size_t)s * NF))[lg]
+                     : make_float4(0.f, 0.f, 0.f, 0.f);
+    };
+    float4 vnext = load_tile(wave * 16);
     for (int s0 = wave * 16; s0 < n; s0 += nwaves * 16) {
         const int s = s0 + lr;
-        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (s < n) v = ((const float4 *)(feat + (size_t)s * NF))[lg];
+        const float4 v = vnext;
+        vnext = load_tile(s0 + nwaves * 16);
         const float vi[4] = {v.x, v.y, v.z, v.w};
         bf16x4 x;
         for (int i = 0; i < 4; ++i) x[i] = bf16_hw((vi[i] - fm[i]) * fi[i]);
