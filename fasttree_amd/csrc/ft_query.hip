@@ -375,12 +375,13 @@ __global__ __launch_bounds__(256) void wave_knn_kernel(DTree<T> t, QSrc<T> qs, i
     }
 }
 
-// Packed-key single-pass packets: at least 4 waves per SIMD (128 VGPRs; the
-// k = 32 kernel spills 72 bytes and still runs 11 % faster than at 3 waves,
-// 144 VGPRs).  Double coordinates keep their natural 3 waves (forcing 4 spilled
-// 116 bytes and was 32 % slower).
+// Packed-key packets: at least 4 waves per SIMD (128 VGPRs).  The k = 32
+// kernel spills 72 bytes and still runs 11 % faster than at 3 waves (144
+// VGPRs); the passes of 32 for k = 64 run 19 % faster than at 2 waves.  Double
+// coordinates keep their natural 3 waves (forcing 4 spilled 116 bytes and was
+// 32 % slower).
 template <typename T, int M, bool HEAP, bool SEEDED, bool PACKED, int MK, bool MULTI = false>
-__global__ __launch_bounds__(64, (PACKED && !MULTI) ? 4 : 2) void packet_knn_kernel(
+__global__ __launch_bounds__(64, PACKED ? 4 : 2) void packet_knn_kernel(
     DTree<T> t, QSrc<T> qs, PacketArgs a, QueryMap qm,
     const typename CoordTraits<T>::D *__restrict__ seed, uint64_t *__restrict__ res,
     typename CoordTraits<T>::D *__restrict__ dists) {
