@@ -1268,20 +1268,25 @@ static int knn_dispatch(hipStream_t s, const ft_tree *tree, const QSrc<T> &qs, i
         return FT_OK;
     }
     // Canonical mode.  Packet tiers: registers for k <= 32, passes of 32 up
-    // to k = 128.  Above k = 32 the wave-per-query tier, except for self
-    // queries over all leaves of integer trees, whose packets share their
-    // walks best (10M clustered int64 points: self k = 64 / 128 73.7 / 158.8 ms
-    // in passes against 116.8 / 166.3 ms; a 20 % subset k = 64 / 128 30.6 /
-    // 65.0 against 24.7 / 34.7 ms; 1M external queries k = 64 / 128 29.7 /
-    // 59.8 against 16.7 / 21.2 ms; 20M clustered doubles: self k = 64 290 ms
-    // against 226 ms, a 20 % subset k = 64 115 against 48 ms).  Above
-    // k = 1024, the per-lane heap walk.
+    // to k = 128.  Above k = 32 the wave-per-query tier, except where packets
+    // of integer queries share their walks best: self queries over all leaves
+    // up to k = 128, and subsets of at least a fifth of the leaves up to
+    // k = 64.  Measured on 10M clustered int64 points, passes against the
+    // wave tier: self k = 33 / 64 / 128 37.6 / 47.3 / 103.3 against 86.9 /
+    // 106.6 / 153.8 ms; a 20 % subset 15.7 / 19.4 / 41.2 against 18.7 / 22.7 /
+    // 32.3 ms; 1M external queries 14.6 / 19.1 / 40.4 against 13.0 / 15.1 /
+    // 20.1 ms.  20M clustered doubles: self k = 64 290 ms against 226 ms, a
+    // 20 % subset k = 64 111 against 45 ms.  Above k = 1024, the per-lane
+    // heap walk.
     const bool packet_ok = k <= 128;  // above 32: passes of 32
     if (algo == FT_ALGO_AUTO) {
-        const bool int_self = !CoordTraits<T>::fp && !external && !qs.leaf_ids;
+        const bool int_q = !CoordTraits<T>::fp && !external;
+        const bool int_self = int_q && !qs.leaf_ids;
+        const bool int_dense_subset = int_q && qs.leaf_ids && (int64_t)nq * 5 >= (int64_t)dt.n;
         if (k > WAVE_MAX_K) algo = FT_ALGO_GROUPED;
-        else if (k > 128 || (k > 32 && !int_self)) algo = FT_ALGO_WAVE;
-        else algo = FT_ALGO_PACKET;
+        else if (k <= 32 || (int_self && k <= 128) || (int_dense_subset && k <= 64))
+            algo = FT_ALGO_PACKET;
+        else algo = FT_ALGO_WAVE;
     }
     if (o.seed_bound && packet_ok) algo = FT_ALGO_PACKET;
     if (algo == FT_ALGO_PACKET && !packet_ok)
@@ -1292,8 +1297,10 @@ static int knn_dispatch(hipStream_t s, const ft_tree *tree, const QSrc<T> &qs, i
     if (algo == FT_ALGO_WAVE && k > WAVE_MAX_K) algo = FT_ALGO_GROUPED;
     if ((algo == FT_ALGO_PACKET || algo == FT_ALGO_WAVE) && (qs.qx || qs.leaf_ids) && nq > 64)
         FT_TRY(query_order<T>(s, dt, qs, nq, order));
-    // auto: 128 leaves on each side (at 10M clustered points 64 / 128 / 192 /
-    // 256 / 384 took 28.7 / 28.55 / 28.5 / 28.6 / 29.0 ms)
+    // auto: 192 leaves on each side (self k = 32 at 10M clustered points: 64 /
+    // 128 / 192 / 256 / 384 took 19.17 / 18.93 / 18.80 / 18.77 / 18.81 ms;
+    // 192 was also as fast or faster at k = 1 to 64, for subsets and for
+    // external queries)
     const int window = o.window < 0 ? 0 : (o.window == 0 ? 192 : std::min(o.window, 1 << 20));
     // queries per packet: 64 (one per lane) unless 16 or 32 is asked for
     if (o.group_width != 0 && o.group_width != 16 && o.group_width != 32 && o.group_width != 64)
