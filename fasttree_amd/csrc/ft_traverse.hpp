@@ -283,6 +283,18 @@ template <typename T> __device__ inline int32_t ret_index(const DTree<T> &t, int
 // k entries right-aligned so the worst is always slot M-1.  Slots below M-k
 // hold (-inf, -1) and never move.  Insertion is a fixed compare/shift network.
 // ---------------------------------------------------------------------------
+// IEEE max / min of doubles, one VALU instruction each (no NaN reaches them).
+__device__ __forceinline__ double f64_max(double a, double b) {
+    double r;
+    asm("v_max_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+__device__ __forceinline__ double f64_min(double a, double b) {
+    double r;
+    asm("v_min_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+
 template <typename D, int M> struct TopK {
     static constexpr bool kPacked = false;
     static constexpr bool kSortFill = false;
@@ -311,16 +323,35 @@ template <typename D, int M> struct TopK {
     // registers; the comparisons are branch-free.  A candidate not below the
     // entry in slot L-1 swaps with no slot under L (the slots are sorted), so
     // the chain can start at L.
+    //
+    // Double distances run the chain top-down instead: slot s takes the entry
+    // of slot s-1 when the candidate is below it, the candidate when it is
+    // below slot s only, else keeps its entry.  The distance of that choice is
+    // the median max(d[s-1], min(c, d[s])) (ties included: the compares that
+    // pick the index are lexicographic, and the three candidates' distances
+    // are ordered), one v_min_f64 and one v_max_f64; a lane that takes nothing
+    // offers the largest distance, which changes no slot.
     template <int L> __device__ inline void insert_from(bool take, D cd, int32_t ci) {
+        if constexpr (std::is_same<D, double>::value) {
+            const double c = take ? cd : dist_max<D>();
 #pragma unroll
-        for (int s = L; s < M; ++s) {
-            const bool lt = take & below(s, cd, ci);
-            const D td = d[s];
-            const int32_t ti = i[s];
-            d[s] = lt ? cd : td;
-            i[s] = lt ? ci : ti;
-            cd = lt ? td : cd;
-            ci = lt ? ti : ci;
+            for (int s = M - 1; s >= L; --s) {
+                const bool bs = take & below(s, cd, ci);
+                const bool bp = (s > L) && (take & below(s - 1, cd, ci));
+                i[s] = bp ? i[s - 1] : (bs ? ci : i[s]);
+                d[s] = s > 0 ? f64_max(d[s - 1], f64_min(c, d[s])) : f64_min(c, d[s]);
+            }
+        } else {
+#pragma unroll
+            for (int s = L; s < M; ++s) {
+                const bool lt = take & below(s, cd, ci);
+                const D td = d[s];
+                const int32_t ti = i[s];
+                d[s] = lt ? cd : td;
+                i[s] = lt ? ci : ti;
+                cd = lt ? td : cd;
+                ci = lt ? ti : ci;
+            }
         }
     }
     // New entries mostly land near the top: the chain starts at the lowest of
@@ -481,16 +512,6 @@ template <typename D, bool REF> struct LdsHeap {
 // whole 64-bit key) instead of a 64-bit compare and four 32-bit selects.
 // Dead slots (below M - k) hold 0 (+0.0), below every key.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double f64_max(double a, double b) {
-    double r;
-    asm("v_max_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
-    return r;
-}
-__device__ __forceinline__ double f64_min(double a, double b) {
-    double r;
-    asm("v_min_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
-    return r;
-}
 __device__ __forceinline__ double key_f64(uint64_t k) { return __longlong_as_double((long long)k); }
 __device__ __forceinline__ uint64_t key_u64(double k) { return (uint64_t)__double_as_longlong(k); }
 
