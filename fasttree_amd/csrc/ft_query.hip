@@ -1275,9 +1275,10 @@ static int knn_dispatch(hipStream_t s, const ft_tree *tree, const QSrc<T> &qs, i
     // wave tier: self k = 33 / 64 / 128 37.6 / 47.3 / 103.3 against 86.9 /
     // 106.6 / 153.8 ms; a 20 % subset 15.7 / 19.4 / 41.2 against 18.7 / 22.7 /
     // 32.3 ms; 1M external queries 14.6 / 19.1 / 40.4 against 13.0 / 15.1 /
-    // 20.1 ms.  20M clustered doubles: self k = 64 290 ms against 226 ms, a
-    // 20 % subset k = 64 111 against 45 ms.  Above k = 1024, the per-lane
-    // heap walk.
+    // 20.1 ms.  20M clustered doubles: self k = 33 / 64 176 / 249 against
+    // 172 / 213 ms, a 20 % subset k = 64 98 against 45 ms, 1M external
+    // queries k = 64 79 against 15 ms.  Above k = 1024, the per-lane heap
+    // walk.
     const bool packet_ok = k <= 128;  // above 32: passes of 32
     if (algo == FT_ALGO_AUTO) {
         const bool int_q = !CoordTraits<T>::fp && !external;
